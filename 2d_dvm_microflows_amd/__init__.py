@@ -1,0 +1,12 @@
+"""MI355X (gfx950) DVM Boltzmann time step: variance-reduced collisions with
+conservative remap, upwind convection, wall / specular / inflow boundaries and
+FP64 moments, as hand-written HIP kernels behind the C-ABI of include/dvm_hip.h.
+"""
+from . import capi
+from .config import (SolverConfig, homogeneous_relaxation, lid_driven_cavity, maxwellian,
+                     normal_shock_1d, node_velocities, rankine_hugoniot, uniform_axis)
+from .dvm import DvmError, Solver, load_library
+
+__all__ = ["capi", "SolverConfig", "Solver", "DvmError", "load_library", "homogeneous_relaxation",
+           "lid_driven_cavity", "normal_shock_1d", "maxwellian", "node_velocities", "rankine_hugoniot",
+           "uniform_axis"]

@@ -1,0 +1,556 @@
+// dvm_capi.hip -- context management and the extern "C" entry points of
+// include/dvm_hip.h. One translation unit: the kernels are included here.
+#include "dvm_kernels.hip"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+using namespace dvm;
+
+struct dvm_ctx {
+  dvm_config cfg;
+  int ncell = 0, N = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {};
+  // host copy of the grid
+  std::vector<double> ax[3], beta[3];
+  Grid g{};                      // device pointers
+  double* d_axes = nullptr;      // [ax0 ax1 ax2 beta0 beta1 beta2]
+  // resident state
+  double *f = nullptr, *fd = nullptr, *eqv = nullptr, *eqc = nullptr, *neqc = nullptr;
+  int2 *eqtab = nullptr, *neqtab = nullptr;
+  double *eqpar = nullptr, *props = nullptr, *factors = nullptr;
+  int *counts = nullptr, *err = nullptr;
+  // walls
+  double *wallM = nullptr, *ratio = nullptr, *spec = nullptr, *inflow = nullptr;
+  CdfParams cdfp{};
+  CollParams collp{};
+  ConvParams convp{};
+  bool lds_collide = false;
+  size_t coll_lds = 0;
+  bool f_valid = false, props_valid = false;
+  int trace_ready = -1;
+  char msg[512] = {0};
+};
+
+static char g_create_msg[512] = "";
+
+static int fail(dvm_ctx* c, int code, const char* fmt, ...) {
+  char* dst = c ? c->msg : g_create_msg;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(dst, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(c, x)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail((c), DVM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+static int dalloc(dvm_ctx* c, T** p, size_t n) {
+  if (hipMalloc((void**)p, n * sizeof(T) + 16) != hipSuccess)
+    return fail(c, DVM_E_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+  return DVM_OK;
+}
+
+// velocity_grid.f90:653-689 find_zero_point (host side, once per grid)
+static void zero_point(const std::vector<double>& a, int& zero, int& pos, int& neg) {
+  const int mx = (int)a.size() - 1;
+  if (a[0] > 0.0) { zero = -1; pos = 0; neg = -1; return; }
+  for (int i = 1; i <= mx; ++i) {
+    if (a[i] > 0.0) {
+      zero = i - 1; pos = i; neg = i - 1;
+      if (a[i - 1] > -kDoubleTol) neg = i - 2;
+      return;
+    }
+  }
+  zero = mx + 1; pos = mx + 1; neg = mx;
+}
+
+// math_utils.f90:83-135 gamma_calculator (Lanczos coefficients as listed there)
+static double gamma_ref(double x) {
+  static const double a[9] = {0.99999999999980993, 676.5203681218851, -1259.1392167224028,
+                              771.32342877765313, -176.61502916214059, 12.507343278686905,
+                              -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7};
+  if (x < 0.5) return kPiRef / (sin(kPiRef * x) * gamma_ref(1.0 - x));
+  x = x - 1.0;
+  double y = a[0];
+  for (int i = 1; i <= 8; ++i) y = y + a[i] / (x + (double)i);
+  const double z = x + 7.0 + 0.5;
+  return sqrt(2.0 * kPiRef) * pow(z, x + 0.5) * exp(-z) * y;
+}
+
+static double maxwell_host(double x, double y, double z, double coeff, double mass, double u, double v, double w,
+                           double temp) {
+  double cx = (x - u); cx = cx * cx;
+  double cy = (y - v); cy = cy * cy;
+  double cz = (z - w); cz = cz * cz;
+  const double csq = cx + cy + cz;
+  return coeff * exp(-csq * mass / temp);
+}
+
+static int validate(const dvm_config* cfg) {
+  if (!cfg) return fail(nullptr, DVM_E_ARG, "null config");
+  if (cfg->dim < DVM_DIM_0D || cfg->dim > DVM_DIM_2D) return fail(nullptr, DVM_E_ARG, "dim must be 0, 1 or 2");
+  if (cfg->nx < 1 || cfg->ny < 1) return fail(nullptr, DVM_E_ARG, "nx, ny must be >= 1");
+  if (cfg->dim == DVM_DIM_0D && (cfg->nx != 1 || cfg->ny != 1))
+    return fail(nullptr, DVM_E_ARG, "space-homogeneous runs have one cell");
+  if (cfg->dim == DVM_DIM_1D && cfg->ny != 1) return fail(nullptr, DVM_E_ARG, "1-D runs need ny = 1");
+  if (cfg->dim == DVM_DIM_1D && cfg->enable_convection && cfg->nx < 3)
+    return fail(nullptr, DVM_E_ARG, "1-D convection needs nx >= 3");
+  const int n[3] = {cfg->nvx, cfg->nvy, cfg->nvz};
+  const double* a[3] = {cfg->vx, cfg->vy, cfg->vz};
+  for (int d = 0; d < 3; ++d) {
+    if (n[d] < 3 || n[d] > kMaxAxis) return fail(nullptr, DVM_E_ARG, "velocity axis %d needs 3..%d points", d, kMaxAxis);
+    if (!a[d]) return fail(nullptr, DVM_E_ARG, "velocity axis %d is null", d);
+    for (int i = 1; i < n[d]; ++i)
+      if (!(a[d][i] > a[d][i - 1])) return fail(nullptr, DVM_E_ARG, "velocity axis %d is not increasing", d);
+  }
+  if (cfg->num_repl < 1) return fail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
+  if (cfg->coll_solver != DVM_COLL_NONE && cfg->coll_solver != DVM_COLL_VR)
+    return fail(nullptr, DVM_E_ARG, "coll_solver must be 0 (none) or 3 (variance reduction)");
+  if (cfg->precision != DVM_PREC_FP64 && cfg->precision != DVM_PREC_FP32)
+    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
+  if (cfg->sample_mode != DVM_SAMPLES_REFERENCE && cfg->sample_mode != DVM_SAMPLES_EXPLICIT)
+    return fail(nullptr, DVM_E_ARG, "invalid sample_mode");
+  if (cfg->sample_mode == DVM_SAMPLES_REFERENCE && !(cfg->coln_rms > 0.0))
+    return fail(nullptr, DVM_E_ARG, "coln_rms must be > 0 for the reference sample-count rule");
+  if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT && cfg->samples_per_cell < 0)
+    return fail(nullptr, DVM_E_ARG, "samples_per_cell must be >= 0");
+  if (!(cfg->dt > 0.0) || !(cfg->kn > 0.0) || !(cfg->mass > 0.0)) return fail(nullptr, DVM_E_ARG, "dt, kn, mass must be > 0");
+  if (cfg->enable_convection && cfg->dim != DVM_DIM_0D && (!(cfg->dx > 0.0) || (cfg->dim == DVM_DIM_2D && !(cfg->dy > 0.0))))
+    return fail(nullptr, DVM_E_ARG, "dx (and dy in 2-D) must be > 0");
+  if (!(cfg->acc_coeff >= 0.0 && cfg->acc_coeff <= 1.0)) return fail(nullptr, DVM_E_ARG, "acc_coeff must be in [0,1]");
+  return DVM_OK;
+}
+
+extern "C" const char* dvm_last_error(const dvm_ctx* ctx) { return ctx ? ctx->msg : g_create_msg; }
+
+extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
+  if (!c) return;
+  void* ptrs[] = {c->d_axes, c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->eqpar, c->props,
+                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
+  c->cfg = *cfg;
+  c->ncell = cfg->nx * cfg->ny;
+  const int n[3] = {cfg->nvx, cfg->nvy, cfg->nvz};
+  const double* src[3] = {cfg->vx, cfg->vy, cfg->vz};
+  c->N = n[0] * n[1] * n[2];
+  const int N = c->N;
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIP_TRY(c, hipEventCreate(&e));
+  // ---- grid (velocity_grid.f90:271-597)
+  std::vector<double> packed;
+  for (int d = 0; d < 3; ++d) {
+    c->ax[d].assign(src[d], src[d] + n[d]);
+    auto& a = c->ax[d];
+    auto& b = c->beta[d];
+    b.resize(n[d]);
+    b[0] = a[1] - a[0];
+    b[n[d] - 1] = a[n[d] - 1] - a[n[d] - 2];
+    for (int i = 1; i < n[d] - 1; ++i) b[i] = (a[i + 1] - a[i]) / 2.0 + (a[i] - a[i - 1]) / 2.0;
+    c->g.n[d] = n[d];
+    c->g.vmin[d] = a[0];
+    c->g.factor[d] = (double)(n[d] - 1) / (a[n[d] - 1] - a[0]);
+    zero_point(a, c->g.zero[d], c->g.pos[d], c->g.neg[d]);
+  }
+  for (int d = 0; d < 3; ++d) packed.insert(packed.end(), c->ax[d].begin(), c->ax[d].end());
+  for (int d = 0; d < 3; ++d) packed.insert(packed.end(), c->beta[d].begin(), c->beta[d].end());
+  if (int rc = dalloc(c, &c->d_axes, packed.size())) return rc;
+  HIP_TRY(c, hipMemcpy(c->d_axes, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
+  {
+    double* p = c->d_axes;
+    for (int d = 0; d < 3; ++d) { c->g.ax[d] = p; p += n[d]; }
+    for (int d = 0; d < 3; ++d) { c->g.beta[d] = p; p += n[d]; }
+  }
+  c->g.N = N;
+  c->g.nxy = n[0] * n[1];
+  double b3min = c->beta[0][0] * c->beta[1][0] * c->beta[2][0];
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j)
+      for (int i = 0; i < n[0]; ++i) {
+        const double b3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
+        if (b3 < b3min) b3min = b3;
+      }
+  // ---- species constants (scaling.f90:163-201 calculate_cross_section)
+  const double m = cfg->mass, d = cfg->diam;
+  const double m_red = m * m / (m + m);
+  const double omega_ab = 0.5 * (cfg->omega + cfg->omega);
+  const double sigma_vhs = 0.25 * (d + d) * (d + d);
+  double sigma = sigma_vhs;
+  if (cfg->csection_coeff == DVM_CS_BIRD)
+    sigma = sigma_vhs * pow(m_red, 0.5 - omega_ab) / gamma_ref(2.5 - omega_ab);
+  else if (cfg->csection_coeff == DVM_CS_VKI)
+    sigma = (3.32335097045 * pow(m_red, 1.0 - omega_ab) * pow(cfg->t_viscosity, omega_ab)) /
+            (gamma_ref(4.5 - omega_ab) * cfg->viscosity);
+  c->cdfp.mass = m;
+  c->cdfp.cdf_tolerance = cfg->cdf_tolerance;
+  c->cdfp.dt = cfg->dt;
+  c->cdfp.kn = cfg->kn;
+  c->cdfp.sigma = sigma;
+  c->cdfp.rms_sq = cfg->coln_rms * cfg->coln_rms;
+  const double b3a = b3min * pow(m, 3.0 / 2.0);
+  c->cdfp.beta3 = 0.5 * (b3a + b3a);
+  c->cdfp.sample_mode = cfg->sample_mode;
+  c->cdfp.samples_per_cell = cfg->samples_per_cell;
+  c->collp.seed = cfg->seed;
+  c->collp.mass = m;
+  c->collp.vhs_exponent = 2.0 - 2.0 * omega_ab;
+  c->collp.hard_sphere = (c->collp.vhs_exponent == 1.0);
+  const double alpha_ab = 0.5 * (cfg->alpha + cfg->alpha);
+  c->collp.ralpha = 1.0 / alpha_ab;
+  c->collp.vss = fabs(c->collp.ralpha - 1.0) > kDoubleTol;
+  c->collp.num_repl = cfg->num_repl;
+  // ---- resident buffers
+  const size_t CN = (size_t)c->ncell * N, CN1 = (size_t)c->ncell * (N + 1);
+  int rc;
+  if ((rc = dalloc(c, &c->f, CN)) || (rc = dalloc(c, &c->fd, CN)) || (rc = dalloc(c, &c->eqv, CN)) ||
+      (rc = dalloc(c, &c->eqc, CN1)) || (rc = dalloc(c, &c->neqc, CN1)) ||
+      (rc = dalloc(c, &c->eqtab, (size_t)c->ncell * kNBucket)) ||
+      (rc = dalloc(c, &c->neqtab, (size_t)c->ncell * kNBucket)) || (rc = dalloc(c, &c->eqpar, (size_t)c->ncell * 5)) ||
+      (rc = dalloc(c, &c->props, (size_t)c->ncell * DVM_NPROP)) || (rc = dalloc(c, &c->factors, (size_t)c->ncell * 3)) ||
+      (rc = dalloc(c, &c->counts, (size_t)c->ncell * 4)) || (rc = dalloc(c, &c->err, 4)))
+    return rc;
+  HIP_TRY(c, hipMemset(c->err, 0, 4 * sizeof(int)));
+  HIP_TRY(c, hipMemset(c->counts, 0, (size_t)c->ncell * 4 * sizeof(int)));
+  // collision accumulator placement: the cell's slice in LDS when it fits
+  c->coll_lds = (size_t)(kAxisSlots + N) * sizeof(double);
+  c->lds_collide = c->coll_lds <= kLdsBytes;
+  if (c->lds_collide) {
+    HIP_TRY(c, hipFuncSetAttribute((const void*)k_collide<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)c->coll_lds));
+  } else {
+    c->coll_lds = (size_t)kAxisSlots * sizeof(double);
+  }
+  // ---- convection constants
+  ConvParams& p = c->convp;
+  p.nx = cfg->nx;
+  p.ny = cfg->ny;
+  p.max_along = cfg->nx > cfg->ny ? cfg->nx : cfg->ny;
+  p.fx = cfg->dt / cfg->dx;
+  p.fy = (cfg->dim == DVM_DIM_2D) ? cfg->dt / cfg->dy : 0.0;
+  p.acc = cfg->acc_coeff;
+  p.bc[0] = cfg->bc_left;
+  p.bc[1] = cfg->bc_right;
+  for (int s = 0; s < 4; ++s) { p.wall_side[s] = 0; p.couette_side[s] = 0; p.flux_out[s] = 1.0; }
+  // boundary_conditions.f90:662-673 (flag kept per grid; the reference's SAVE makes it sticky across grids)
+  const int izero = c->g.zero[0], jzero = c->g.zero[1];
+  p.izflag = (izero != 0 && izero != c->g.neg[0] && izero != c->g.pos[0] && izero != n[0] - 1);
+  p.jzflag = (jzero != 0 && jzero != c->g.neg[1] && jzero != c->g.pos[1] && jzero != n[1] - 1);
+  if (cfg->dim == DVM_DIM_2D && cfg->enable_convection) {
+    p.wall_side[0] = cfg->bc_left == DVM_BC_WALL;
+    p.wall_side[1] = cfg->bc_right == DVM_BC_WALL;
+    p.wall_side[2] = 1;  // bottom and top are walls unconditionally in trunk 2-D
+    p.wall_side[3] = 1;
+    p.couette_side[0] = cfg->bc_left == DVM_BC_COUETTE;
+    p.couette_side[1] = cfg->bc_right == DVM_BC_COUETTE;
+    // wall Maxwellian * beta^3 on each wall's outgoing half-space and its flux
+    // (boundary_conditions.f90:785-828 and analogues), summed in the
+    // reference's loop order
+    std::vector<double> wm((size_t)4 * N, 0.0);
+    for (int s = 0; s < 4; ++s) {
+      if (!p.wall_side[s]) continue;
+      const double* wl = cfg->wall[s];
+      const double kt = wl[DVM_W_TEMP], u = wl[DVM_W_U], v = wl[DVM_W_V], w = wl[DVM_W_W];
+      const double mc = sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * kt * kt * kt));
+      double flux_out = 0.0;
+      double* gm = wm.data() + (size_t)s * N;
+      auto put = [&](int i, int j, int k) {
+        const double beta3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
+        const double mv = maxwell_host(c->ax[0][i], c->ax[1][j], c->ax[2][k], mc, m, u, v, w, kt);
+        gm[i + n[0] * (j + n[1] * k)] = mv * beta3;
+        const double vn = (s < 2) ? c->ax[0][i] : c->ax[1][j];
+        flux_out = flux_out + fabs(vn) * mv * beta3;
+      };
+      if (s == 0) { for (int k = 0; k < n[2]; ++k) for (int j = 0; j < n[1]; ++j) for (int i = c->g.pos[0]; i < n[0]; ++i) put(i, j, k); }
+      if (s == 1) { for (int k = 0; k < n[2]; ++k) for (int j = 0; j < n[1]; ++j) for (int i = 0; i <= c->g.neg[0]; ++i) put(i, j, k); }
+      if (s == 2) { for (int k = 0; k < n[2]; ++k) for (int i = 0; i < n[0]; ++i) for (int j = c->g.pos[1]; j < n[1]; ++j) put(i, j, k); }
+      if (s == 3) { for (int k = 0; k < n[2]; ++k) for (int i = 0; i < n[0]; ++i) for (int j = 0; j <= c->g.neg[1]; ++j) put(i, j, k); }
+      p.flux_out[s] = flux_out;
+    }
+    if ((rc = dalloc(c, &c->wallM, (size_t)4 * N)) || (rc = dalloc(c, &c->ratio, (size_t)4 * p.max_along))) return rc;
+    HIP_TRY(c, hipMemcpy(c->wallM, wm.data(), wm.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->ratio, 0, (size_t)4 * p.max_along * sizeof(double)));
+    if (cfg->acc_coeff < 1.0) {
+      if ((rc = dalloc(c, &c->spec, (size_t)4 * p.max_along * N))) return rc;
+    }
+  }
+  if (cfg->dim == DVM_DIM_1D && cfg->enable_convection) {
+    // apply_fixed_in_out_flow (grid_ref_code boundary_conditions.f90:783-913):
+    // the wall Maxwellian (n, u, T; v = w = 0) times beta^3
+    std::vector<double> inf((size_t)2 * N, 0.0);
+    for (int s = 0; s < 2; ++s) {
+      const double* wl = cfg->wall[s];
+      const double dens = wl[DVM_W_DENS], temp = wl[DVM_W_TEMP], u = wl[DVM_W_U];
+      const double mc = dens * sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
+      for (int k = 0, l = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+          for (int i = 0; i < n[0]; ++i, ++l) {
+            const double beta3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
+            inf[(size_t)s * N + l] = maxwell_host(c->ax[0][i], c->ax[1][j], c->ax[2][k], mc, m, u, 0.0, 0.0, temp) * beta3;
+          }
+    }
+    if ((rc = dalloc(c, &c->inflow, (size_t)2 * N))) return rc;
+    HIP_TRY(c, hipMemcpy(c->inflow, inf.data(), inf.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return DVM_OK;
+}
+
+extern "C" int dvm_ctx_create(const dvm_config* cfg, dvm_ctx** out) {
+  if (!out) return fail(nullptr, DVM_E_ARG, "null output pointer");
+  *out = nullptr;
+  if (int rc = validate(cfg)) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, DVM_E_HIP, "no HIP device is visible");
+  dvm_ctx* c = new dvm_ctx();
+  const int rc = create_impl(cfg, c);
+  if (rc) {
+    snprintf(g_create_msg, sizeof g_create_msg, "%s", c->msg);
+    dvm_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return DVM_OK;
+}
+
+static int check_err(dvm_ctx* c) {
+  int h[4] = {0, 0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(h, c->err, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (h[0]) {
+    HIP_TRY(c, hipMemset(c->err, 0, sizeof h));
+    return fail(c, DVM_E_REMAP, "remap found no exterior point (error flags 0x%x)", h[0]);
+  }
+  return DVM_OK;
+}
+
+extern "C" int dvm_upload_f(dvm_ctx* c, int32_t species, const double* f) {
+  if (!c || !f) return fail(c, DVM_E_ARG, "null argument");
+  if (species != 0) return fail(c, DVM_E_ARG, "only species 0 is supported");
+  const size_t bytes = (size_t)c->ncell * c->N * sizeof(double);
+  HIP_TRY(c, hipMemcpyAsync(c->f, f, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemsetAsync(c->eqpar, 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->f_valid = true;
+  c->props_valid = false;
+  return DVM_OK;
+}
+
+extern "C" int dvm_download_f(dvm_ctx* c, int32_t species, double* f) {
+  if (!c || !f) return fail(c, DVM_E_ARG, "null argument");
+  if (species != 0) return fail(c, DVM_E_ARG, "only species 0 is supported");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "no distribution uploaded");
+  const size_t bytes = (size_t)c->ncell * c->N * sizeof(double);
+  HIP_TRY(c, hipMemcpyAsync(f, c->f, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return DVM_OK;
+}
+
+static int moments_async(dvm_ctx* c) {
+  hipLaunchKernelGGL(k_moments, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass);
+  HIP_TRY(c, hipGetLastError());
+  c->props_valid = true;
+  return DVM_OK;
+}
+
+extern "C" int dvm_moments(dvm_ctx* c, double* props_out) {
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "moments before upload");
+  HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
+  if (int rc = moments_async(c)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
+  if (props_out) {
+    HIP_TRY(c, hipMemcpyAsync(props_out, c->props, (size_t)c->ncell * DVM_NPROP * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return DVM_OK;
+}
+
+static int cdf_async(dvm_ctx* c) {
+  hipLaunchKernelGGL(k_cdf, dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f, c->props, c->eqv,
+                     c->eqc, c->neqc, c->eqtab, c->neqtab, c->eqpar, c->counts, c->factors);
+  HIP_TRY(c, hipGetLastError());
+  return DVM_OK;
+}
+
+static int collide_async(dvm_ctx* c, int ntime) {
+  if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
+  if (int rc = cdf_async(c)) return rc;
+  if (c->lds_collide) {
+    hipLaunchKernelGGL(k_collide<true>, dim3(c->ncell), dim3(kCollBS), c->coll_lds, c->stream, c->g, c->collp, ntime,
+                       c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors, c->err);
+  } else {
+    HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+    hipLaunchKernelGGL(k_collide<false>, dim3(c->ncell), dim3(kCollBS), c->coll_lds, c->stream, c->g, c->collp,
+                       ntime, c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors,
+                       c->err);
+  }
+  HIP_TRY(c, hipGetLastError());
+  double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
+  return DVM_OK;
+}
+
+extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "collide needs f and its moments");
+  if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if (int rc = collide_async(c, ntime)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  if (samples_out) {
+    std::vector<int> h((size_t)c->ncell * 4, 0);
+    if (c->cfg.coll_solver == DVM_COLL_VR)
+      HIP_TRY(c, hipMemcpyAsync(h.data(), c->counts, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < c->ncell; ++q) samples_out[q] = h[(size_t)q * 4 + 3];
+  }
+  return check_err(c);
+}
+
+static int convect_async(dvm_ctx* c) {
+  const dvm_config& cfg = c->cfg;
+  if (!cfg.enable_convection || cfg.dim == DVM_DIM_0D) return DVM_OK;
+  const int N = c->N;
+  const size_t total = (size_t)c->ncell * N;
+  const unsigned blocks = (unsigned)((total + kConvBS - 1) / kConvBS);
+  if (cfg.dim == DVM_DIM_2D) {
+    ConvParams& p = c->convp;
+    if (p.acc > 0.0)
+      hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, c->stream, c->g, p, c->f, c->ratio);
+    if (p.acc < 1.0) {
+      const long long ns = (long long)4 * p.max_along * N;
+      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->spec, ns, 0.0);
+      hipLaunchKernelGGL(k_spec_ghost, dim3((N + 255) / 256, p.max_along, 4), dim3(256), 0, c->stream, c->g, p, c->f,
+                         c->spec, c->err);
+    }
+    hipLaunchKernelGGL(k_convect2d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, p, c->f, c->fd, c->wallM,
+                       c->ratio, c->spec);
+  } else {
+    hipLaunchKernelGGL(k_convect1d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd, c->inflow);
+  }
+  HIP_TRY(c, hipGetLastError());
+  double* t = c->f; c->f = c->fd; c->fd = t;  // phi := phi_conv
+  return DVM_OK;
+}
+
+extern "C" int dvm_convect(dvm_ctx* c, int32_t species, int32_t ntime) {
+  (void)ntime;
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (species != 0) return fail(c, DVM_E_ARG, "only species 0 is supported");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "convect before upload");
+  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  if (int rc = convect_async(c)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  return check_err(c);
+}
+
+extern "C" int dvm_step(dvm_ctx* c, int32_t ntime) {
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "step needs f and its moments");
+  if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if (int rc = collide_async(c, ntime)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  if (int rc = convect_async(c)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
+  if (int rc = moments_async(c)) return rc;
+  HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
+  return DVM_OK;
+}
+
+extern "C" int dvm_remap(dvm_ctx* c, int64_t n, const double* uvw, double f_off, int32_t* stencil, double* mass) {
+  if (!c || (n > 0 && (!uvw || !stencil || !mass))) return fail(c, DVM_E_ARG, "null argument");
+  if (n <= 0) return DVM_OK;
+  double *d_uvw = nullptr, *d_m = nullptr;
+  int* d_st = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &d_uvw, (size_t)3 * n)) || (rc = dalloc(c, &d_m, (size_t)7 * n)) ||
+      (rc = dalloc(c, &d_st, (size_t)7 * n))) {
+    if (d_uvw) (void)hipFree(d_uvw);
+    if (d_m) (void)hipFree(d_m);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(d_uvw, uvw, (size_t)3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_remap_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->g, (long long)n,
+                       d_uvw, f_off, d_st, d_m);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(stencil, d_st, (size_t)7 * n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(mass, d_m, (size_t)7 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_uvw);
+  (void)hipFree(d_m);
+  (void)hipFree(d_st);
+  if (e != hipSuccess) return fail(c, DVM_E_HIP, "dvm_remap: %s", hipGetErrorString(e));
+  return DVM_OK;
+}
+
+extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_t max_records, int32_t* irec,
+                                 double* drec, int64_t* n_out) {
+  if (!c || !n_out) return fail(c, DVM_E_ARG, "null argument");
+  if (cell < 0 || cell >= c->ncell) return fail(c, DVM_E_ARG, "cell out of range");
+  if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "trace needs f and its moments");
+  if (max_records > 0 && (!irec || !drec)) return fail(c, DVM_E_ARG, "null record buffers");
+  // The CDF rebuild below commits the equilibrium cache exactly as the next
+  // collide would (same f, same moments), so the trace leaves later steps
+  // unchanged.
+  if (c->cfg.coll_solver != DVM_COLL_VR) { *n_out = 0; return DVM_OK; }
+  if (int rc = cdf_async(c)) return rc;
+  int h[4];
+  HIP_TRY(c, hipMemcpyAsync(h, c->counts + (size_t)cell * 4, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const long long total = h[3];
+  const long long nrec = max_records < total ? max_records : total;
+  if (nrec > 0) {
+    int* d_i = nullptr;
+    double* d_d = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &d_i, (size_t)32 * nrec)) || (rc = dalloc(c, &d_d, (size_t)16 * nrec))) {
+      if (d_i) (void)hipFree(d_i);
+      return rc;
+    }
+    const unsigned blocks = (unsigned)((nrec + kCollBS - 1) / kCollBS);
+    hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kCollBS), 0, c->stream, c->g, c->collp, ntime, cell, c->f, c->eqv,
+                       c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors, (long long)nrec, d_i, d_d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(irec, d_i, (size_t)32 * nrec * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(drec, d_d, (size_t)16 * nrec * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_i);
+    (void)hipFree(d_d);
+    if (e != hipSuccess) return fail(c, DVM_E_HIP, "dvm_collide_trace: %s", hipGetErrorString(e));
+  }
+  *n_out = total;
+  return DVM_OK;
+}
+
+extern "C" int dvm_last_timings(const dvm_ctx* c, double* collide_ms, double* convect_ms, double* moments_ms) {
+  if (!c) return DVM_E_ARG;
+  float ms = 0.0f;
+  if (collide_ms) { *collide_ms = (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) ? ms : -1.0; }
+  if (convect_ms) { *convect_ms = (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) ? ms : -1.0; }
+  if (moments_ms) { *moments_ms = (hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) ? ms : -1.0; }
+  return DVM_OK;
+}

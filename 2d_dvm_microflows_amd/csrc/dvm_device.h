@@ -1,0 +1,235 @@
+// dvm_device.h -- device-side building blocks of the gfx950 DVM time step.
+//
+// Every routine here is written for one lane of a 64-wide wavefront; the
+// kernels in dvm_kernels.hip compose them. FP64 routines follow the reference's
+// expression order (the library is built with -ffp-contract=off) so that the
+// sampled nodes and remap stencils equal the reference's bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dvm {
+
+// constants.f90:30 -- the reference's value of pi (a typo for 3.14159...),
+// kept because it enters the equilibrium normalisation and the scattering
+// azimuth.
+constexpr double kPiRef = 3.1459265358979323;
+constexpr double kDoubleTol = 1.0e-13;  // constants.f90:52
+constexpr int kNBucket = 1000;          // search table size, variance_reduction.f90:471-493
+constexpr int kSearchCap = 96;          // bound on bisection steps (>> log2 of any grid)
+
+// Velocity grid as seen by a kernel. Axes and quadrature weights may live in
+// global memory or be staged into LDS by the caller.
+struct Grid {
+  int n[3];        // points per axis
+  int N;           // n0*n1*n2
+  int nxy;         // n0*n1
+  double vmin[3];  // first coordinate of each axis
+  double factor[3];// (n-1)/(vmax-vmin): first guess of the nearest-node search
+  int zero[3], neg[3], pos[3];  // find_zero_point results per axis
+  const double* ax[3];
+  const double* beta[3];
+};
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon et al., SC'11), the counter-based generator of
+// Random123; one call gives four 32-bit words.
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0;
+    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Two FP64 uniforms in [0,1) with 53 random bits each, for counter
+// (sample, tag, cell, ntime): tag = integral (1..3) | slot << 2, slot 0 for
+// the partner picks and r = 1..NRepl for the scattering angles of location r.
+__device__ __forceinline__ void uniforms2(uint64_t seed, uint32_t sample, uint32_t tag, uint32_t cell,
+                                          uint32_t ntime, double& ua, double& ub) {
+  uint32_t o[4];
+  philox4x32_10(sample, tag, cell, ntime, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+  const uint64_t a = ((uint64_t)o[1] << 32) | o[0];
+  const uint64_t b = ((uint64_t)o[3] << 32) | o[2];
+  ua = (double)(a >> 11) * 0x1.0p-53;
+  ub = (double)(b >> 11) * 0x1.0p-53;
+}
+
+// ---------------------------------------------------------------- signs
+// isgn / dsgn (math_utils.f90:38-60): sign(1, x) with sign(1, -0.0) = -1.
+__device__ __forceinline__ int isgn(double x) { return signbit(x) ? -1 : 1; }
+__device__ __forceinline__ double dsgn(double x) { return signbit(x) ? -1.0 : 1.0; }
+__device__ __forceinline__ int isign1(int k) { return k >= 0 ? 1 : -1; }
+
+// ---------------------------------------------------------- CDF sampling
+// referenced_search (sort_and_search.f90:40-86): inverse-CDF pick of node l
+// with u*cumul[N] in (cumul[l], cumul[l+1]], bracketed first by a
+// 1000-bucket table (tab[b] = (left, right)).
+__device__ __forceinline__ int referenced_search(const double* __restrict__ cumul, int N,
+                                                 const int2* __restrict__ tab, double u) {
+  if (u < kDoubleTol) return 0;
+  int b = (int)ceil(u * 1000.0);
+  b = b < 1 ? 1 : (b > kNBucket ? kNBucket : b);
+  const int2 lr = tab[b - 1];
+  int left = lr.x, right = lr.y;
+  int index = left + (right - left) / 2;
+  if (left == right) return index;
+  const double adj = u * cumul[N];
+  for (int it = 0; it < kSearchCap; ++it) {
+    const double c0 = cumul[index], c1 = cumul[index + 1];
+    if (adj > c0 && adj <= c1) break;
+    if (adj > c0) left = index; else right = index;
+    index = left + (right - left) / 2;
+  }
+  return index;
+}
+
+// ---------------------------------------------------------------- remap
+// binary_search_with_first_guess (sort_and_search.f90:130-183): nearest axis
+// node to val, ties to the lower index, clamped to the axis ends.
+__device__ __forceinline__ int nearest_node(const double* __restrict__ a, int mx, double val, int index) {
+  if (a[0] > val) return 0;
+  if (a[mx] < val) return mx;
+  index = index < 0 ? 0 : (index > mx ? mx : index);
+  int left = 0, right = mx;
+  if (fabs(val - a[index]) < kDoubleTol) return index;
+  if (index == right) index = index - 1;
+  for (int it = 0; it < kSearchCap; ++it) {
+    const double a0 = a[index], a1 = a[index + 1];
+    if (val > a0 && val <= a1) {
+      if (fabs(val - a0) > fabs(val - a1)) index = index + 1;
+      return index;
+    }
+    if (val > a0) left = index; else right = index;
+    index = left + (right - left) / 2;
+  }
+  return index;
+}
+
+__device__ __forceinline__ int first_guess(double factor, double val, double vmin, int mx) {
+  double g = round(factor * (val - vmin));  // Fortran nint: halves away from zero
+  g = g < -1.0 ? -1.0 : (g > (double)(mx + 1) ? (double)(mx + 1) : g);
+  return (int)g;
+}
+
+// find_boundary_stencil (remapping.f90:396-438)
+__device__ __forceinline__ void boundary_stencil(int& sa, int& sb, int& sc, double abc, int oi, int oj, int ok,
+                                                 const double* X, const double* Y, const double* Z,
+                                                 int imax, int jmax, int kmax) {
+  while (oi + sa > 0 && oi + sa < imax) {
+    if (fabs(abc) > 0.5 * fabs(X[oi + sa] - X[oi])) sa = sa + isign1(sa); else break;
+  }
+  while (oj + sb > 0 && oj + sb < jmax && oj - sb > 0 && oj - sb < jmax) {
+    if (fabs(abc) > 0.5 * fabs(Y[oj + sb] - Y[oj])) sb = sb + isign1(sb); else break;
+  }
+  while (ok + sc > 0 && ok + sc < kmax && ok - sc > 0 && ok - sc < kmax) {
+    if (fabs(abc) > 0.5 * fabs(Z[ok + sc] - Z[ok])) sc = sc + isign1(sc); else break;
+  }
+}
+
+// Result of a conservative remap: origin node, interior offsets, exterior
+// flag (bit 0 x, bit 1 y, bit 2 z) and the seven weights.
+struct Stencil {
+  int oi, oj, ok, a, b, c, flag;
+  double m[7];  // f_o, f_ix, f_iy, f_iz, f_ex, f_ey, f_ez
+};
+
+// compute_remapping_quantities (remapping.f90:92-394), the non-uniform branch
+// that trunk always takes. Moments 0, 1 and 2 of the 7-point stencil equal
+// f_off at (u,v,w) in exact arithmetic. flag == 0 is the reference's fatal case.
+__device__ __forceinline__ void remap(const Grid& g, double u, double v, double w, double f_off, Stencil& s) {
+  const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
+  const int imax = g.n[0] - 1, jmax = g.n[1] - 1, kmax = g.n[2] - 1;
+  int corner = 0;
+  int oi = nearest_node(X, imax, u, first_guess(g.factor[0], u, g.vmin[0], imax));
+  if (oi >= imax) { oi = imax; ++corner; } else if (oi <= 0) { oi = 0; ++corner; }
+  int oj = nearest_node(Y, jmax, v, first_guess(g.factor[1], v, g.vmin[1], jmax));
+  if (oj >= jmax) { oj = jmax; ++corner; } else if (oj <= 0) { oj = 0; ++corner; }
+  int ok = nearest_node(Z, kmax, w, first_guess(g.factor[2], w, g.vmin[2], kmax));
+  if (ok >= kmax) { ok = kmax; ++corner; } else if (ok <= 0) { ok = 0; ++corner; }
+  if (corner == 3) {
+    oi -= isign1(oi - 1);
+    oj -= isign1(oj - 1);
+    ok -= isign1(ok - 1);
+  }
+  const double ox = X[oi], oy = Y[oj], oz = Z[ok];
+  const double a = u - ox, b = v - oy, c = w - oz;
+  int sa = isgn(a), sb = isgn(b), sc = isgn(c);
+  if (oi + sa > imax || oi + sa < 0) { sa = -sa; boundary_stencil(sa, sb, sc, a, oi, oj, ok, X, Y, Z, imax, jmax, kmax); }
+  if (oj + sb > jmax || oj + sb < 0) { sb = -sb; boundary_stencil(sb, sa, sc, b, oj, oi, ok, Y, X, Z, jmax, imax, kmax); }
+  if (ok + sc > kmax || ok + sc < 0) { sc = -sc; boundary_stencil(sc, sb, sa, c, ok, oj, oi, Z, Y, X, kmax, jmax, imax); }
+  const double ix = X[oi + sa] - ox, iy = Y[oj + sb] - oy, iz = Z[ok + sc] - oz;
+  int flag = 0;
+  double ex = 0.0, ey = 0.0, ez = 0.0;
+  if (oi - sa <= imax && oi - sa >= 0) { flag |= 1; ex = X[oi - sa] - ox; }
+  if (oj - sb <= jmax && oj - sb >= 0) { flag |= 2; ey = Y[oj - sb] - oy; }
+  if (ok - sc <= kmax && ok - sc >= 0) { flag |= 4; ez = Z[ok - sc] - oz; }
+  const double num = f_off * (a * a + b * b + c * c - a * ix - b * iy - c * iz);
+  const double dx = ex * (ex - ix), dy = ey * (ey - iy), dz = ez * (ez - iz);
+  double den;
+  switch (flag) {
+    case 7: den = dx + dy + dz; break;
+    case 1: den = dx; break;
+    case 2: den = dy; break;
+    case 4: den = dz; break;
+    case 3: den = dx + dy; break;
+    case 5: den = dx + dz; break;
+    case 6: den = dy + dz; break;
+    default: den = 1.0; break;
+  }
+  const double fe = (flag != 0) ? num / den : 0.0;
+  const double fex = (flag & 1) ? fe : 0.0;
+  const double fey = (flag & 2) ? fe : 0.0;
+  const double fez = (flag & 4) ? fe : 0.0;
+  const double fix = (f_off * a - fex * ex) / ix;
+  const double fiy = (f_off * b - fey * ey) / iy;
+  const double fiz = (f_off * c - fez * ez) / iz;
+  s.oi = oi; s.oj = oj; s.ok = ok; s.a = sa; s.b = sb; s.c = sc; s.flag = flag;
+  s.m[0] = f_off - fix - fiy - fiz - fex - fey - fez;
+  s.m[1] = fix; s.m[2] = fiy; s.m[3] = fiz; s.m[4] = fex; s.m[5] = fey; s.m[6] = fez;
+  if (flag == 0) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) s.m[q] = 0.0;
+  }
+}
+
+// Node index of stencil point q (0 origin, 1..3 interior, 4..6 exterior).
+__device__ __forceinline__ int stencil_node(const Stencil& s, int q, int nx, int nxy) {
+  int i = s.oi, j = s.oj, k = s.ok;
+  switch (q) {
+    case 1: i += s.a; break;
+    case 2: j += s.b; break;
+    case 3: k += s.c; break;
+    case 4: i -= s.a; break;
+    case 5: j -= s.b; break;
+    case 6: k -= s.c; break;
+    default: break;
+  }
+  return i + nx * j + nxy * k;
+}
+
+// ------------------------------------------------------- wave reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wavefront.
+__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+}  // namespace dvm

@@ -1,0 +1,83 @@
+// dvm_kernels.h -- kernel parameter blocks and declarations shared by the
+// kernels (dvm_kernels.hip) and the context / C-ABI code (dvm_capi.hip).
+#pragma once
+
+#include "dvm_device.h"
+#include "../../include/dvm_hip.h"
+
+namespace dvm {
+
+constexpr int kMomBS = 256;      // moments / wall-flux workgroup
+constexpr int kCdfBS = 256;      // CDF workgroup
+constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell's LDS slice
+constexpr int kConvBS = 256;     // convection lanes per workgroup
+constexpr int kAxisSlots = 384;  // LDS doubles reserved for the three axes (<= 128 points each)
+constexpr int kMaxAxis = 128;
+// LDS bytes available to one workgroup on gfx950
+constexpr size_t kLdsBytes = 160 * 1024;
+
+struct CdfParams {
+  double mass, cdf_tolerance, dt, kn, sigma;
+  double rms_sq, beta3;          // reference sample-count rule
+  int sample_mode;
+  long long samples_per_cell;
+};
+
+struct CollParams {
+  uint64_t seed;
+  double mass, vhs_exponent, ralpha;
+  int hard_sphere;               // vhs_exponent == 1: g^(2-2w) == g exactly
+  int vss;                       // |1/alpha - 1| > 1e-13: Bird's VSS deflection
+  int num_repl;
+};
+
+struct CellView {
+  const double* f;
+  const double* eqv;
+  const double* eqc;
+  const double* neqc;
+  const int2* eqtab;
+  const int2* neqtab;
+};
+
+struct ConvParams {
+  int nx, ny, max_along;
+  double fx, fy;                 // dt/dx, dt/dy
+  double acc;                    // accommodation coefficient
+  int wall_side[4];              // diffuse/specular wall on side s
+  int couette_side[4];
+  int bc[2];                     // 1-D boundary types (left, right)
+  int izflag, jzflag;            // zero-plane flags of boundary_conditions.f90:662-673
+  double flux_out[4];            // sum |v_n| M_wall beta^3 over the wall's outgoing half-space
+};
+
+__global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass);
+__global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
+                      double* __restrict__ eqv, double* __restrict__ eqc, double* __restrict__ neqc,
+                      int2* __restrict__ eqtab, int2* __restrict__ neqtab, double* __restrict__ eqpar,
+                      int* __restrict__ counts, double* __restrict__ factors);
+template <bool kLds>
+__global__ void k_collide(Grid g, CollParams cp, int ntime, const double* __restrict__ f, double* __restrict__ fd,
+                          const double* __restrict__ eqv, const double* __restrict__ eqc,
+                          const double* __restrict__ neqc, const int2* __restrict__ eqtab,
+                          const int2* __restrict__ neqtab, const int* __restrict__ counts,
+                          const double* __restrict__ factors, int* __restrict__ err);
+__global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
+                        const double* __restrict__ eqv, const double* __restrict__ eqc,
+                        const double* __restrict__ neqc, const int2* __restrict__ eqtab,
+                        const int2* __restrict__ neqtab, const int* __restrict__ counts,
+                        const double* __restrict__ factors, long long max_records, int* __restrict__ irec,
+                        double* __restrict__ drec);
+__global__ void k_remap_batch(Grid g, long long n, const double* __restrict__ uvw, double f_off,
+                              int* __restrict__ st, double* __restrict__ m);
+__global__ void k_wall_ratio(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ ratio);
+__global__ void k_spec_ghost(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ spec,
+                             int* __restrict__ err);
+__global__ void k_fill(double* __restrict__ a, long long n, double v);
+__global__ void k_convect2d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ fo,
+                            const double* __restrict__ wallM, const double* __restrict__ ratio,
+                            const double* __restrict__ spec);
+__global__ void k_convect1d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ fo,
+                            const double* __restrict__ inflow);
+
+}  // namespace dvm

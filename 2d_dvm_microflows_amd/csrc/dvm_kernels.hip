@@ -1,0 +1,646 @@
+// dvm_kernels.hip -- gfx950 kernels of the DVM time step.
+//
+// Layout in HBM (all FP64 unless noted), N = velocity nodes, C = cells:
+//   f, fd        [C][N]      distribution and its double buffer, node fastest
+//   eqv          [C][N]      cached discrete equilibrium per node (f_eq * beta^3)
+//   eqc, neqc    [C][N+1]    cumulative equilibrium / |deviation| distributions
+//   eqtab,neqtab [C][1000]   int2 bucket brackets of the inverse-CDF search
+//   eqpar        [C][5]      (dens,u,v,w,T) the equilibrium was built for
+//   props        [C][8]      moments (DVM_P_*)
+//   counts       [C][4]      int32 samples of the 3 integrals + their sum
+//   factors      [C][3]      depletion factor of each integral
+// Kernels:
+//   k_moments    one workgroup per cell; two passes, FP64 wave reductions
+//   k_cdf        one workgroup per cell; cache test, equilibrium rebuild on a
+//                miss, |deviation| prefix scan, bucket tables, pair counts
+//   k_collide    one workgroup per cell; samples spread over the lanes, the
+//                cell's increments accumulated in LDS (ds_add_f64) when the
+//                slice fits, else global atomics into fd
+//   k_convect2d  one lane per (cell,node); interior donor-cell upwind and the
+//                trunk boundary-cell update with wall / neighbour ghosts
+//   k_convect1d  one lane per (cell,node); zone-grid 1-D upwind and in/out flow
+//   k_wall_ratio, k_spec_ghost  diffuse flux balance and specular remap ghosts
+#include "dvm_kernels.h"
+
+namespace dvm {
+
+// ------------------------------------------------------------ reductions
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// Inclusive scan over the workgroup; returns the lane's prefix and the
+// workgroup total in `total`.
+template <int BS>
+__device__ __forceinline__ double block_scan(double v, double* red, double& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double inc = wave_inclusive_scan(v, lane);
+  if (lane == 63) red[wid] = inc;
+  __syncthreads();
+  double off = 0.0, tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) {
+    const double s = red[w];
+    if (w < wid) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return inc + off;
+}
+
+__device__ __forceinline__ void node_ijk(int l, int nx, int ny, int& i, int& j, int& k) {
+  k = l / (nx * ny);
+  const int r = l - k * nx * ny;
+  j = r / nx;
+  i = r - j * nx;
+}
+
+// --------------------------------------------------------------- moments
+// calculate_physical_properties (physical_props.f90:285-485) for one
+// monatomic species: n, u, v, w, <|v|>, T and the derived fields.
+__global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
+                                                    double* __restrict__ props, double mass) {
+  __shared__ double red[kMomBS / 64];
+  const int cell = blockIdx.x;
+  const double* fc = f + (size_t)cell * g.N;
+  const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
+  double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, ss = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    const double v = fc[l], x = X[i], y = Y[j], z = Z[k];
+    s0 += v;
+    sx += v * x;
+    sy += v * y;
+    sz += v * z;
+    ss += v * sqrt(x * x + y * y + z * z);
+  }
+  const double dens = block_sum<kMomBS>(s0, red);
+  sx = block_sum<kMomBS>(sx, red);
+  sy = block_sum<kMomBS>(sy, red);
+  sz = block_sum<kMomBS>(sz, red);
+  ss = block_sum<kMomBS>(ss, red);
+  double* p = props + (size_t)cell * DVM_NPROP;
+  if (dens == 0.0) {
+    if (threadIdx.x < DVM_NPROP) p[threadIdx.x] = 0.0;
+    return;
+  }
+  const double u = sx / dens, v = sy / dens, w = sz / dens;
+  double st = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    double cx = X[i] - u; cx = cx * cx;
+    double cy = Y[j] - v; cy = cy * cy;
+    double cz = Z[k] - w; cz = cz * cz;
+    st += fc[l] * (cx + cy + cz);
+  }
+  st = block_sum<kMomBS>(st, red);
+  if (threadIdx.x == 0) {
+    const double t = 2.0 * mass * st / (3.0 * dens);
+    p[DVM_P_DENS] = dens;
+    p[DVM_P_U] = u;
+    p[DVM_P_V] = v;
+    p[DVM_P_W] = w;
+    p[DVM_P_TRTEMP] = t;
+    p[DVM_P_TEMP] = (3.0 * t + 0.0 + 0.0) / 3.0;  // compute_overall_temperature, no internal dof
+    p[DVM_P_PRESSURE] = 0.5 * dens * t;
+    p[DVM_P_AVGSPEED] = ss / dens;
+  }
+}
+
+// ------------------------------------------------------------------- CDF
+// Bucket brackets of the inverse-CDF search (variance_reduction.f90:468-493):
+// right[r] is the first l >= 1 with cumul[l] >= r*(0.001*total) (N+1 if none),
+// left[r] = right[r-1] - 1, left[1] = 0, right[1000] = N.
+__device__ __forceinline__ void build_buckets(const double* __restrict__ cumul, int N, int2* __restrict__ tab,
+                                              int* __restrict__ rights) {
+  const double mx = 0.001 * cumul[N];
+  for (int r = threadIdx.x + 1; r <= kNBucket; r += kCdfBS) {
+    const double t = (double)r * mx;
+    int lo = 1, hi = N + 1;  // first l in [1,N] with cumul[l] >= t, else N+1
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (t > cumul[mid]) lo = mid + 1; else hi = mid;
+    }
+    rights[r - 1] = lo;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x + 1; r <= kNBucket; r += kCdfBS) {
+    const int left = (r == 1) ? 0 : rights[r - 2] - 1;
+    const int right = (r == kNBucket) ? N : rights[r - 1];
+    tab[r - 1] = make_int2(left, right);
+  }
+  __syncthreads();
+}
+
+// compute_cdf_vr (variance_reduction.f90:234-496) for the like-species pair,
+// plus compute_num_coll_pairs (collision_utilities.f90:106-140) and the
+// depletion factors (variance_reduction.f90:598-604,679,831,976).
+__global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
+                                                const double* __restrict__ props, double* __restrict__ eqv,
+                                                double* __restrict__ eqc, double* __restrict__ neqc,
+                                                int2* __restrict__ eqtab, int2* __restrict__ neqtab,
+                                                double* __restrict__ eqpar, int* __restrict__ counts,
+                                                double* __restrict__ factors) {
+  __shared__ double red[kCdfBS / 64];
+  __shared__ double pref[kCdfBS + 1];
+  __shared__ int rights[kNBucket];
+  __shared__ int s_new;
+  const int cell = blockIdx.x;
+  const int N = g.N;
+  const double* p = props + (size_t)cell * DVM_NPROP;
+  const double* fc = f + (size_t)cell * N;
+  double* ev = eqv + (size_t)cell * N;
+  double* ec = eqc + (size_t)cell * (N + 1);
+  double* nc = neqc + (size_t)cell * (N + 1);
+  double* par = eqpar + (size_t)cell * 5;
+  const double dens = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W];
+  double temp = p[DVM_P_TEMP];
+  const double mass = cp.mass;
+  if (threadIdx.x == 0) {
+    const double tol = cp.cdf_tolerance;
+    // Q4: w is not compared; a negative u or v makes its test always true
+    s_new = (fabs(dens - par[0]) > tol * dens || fabs(u - par[1]) > tol * u ||
+             fabs(v - par[2]) > tol * v || fabs(temp - par[4]) > tol * temp) ? 1 : 0;
+  }
+  __syncthreads();
+  const bool new_eq = s_new != 0;
+  __syncthreads();
+  if (new_eq) {
+    double coeff = 0.0;
+    if (dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
+    // eq_df = prefix of f_eq*beta^3; eqv = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
+    double carry = 0.0, carry_c = 0.0;
+    if (threadIdx.x == 0) ec[0] = 0.0;
+    for (int base = 0; base < N; base += kCdfBS) {
+      const int l = base + threadIdx.x;
+      double val = 0.0;
+      if (l < N) {
+        int i, j, k;
+        node_ijk(l, g.n[0], g.n[1], i, j, k);
+        const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+        const double beta3 = g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
+        double phi_eq = 0.0;
+        if (temp > 0.0) {
+          double cx = (x - u); cx = cx * cx;
+          double cy = (y - v); cy = cy * cy;
+          double cz = (z - w); cz = cz * cz;
+          const double csq = cx + cy + cz;
+          phi_eq = coeff * exp(-csq * mass / temp);
+        }
+        val = phi_eq * beta3;
+      }
+      double tot;
+      const double pre = carry + block_scan<kCdfBS>(val, red, tot);
+      if (threadIdx.x == 0) pref[0] = carry;
+      pref[threadIdx.x + 1] = pre;
+      __syncthreads();
+      const double e = (l < N) ? pref[threadIdx.x + 1] - pref[threadIdx.x] : 0.0;
+      __syncthreads();
+      carry = carry + tot;
+      double tot_c;
+      const double prc = carry_c + block_scan<kCdfBS>(e, red, tot_c);
+      carry_c = carry_c + tot_c;
+      if (l < N) {
+        ev[l] = e;
+        ec[l + 1] = prc;
+      }
+    }
+    __syncthreads();
+    build_buckets(ec, N, eqtab + (size_t)cell * kNBucket, rights);
+    if (threadIdx.x == 0) {
+      par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
+    }
+  }
+  // |f - f_eq| prefix
+  double carry = 0.0;
+  if (threadIdx.x == 0) nc[0] = 0.0;
+  for (int base = 0; base < N; base += kCdfBS) {
+    const int l = base + threadIdx.x;
+    const double d = (l < N) ? fabs(fc[l] - ev[l]) : 0.0;
+    double tot;
+    const double pre = carry + block_scan<kCdfBS>(d, red, tot);
+    carry = carry + tot;
+    if (l < N) nc[l + 1] = pre;
+  }
+  __syncthreads();
+  build_buckets(nc, N, neqtab + (size_t)cell * kNBucket, rights);
+  if (threadIdx.x == 0) {
+    const double neq_dens = nc[N], eq_dens = ec[N];
+    const double h = 0.5 * neq_dens / dens;
+    const double dA[3] = {1.0, h, h}, dB[3] = {h, 1.0, h};
+    const double wA[3] = {eq_dens, neq_dens, neq_dens}, wB[3] = {neq_dens, eq_dens, neq_dens};
+    const double factor_coeff = cp.dt * 0.5 / cp.kn;
+    const double tot = dA[0] * dB[0] + dA[1] * dB[1] + dA[2] * dB[2];
+    int sum = 0;
+    for (int t = 0; t < 3; ++t) {
+      double num;
+      if (cp.sample_mode == DVM_SAMPLES_EXPLICIT) {
+        num = (tot > 0.0) ? round((double)cp.samples_per_cell * (dA[t] * dB[t]) / tot) : 0.0;
+      } else {
+        const double ct = pow(0.5 * (p[DVM_P_TEMP] + p[DVM_P_TEMP]), 2.0 / 3.0);
+        num = round((0.5 * cp.dt * ct * dA[t] * dB[t]) / (cp.kn * cp.rms_sq * cp.beta3));
+      }
+      if (!(num > 0.0)) num = 0.0;
+      if (num > 2147483647.0) num = 2147483647.0;
+      const int n = (int)num;
+      counts[cell * 4 + t] = n;
+      sum += n;
+      factors[cell * 3 + t] = (n > 0) ? factor_coeff * cp.sigma * wA[t] * wB[t] / (double)n : 0.0;
+    }
+    counts[cell * 4 + 3] = sum;
+  }
+}
+
+// --------------------------------------------------------------- collide
+struct SampleRec {
+  int lA, lB, skipped, st1[7], st2[7];
+  double kd1, kd2, m1[7], m2[7];
+};
+
+// One VR sample: variance_reduction.f90:686-816 / 838-961 / 983-1116 with
+// vr_depletion_routine (:1127-1237) and the elastic branch of
+// replenish_collision (replenishing_collisions.f90:134-354). Scatter(node,val)
+// receives the depletions and the replenishment of every stencil point.
+template <bool kRecord, class Scatter>
+__device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, const CellView& cv, int cell,
+                                         int ntime, int t, int s, double factor, Scatter&& scatter,
+                                         SampleRec* rec) {
+  const int N = g.N, nx = g.n[0], nxy = g.nxy;
+  double uA, uB;
+  uniforms2(cp.seed, (uint32_t)s, (uint32_t)(t + 1), (uint32_t)cell, (uint32_t)ntime, uA, uB);
+  const int lA = (t == 0) ? referenced_search(cv.eqc, N, cv.eqtab, uA)
+                          : referenced_search(cv.neqc, N, cv.neqtab, uA);
+  const int lB = (t == 1) ? referenced_search(cv.eqc, N, cv.eqtab, uB)
+                          : referenced_search(cv.neqc, N, cv.neqtab, uB);
+  int iA, jA, kA, iB, jB, kB;
+  node_ijk(lA, nx, g.n[1], iA, jA, kA);
+  node_ijk(lB, nx, g.n[1], iB, jB, kB);
+  const double x1 = g.ax[0][iA], y1 = g.ax[1][jA], z1 = g.ax[2][kA];
+  const double x2 = g.ax[0][iB], y2 = g.ax[1][jB], z2 = g.ax[2][kB];
+  if (kRecord) { rec->lA = lA; rec->lB = lB; rec->skipped = 0; }
+  const double gx = x2 - x1, gy = y2 - y1, gz = z2 - z1;
+  const double gg = sqrt(gx * gx + gy * gy + gz * gz);
+  if (gg < kDoubleTol) {
+    if (kRecord) rec->skipped = 1;
+    return 0;
+  }
+  const double g_sigma = cp.hard_sphere ? gg : pow(gg, cp.vhs_exponent);
+  const double signA = (t == 0) ? 1.0 : dsgn(cv.f[lA] - cv.eqv[lA]);
+  const double signB = (t == 1) ? 1.0 : dsgn(cv.f[lB] - cv.eqv[lB]);
+  const double sumA = signA, sumB = signB;
+  const double dep1 = sumB * 1.0 * factor * g_sigma;
+  const double dep2 = sumA * 1.0 * factor * g_sigma;
+  const double kd1 = sumA * dep1, kd2 = sumB * dep2;
+  if (kRecord) { rec->kd1 = kd1; rec->kd2 = kd2; }
+  if (!kRecord) {
+    scatter(lA, -kd1);
+    scatter(lB, -kd2);
+  }
+  const double kr1 = kd1 / (double)cp.num_repl, kr2 = kd2 / (double)cp.num_repl;
+  const double mf = cp.mass / (cp.mass + cp.mass);
+  const double vc0 = mf * x1 + mf * x2, vc1 = mf * y1 + mf * y2, vc2 = mf * z1 + mf * z2;
+  const double gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
+  const double gx2 = fabs(x2 - vc0), gy2 = fabs(y2 - vc1), gz2 = fabs(z2 - vc2);
+  const double ur = gx1 + gx2, vr = gy1 + gy2, wr = gz1 + gz2;
+  const double gp1 = sqrt(gx1 * gx1 + gy1 * gy1 + gz1 * gz1);
+  const double gp2 = sqrt(gx2 * gx2 + gy2 * gy2 + gz2 * gz2);
+  const double gr1 = gp1 / gg, gr2 = gp2 / gg;
+  int bad = 0;
+  for (int r = 1; r <= cp.num_repl; ++r) {
+    double uphi, uth;
+    uniforms2(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, uphi, uth);
+    // pick_random_velocity_on_sphere (collision_utilities.f90:284-352)
+    const double ph = 2.0 * kPiRef * uphi;
+    double sph, cph;
+    sincos(ph, &sph, &cph);
+    double rx, ry, rz;
+    if (cp.vss) {
+      const double ct = 2.0 * pow(uth, cp.ralpha) - 1.0;
+      const double stt = sqrt(1.0 - ct * ct);
+      const double den = sqrt(vr * vr + wr * wr);
+      if (den > kDoubleTol * gg) {
+        rx = ct * ur + stt * sph * den;
+        ry = ct * vr + stt * (gg * wr * cph - ur * vr * sph) / den;
+        rz = ct * wr - stt * (gg * vr * cph + ur * wr * sph) / den;
+      } else {
+        rx = ct * ur;
+        ry = stt * cph * ur;
+        rz = stt * sph * ur;
+      }
+    } else {
+      const double ct = 2.0 * uth - 1.0;
+      const double stt = sqrt(1.0 - ct * ct);
+      rx = gg * stt * cph;
+      ry = gg * stt * sph;
+      rz = gg * ct;
+    }
+    Stencil s1, s2;
+    remap(g, vc0 + gr1 * rx, vc1 + gr1 * ry, vc2 + gr1 * rz, 1.0, s1);
+    remap(g, vc0 + -(gr2 * rx), vc1 + -(gr2 * ry), vc2 + -(gr2 * rz), 1.0, s2);
+    if (kRecord && r == 1) {
+      rec->st1[0] = s1.oi; rec->st1[1] = s1.oj; rec->st1[2] = s1.ok; rec->st1[3] = s1.a;
+      rec->st1[4] = s1.b; rec->st1[5] = s1.c; rec->st1[6] = s1.flag;
+      rec->st2[0] = s2.oi; rec->st2[1] = s2.oj; rec->st2[2] = s2.ok; rec->st2[3] = s2.a;
+      rec->st2[4] = s2.b; rec->st2[5] = s2.c; rec->st2[6] = s2.flag;
+#pragma unroll
+      for (int q = 0; q < 7; ++q) { rec->m1[q] = s1.m[q]; rec->m2[q] = s2.m[q]; }
+    }
+    if (s1.flag == 0 || s2.flag == 0) { bad = 1; continue; }
+    if (!kRecord) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        if (q < 4 || (s1.flag & (1 << (q - 4)))) scatter(stencil_node(s1, q, nx, nxy), kr1 * s1.m[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        if (q < 4 || (s2.flag & (1 << (q - 4)))) scatter(stencil_node(s2, q, nx, nxy), kr2 * s2.m[q]);
+      }
+    }
+  }
+  return bad;
+}
+
+// Stage the three axes into LDS and point the grid at them.
+__device__ __forceinline__ void stage_axes(Grid& g, double* sh) {
+  double* p = sh;
+  for (int d = 0; d < 3; ++d) {
+    for (int q = threadIdx.x; q < g.n[d]; q += blockDim.x) p[q] = g.ax[d][q];
+    g.ax[d] = p;
+    p += g.n[d];
+  }
+}
+
+// Collision phase of one cell per workgroup. kLds: the cell's increments are
+// accumulated in LDS (dynamic shared memory of N doubles after the axes) and
+// added to f once; otherwise fd (a copy of f) takes global atomics.
+template <bool kLds>
+__global__ void __launch_bounds__(kCollBS) k_collide(Grid g, CollParams cp, int ntime,
+                                                     const double* __restrict__ f, double* __restrict__ fd,
+                                                     const double* __restrict__ eqv, const double* __restrict__ eqc,
+                                                     const double* __restrict__ neqc, const int2* __restrict__ eqtab,
+                                                     const int2* __restrict__ neqtab, const int* __restrict__ counts,
+                                                     const double* __restrict__ factors, int* __restrict__ err) {
+  extern __shared__ double dyn[];
+  const int cell = blockIdx.x;
+  const int N = g.N;
+  stage_axes(g, dyn);
+  double* acc = dyn + kAxisSlots;
+  if (kLds) {
+    for (int l = threadIdx.x; l < N; l += kCollBS) acc[l] = 0.0;
+  }
+  __syncthreads();
+  CellView cv;
+  cv.f = f + (size_t)cell * N;
+  cv.eqv = eqv + (size_t)cell * N;
+  cv.eqc = eqc + (size_t)cell * (N + 1);
+  cv.neqc = neqc + (size_t)cell * (N + 1);
+  cv.eqtab = eqtab + (size_t)cell * kNBucket;
+  cv.neqtab = neqtab + (size_t)cell * kNBucket;
+  const int n0 = counts[cell * 4 + 0], n1 = counts[cell * 4 + 1], n2 = counts[cell * 4 + 2];
+  const int total = n0 + n1 + n2;
+  double* gd = fd + (size_t)cell * N;
+  int bad = 0;
+  for (int q = threadIdx.x; q < total; q += kCollBS) {
+    const int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
+    const int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
+    const double factor = factors[cell * 3 + t];
+    if (kLds) {
+      bad |= vr_sample<false>(g, cp, cv, cell, ntime, t, s, factor,
+                              [&](int node, double val) { atomicAdd(&acc[node], val); }, nullptr);
+    } else {
+      bad |= vr_sample<false>(g, cp, cv, cell, ntime, t, s, factor,
+                              [&](int node, double val) { atomicAdd(&gd[node], val); }, nullptr);
+    }
+  }
+  if (bad) atomicOr(err, 1);
+  if (kLds) {
+    __syncthreads();
+    for (int l = threadIdx.x; l < N; l += kCollBS) gd[l] = cv.f[l] + acc[l];
+  }
+}
+
+// Parity trace of one cell: the same samples as k_collide, recorded instead
+// of scattered.
+__global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int ntime, int cell,
+                                                   const double* __restrict__ f, const double* __restrict__ eqv,
+                                                   const double* __restrict__ eqc, const double* __restrict__ neqc,
+                                                   const int2* __restrict__ eqtab, const int2* __restrict__ neqtab,
+                                                   const int* __restrict__ counts, const double* __restrict__ factors,
+                                                   long long max_records, int* __restrict__ irec,
+                                                   double* __restrict__ drec) {
+  const int N = g.N;
+  CellView cv;
+  cv.f = f + (size_t)cell * N;
+  cv.eqv = eqv + (size_t)cell * N;
+  cv.eqc = eqc + (size_t)cell * (N + 1);
+  cv.neqc = neqc + (size_t)cell * (N + 1);
+  cv.eqtab = eqtab + (size_t)cell * kNBucket;
+  cv.neqtab = neqtab + (size_t)cell * kNBucket;
+  const int n0 = counts[cell * 4 + 0], n1 = counts[cell * 4 + 1], n2 = counts[cell * 4 + 2];
+  const int total = n0 + n1 + n2;
+  for (int q = threadIdx.x + blockIdx.x * blockDim.x; q < total && q < max_records; q += blockDim.x * gridDim.x) {
+    const int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
+    const int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
+    SampleRec rec;
+    for (int z = 0; z < 7; ++z) { rec.st1[z] = 0; rec.st2[z] = 0; rec.m1[z] = 0.0; rec.m2[z] = 0.0; }
+    rec.kd1 = 0.0; rec.kd2 = 0.0;
+    vr_sample<true>(g, cp, cv, cell, ntime, t, s, factors[cell * 3 + t], [](int, double) {}, &rec);
+    int* ir = irec + (size_t)q * 32;
+    double* dr = drec + (size_t)q * 16;
+    for (int z = 0; z < 32; ++z) ir[z] = 0;
+    for (int z = 0; z < 16; ++z) dr[z] = 0.0;
+    ir[0] = t + 1; ir[1] = s; ir[2] = rec.lA; ir[3] = rec.lB; ir[4] = rec.skipped;
+    if (!rec.skipped) {
+      for (int z = 0; z < 7; ++z) {
+        ir[5 + z] = rec.st1[z]; ir[12 + z] = rec.st2[z];
+        dr[2 + z] = rec.m1[z]; dr[9 + z] = rec.m2[z];
+      }
+      dr[0] = rec.kd1; dr[1] = rec.kd2;
+    }
+  }
+}
+
+// perform_remapping over a batch of velocities (dvm_remap).
+__global__ void k_remap_batch(Grid g, long long n, const double* __restrict__ uvw, double f_off,
+                              int* __restrict__ st, double* __restrict__ m) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  Stencil s;
+  remap(g, uvw[3 * q], uvw[3 * q + 1], uvw[3 * q + 2], f_off, s);
+  int* o = st + 7 * q;
+  o[0] = s.oi; o[1] = s.oj; o[2] = s.ok; o[3] = s.a; o[4] = s.b; o[5] = s.c; o[6] = s.flag;
+  for (int z = 0; z < 7; ++z) m[7 * q + z] = s.m[z];
+}
+
+// ------------------------------------------------------------ convection
+// Diffuse-wall flux balance of each wall-adjacent cell
+// (boundary_conditions.f90:753-829 and the right/bottom/top analogues):
+// ratio = flux_in / flux_out with flux_in = sum over incoming nodes of
+// |v_n| * acc * f. blockIdx.y = side (0 left, 1 right, 2 bottom, 3 top).
+__global__ void __launch_bounds__(kMomBS) k_wall_ratio(Grid g, ConvParams p, const double* __restrict__ f,
+                                                       double* __restrict__ ratio) {
+  __shared__ double red[kMomBS / 64];
+  const int side = blockIdx.y, pos = blockIdx.x;
+  const int along = (side < 2) ? p.ny : p.nx;
+  if (pos >= along || !p.wall_side[side]) return;
+  const int ix = (side == 0) ? 0 : (side == 1 ? p.nx - 1 : pos);
+  const int iy = (side == 2) ? 0 : (side == 3 ? p.ny - 1 : pos);
+  const double* fc = f + (size_t)(ix + p.nx * iy) * g.N;
+  double s = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    bool in;
+    double vn;
+    if (side == 0) { in = i <= g.neg[0]; vn = g.ax[0][i]; }
+    else if (side == 1) { in = i >= g.pos[0]; vn = g.ax[0][i]; }
+    else if (side == 2) { in = j <= g.neg[1]; vn = g.ax[1][j]; }
+    else { in = j >= g.pos[1]; vn = g.ax[1][j]; }
+    if (in) s += fabs(vn) * p.acc * fc[l];
+  }
+  s = block_sum<kMomBS>(s, red);
+  if (threadIdx.x == 0) ratio[side * p.max_along + pos] = s / p.flux_out[side];
+}
+
+// Specular part of the wall ghosts (boundary_conditions.f90:832-858 and
+// analogues): (1-acc)*f at each incoming node, reflected and remapped onto the
+// ghost with apply_bc_remapping (:1852-1929). One lane per (incoming node).
+__global__ void k_spec_ghost(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ spec,
+                             int* __restrict__ err) {
+  const int side = blockIdx.z, pos = blockIdx.y;
+  const int along = (side < 2) ? p.ny : p.nx;
+  if (pos >= along || !p.wall_side[side]) return;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= g.N) return;
+  int i, j, k;
+  node_ijk(l, g.n[0], g.n[1], i, j, k);
+  bool in;
+  if (side == 0) in = i <= g.neg[0];
+  else if (side == 1) in = i >= g.pos[0];
+  else if (side == 2) in = j <= g.neg[1];
+  else in = j >= g.pos[1];
+  if (!in) return;
+  const int ix = (side == 0) ? 0 : (side == 1 ? p.nx - 1 : pos);
+  const int iy = (side == 2) ? 0 : (side == 3 ? p.ny - 1 : pos);
+  const double dens = (1.0 - p.acc) * f[(size_t)(ix + p.nx * iy) * g.N + l];
+  double x = g.ax[0][i], y = g.ax[1][j];
+  const double z = g.ax[2][k];
+  if (side < 2) x = -x; else y = -y;
+  Stencil s;
+  remap(g, x, y, z, 1.0, s);
+  if (s.flag == 0) { atomicOr(err, 2); return; }
+  double* gh = spec + ((size_t)side * p.max_along + pos) * g.N;
+  for (int q = 0; q < 7; ++q)
+    if (q < 4 || (s.flag & (1 << (q - 4)))) atomicAdd(&gh[stencil_node(s, q, g.n[0], g.nxy)], dens * s.m[q]);
+}
+
+__global__ void k_fill(double* __restrict__ a, long long n, double v) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    a[q] = v;
+}
+
+// Ghost value seen by cell (ix,iy) through `side` at node l.
+__device__ __forceinline__ double ghost(const ConvParams& p, const double* __restrict__ f, const double* __restrict__ wallM,
+                                        const double* __restrict__ ratio, const double* __restrict__ spec, int N,
+                                        int side, int ix, int iy, int l, size_t idx) {
+  const bool at_wall = (side == 0) ? ix == 0 : (side == 1) ? ix == p.nx - 1 : (side == 2) ? iy == 0 : iy == p.ny - 1;
+  if (!at_wall) {
+    const long long off = (side == 0) ? -(long long)N : (side == 1) ? (long long)N
+                        : (side == 2) ? -(long long)p.nx * N : (long long)p.nx * N;
+    return f[(long long)idx + off];
+  }
+  const int pos = (side < 2) ? iy : ix;
+  if (p.wall_side[side]) {
+    double gv = 0.0;
+    if (p.acc > 0.0) gv = ratio[side * p.max_along + pos] * wallM[(size_t)side * N + l];
+    if (p.acc < 1.0) gv += spec[((size_t)side * p.max_along + pos) * N + l];
+    return gv;
+  }
+  if (p.couette_side[side]) {
+    if (side == 0) return f[idx];
+    return f[(size_t)(0 + p.nx * iy) * N + l];
+  }
+  return 0.0;  // open side in 2-D trunk semantics: zero ghost
+}
+
+// 2-D upwind sweep: interior cells (convection.f90:119-276) and boundary cells
+// (boundary_conditions.f90:1548-1643) in one pass, one lane per (cell, node).
+__global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, const double* __restrict__ f,
+                                                       double* __restrict__ fo, const double* __restrict__ wallM,
+                                                       const double* __restrict__ ratio,
+                                                       const double* __restrict__ spec) {
+  const int N = g.N;
+  const size_t total = (size_t)p.nx * p.ny * N;
+  for (size_t idx = (size_t)blockIdx.x * kConvBS + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kConvBS) {
+    const int cell = (int)(idx / N);
+    const int l = (int)(idx - (size_t)cell * N);
+    const int ix = cell % p.nx, iy = cell / p.nx;
+    const int i = l % g.n[0], j = (l / g.n[0]) % g.n[1];
+    const double xv = g.ax[0][i], yv = g.ax[1][j];
+    const double f0 = f[idx];
+    const double cfl_x = p.fx * xv, cfl_y = p.fy * yv;
+    const bool boundary = ix == 0 || iy == 0 || ix == p.nx - 1 || iy == p.ny - 1;
+    if (!boundary) {
+      const double px = (xv < 0.0) ? f[idx + N] : f[idx - N];
+      const double py = (yv < 0.0) ? f[idx + (size_t)p.nx * N] : f[idx - (size_t)p.nx * N];
+      fo[idx] = f0 + fabs(cfl_x) * (px - f0) + fabs(cfl_y) * (py - f0);
+      continue;
+    }
+    // x region: 0 upwind from the right ghost, 1 no x term (zero plane), 2 from the left ghost
+    int xr = -1, yr = -1;
+    if (i <= g.neg[0]) xr = 0; else if (p.izflag && i == g.zero[0]) xr = 1; else if (i >= g.pos[0]) xr = 2;
+    if (j <= g.neg[1]) yr = 0; else if (p.jzflag && j == g.zero[1]) yr = 1; else if (j >= g.pos[1]) yr = 2;
+    if (xr < 0 || yr < 0) { fo[idx] = 0.0; continue; }
+    double out = f0;
+    if (xr != 1) {
+      const double gx = ghost(p, f, wallM, ratio, spec, N, xr == 0 ? 1 : 0, ix, iy, l, idx);
+      out = out + fabs(cfl_x) * (gx - f0);
+    }
+    if (yr != 1) {
+      const double gy = ghost(p, f, wallM, ratio, spec, N, yr == 0 ? 3 : 2, ix, iy, l, idx);
+      out = out + fabs(cfl_y) * (gy - f0);
+    }
+    fo[idx] = out;
+  }
+}
+
+// 1-D sweep of the zone-grid solver (grid_ref_code convection.f90:111-256)
+// with its in/out-flow (boundary_conditions.f90:783-913) and zero-gradient
+// (:915-973) boundary cells.
+__device__ __forceinline__ double upwind1d(const double* __restrict__ f, size_t idx, int N, double cfl, double xv) {
+  const double p0 = f[idx];
+  if (xv < 0.0) return p0 - cfl * (f[idx + N] - p0);
+  return p0 - cfl * (p0 - f[idx - N]);
+}
+
+__global__ void __launch_bounds__(kConvBS) k_convect1d(Grid g, ConvParams p, const double* __restrict__ f,
+                                                       double* __restrict__ fo, const double* __restrict__ inflow) {
+  const int N = g.N;
+  const size_t total = (size_t)p.nx * N;
+  for (size_t idx = (size_t)blockIdx.x * kConvBS + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kConvBS) {
+    const int ix = (int)(idx / N);
+    const int l = (int)(idx - (size_t)ix * N);
+    const double xv = g.ax[0][l % g.n[0]];
+    const double cfl = p.fx * xv;
+    if (ix > 0 && ix < p.nx - 1) { fo[idx] = upwind1d(f, idx, N, cfl, xv); continue; }
+    const int side = (ix == 0) ? 0 : 1;
+    const int bc = p.bc[side];
+    if (bc == DVM_BC_INOUT) fo[idx] = inflow[(size_t)side * N + l];
+    else if (bc == DVM_BC_ZEROGRAD) fo[idx] = upwind1d(f, (side == 0) ? idx + N : idx - N, N, cfl, xv);
+    else fo[idx] = f[idx];
+  }
+}
+
+}  // namespace dvm

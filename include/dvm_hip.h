@@ -1,0 +1,194 @@
+/*
+ * dvm_hip.h -- C-ABI of the MI355X (gfx950) DVM time step.
+ *
+ * The library replaces the hot path of the reference's serial Fortran time step:
+ *   collide  (variance-reduced Monte Carlo collision integral + conservative remap)
+ *   convect  (diffuse / specular / neighbour ghosts, boundary-cell and interior upwind sweep)
+ *   moments  (per-cell density, velocity, temperature)
+ * behind plain C entry points (no C++ or torch types), callable from Fortran
+ * through ISO_C_BINDING and from Python through ctypes.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative DVM_E_* code on
+ *     failure; dvm_last_error() then holds a message (the reference's own
+ *     convention is `write(*,*) ...; stop`, which the Fortran wrappers restore).
+ *   - Distributions are exchanged as one contiguous FP64 slab
+ *     f[cell][node], cell = ix + nx*iy (ix fastest, as phi(nx,ny) in Fortran),
+ *     node = i + nvx*(j + nvy*k) (i fastest, as value(i,j,k) in Fortran).
+ *   - Velocity-grid indices are 0-based on both sides of the boundary.
+ *   - One host thread per context; each context owns one HIP stream.
+ */
+#ifndef DVM_HIP_H
+#define DVM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define DVM_OK            0
+#define DVM_E_ARG        -1   /* invalid argument / configuration            */
+#define DVM_E_HIP        -2   /* HIP runtime error                           */
+#define DVM_E_NOMEM      -3   /* device allocation failed                    */
+#define DVM_E_STATE      -4   /* call out of order (e.g. collide before f)   */
+#define DVM_E_REMAP      -5   /* remap found no exterior point (ext flag 0)  */
+
+/* ---- enumerations (values equal the reference's deck integers) --------- */
+/* collision solver, collisions/collision_integral.f90:35-38 */
+#define DVM_COLL_NONE     0
+#define DVM_COLL_VR       3
+/* side boundary types, conditions/boundary_conditions.f90:56-59 */
+#define DVM_BC_WALL       1
+#define DVM_BC_INOUT      2
+#define DVM_BC_ZEROGRAD   3
+#define DVM_BC_COUETTE    4
+/* cross-section coefficient, solver/scaling.f90:34-36 */
+#define DVM_CS_CLARKE     1
+#define DVM_CS_BIRD       2
+#define DVM_CS_VKI        3
+/* physical-space dimension of the run */
+#define DVM_DIM_0D        0   /* space-homogeneous: one cell, no convection   */
+#define DVM_DIM_1D        1   /* 1-D in x, boundary semantics of the zone-grid 1-D solver */
+#define DVM_DIM_2D        2   /* 2-D in x,y, boundary semantics of the trunk solver       */
+/* collision arithmetic */
+#define DVM_PREC_FP64     0   /* everything FP64 (parity path)                */
+#define DVM_PREC_FP32     1   /* FP32 collision arithmetic, FP64 remap weights,
+                                 moments and conservation correction          */
+/* sample-count rule */
+#define DVM_SAMPLES_REFERENCE 0 /* nint(dt*T^(2/3)*nA*nB/(2 Kn rms^2 beta3)) per integral */
+#define DVM_SAMPLES_EXPLICIT  1 /* samples_per_cell split over the 3 integrals
+                                   in the reference's nA*nB proportions       */
+
+/* ---- per-cell property record returned by dvm_moments ------------------ */
+#define DVM_P_DENS        0   /* number density                 */
+#define DVM_P_U           1   /* x velocity                     */
+#define DVM_P_V           2   /* y velocity                     */
+#define DVM_P_W           3   /* z velocity                     */
+#define DVM_P_TRTEMP      4   /* translational temperature      */
+#define DVM_P_TEMP        5   /* species (overall) temperature  */
+#define DVM_P_PRESSURE    6   /* 0.5 * n * T                    */
+#define DVM_P_AVGSPEED    7   /* sum f |v| / n                  */
+#define DVM_NPROP         8
+
+/* Wall record indices: wall[side][...] with side 0 left, 1 right, 2 bottom, 3 top */
+#define DVM_W_DENS 0
+#define DVM_W_U    1
+#define DVM_W_V    2
+#define DVM_W_W    3
+#define DVM_W_TEMP 4
+
+typedef struct dvm_config {
+  /* physical space (physical_grid.f90:42-55) */
+  int32_t dim;              /* DVM_DIM_*                                        */
+  int32_t nx, ny;           /* nx_space, ny_space (ny = 1 for 0-D and 1-D)      */
+  double  dx, dy;
+  /* velocity grid axes, copied at create (velocity_grid.f90:271-350) */
+  int32_t nvx, nvy, nvz;
+  const double* vx;         /* [nvx] ascending                                  */
+  const double* vy;         /* [nvy]                                            */
+  const double* vz;         /* [nvz]                                            */
+  /* time stepping and scaling */
+  double  dt;               /* get_deltat                                       */
+  double  kn;               /* get_knudsen_number                               */
+  /* species (single monatomic species; species_data.f90:46-78) */
+  double  mass, diam, omega, alpha, viscosity, t_viscosity;
+  int32_t csection_coeff;   /* DVM_CS_*                                         */
+  /* collision integral */
+  int32_t coll_solver;      /* DVM_COLL_NONE or DVM_COLL_VR                     */
+  double  cdf_tolerance;    /* set_cdf_tolerance                                */
+  int32_t num_repl;         /* set_num_repl_locations (NRepl)                   */
+  double  coln_rms;         /* ColnRMS                                          */
+  int32_t sample_mode;      /* DVM_SAMPLES_*                                    */
+  int64_t samples_per_cell; /* for DVM_SAMPLES_EXPLICIT                         */
+  uint64_t seed;            /* Philox4x32-10 key                                */
+  int32_t precision;        /* DVM_PREC_*                                       */
+  /* convection and boundaries */
+  int32_t enable_convection;
+  int32_t bc_left, bc_right;/* DVM_BC_* (bottom/top are walls in 2-D, as trunk) */
+  double  acc_coeff;        /* accommodation: 1 diffuse, 0 specular             */
+  double  wall[4][5];       /* [side][DVM_W_*]                                  */
+} dvm_config;
+
+typedef struct dvm_ctx dvm_ctx;
+
+/* Creates a context on the current HIP device and allocates the resident
+   distribution, CDF and moment buffers.
+   Replaces: initialize_solver (solver/dvm_solver.f90:92-229) allocation of
+   phi/delta_phi/properties, create_velocity_grid (grid/velocity_grid.f90:271),
+   create_collision_integral / create_variance_reduction
+   (collisions/collision_integral.f90:98, collisions/variance_reduction.f90:71). */
+int dvm_ctx_create(const dvm_config* cfg, dvm_ctx** out);
+
+/* Frees every device buffer of the context.
+   Replaces: destroy_solver (solver/dvm_solver.f90:371-414). */
+void dvm_ctx_destroy(dvm_ctx* ctx);
+
+/* Message of the last failing call on this context (or of the last failing
+   dvm_ctx_create when ctx is NULL). */
+const char* dvm_last_error(const dvm_ctx* ctx);
+
+/* Host -> device copy of f[ncell][nnode]; also resets the equilibrium cache
+   as create_variance_reduction does (variance_reduction.f90:119-125).
+   Replaces: the host-side phi(:,:,:)%value fill of initialize_dist_func
+   (conditions/initial_conditions.f90:212). */
+int dvm_upload_f(dvm_ctx* ctx, int32_t species, const double* f);
+
+/* Device -> host copy of f[ncell][nnode]. */
+int dvm_download_f(dvm_ctx* ctx, int32_t species, double* f);
+
+/* Per-cell moments of the resident f, kept on the device for the next
+   collide call and copied to props_out[ncell][DVM_NPROP] when non-NULL.
+   Replaces: calculate_physical_properties (dist_funcs/physical_props.f90:285)
+   called per cell at solver/dvm_solver.f90:277,336. */
+int dvm_moments(dvm_ctx* ctx, double* props_out);
+
+/* One collision phase over every cell at time step ntime (>= 1): CDF build with
+   the cached equilibrium, the three sampled VR integrals, phi := delta_phi.
+   samples_out[ncell] (may be NULL) receives tot_colls per cell.
+   Replaces: compute_collision_integral (collisions/collision_integral.f90:163)
+   called per cell at solver/dvm_solver.f90:305-315. */
+int dvm_collide(dvm_ctx* ctx, int32_t ntime, int64_t* samples_out);
+
+/* One convection phase: boundary ghosts, boundary-cell and interior upwind
+   update, phi := phi_conv.
+   Replaces: compute_convection (convection/convection.f90:69) called at
+   solver/dvm_solver.f90:318-329 (2-D) and the 1-D first_order_scheme +
+   apply_boundary_conditions of the zone-grid solver
+   (code_versions/grid_ref_code/src/convection/convection.f90:111-256). */
+int dvm_convect(dvm_ctx* ctx, int32_t species, int32_t ntime);
+
+/* Fused step: collide -> convect -> moments, the body of the time loop
+   (solver/dvm_solver.f90:299-357) without its screen and file output. */
+int dvm_step(dvm_ctx* ctx, int32_t ntime);
+
+/* Remaps n off-grid velocities uvw[n][3] with weight f_off onto the context's
+   velocity grid on the device: stencil[n][7] = (o_i,o_j,o_k,a,b,c,flag),
+   mass[n][7] = (f_o,f_ix,f_iy,f_iz,f_ex,f_ey,f_ez).
+   Replaces: perform_remapping (collisions/remapping.f90:55-90). */
+int dvm_remap(dvm_ctx* ctx, int64_t n, const double* uvw, double f_off,
+              int32_t* stencil, double* mass);
+
+/* Parity trace of one cell's collision phase at time step ntime, computed on
+   the device from the resident f without modifying it. For each sample s of
+   integral t (t = 1..3, in order) one record of 32 int32:
+     [0] integral, [1] sample index, [2] node A, [3] node B, [4] skipped (g<1e-13),
+     [5..11] stencil of the first post-collision velocity (repl location 1),
+     [12..18] stencil of the second, [19..31] zero (reserved),
+   and 16 doubles: [0] kin_depl A, [1] kin_depl B, [2..8] mass of remap 1,
+   [9..15] mass of remap 2. max_records bounds the output; n_out receives the
+   number of samples of the cell. */
+int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_records,
+                      int32_t* irec, double* drec, int64_t* n_out);
+
+/* Device time of the most recent collide / convect / moments phases in ms
+   (HIP events on the context's stream); any pointer may be NULL. */
+int dvm_last_timings(const dvm_ctx* ctx, double* collide_ms, double* convect_ms,
+                     double* moments_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DVM_HIP_H */

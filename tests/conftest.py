@@ -1,0 +1,24 @@
+import importlib
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def pkg():
+    return importlib.import_module("2d_dvm_microflows_amd")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from tests._oracle import load_oracle
+    return load_oracle()

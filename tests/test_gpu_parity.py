@@ -1,0 +1,160 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle on the same
+seeded inputs.
+
+Bars: integer outputs (sampled nodes, remap stencils, sample counts) equal;
+remap weights equal bit for bit (only +,-,*,/ and compares, correctly rounded
+on both sides); everything that passes through exp/sin/cos/pow or through a
+reassociated sum (prefix scans, atomics, reductions) within 1e-12 relative,
+norm-wise per cell: max|a-b| <= 1e-12 * max|b|.
+"""
+import numpy as np
+import pytest
+
+from tests._oracle import golden_cases, oracle_solver
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def close(a, b, rtol=RTOL):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    err = np.max(np.abs(a - b)) if b.size else 0.0
+    return err <= rtol * max(scale, 1e-300), err, scale
+
+
+def cellwise_close(a, b, rtol=RTOL):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    err = np.max(np.abs(a - b), axis=1)
+    scale = np.max(np.abs(b), axis=1)
+    bad = np.nonzero(err > rtol * np.maximum(scale, 1e-300))[0]
+    return bad.size == 0, bad, float(np.max(err / np.maximum(scale, 1e-300)))
+
+
+@pytest.mark.parametrize("name", ["u25", "u18", "u357", "nonuni", "shift"])
+def test_remap_bit_exact_vs_reference_vectors(pkg, name):
+    case = golden_cases("remap_grid.npz")[name]
+    cfg = pkg.SolverConfig(vx=case["ax"], vy=case["ay"], vz=case["az"], samples_per_cell=0)
+    print(f"[remap] grid {name} n={case['n'].tolist()} velocities={len(case['uvw'])}", flush=True)
+    with pkg.Solver(cfg) as s:
+        st, m = s.remap(case["uvw"], float(case["f_off"]))
+    assert np.array_equal(st, case["stencil"])
+    assert np.array_equal(m.view(np.int64), case["mass"].view(np.int64))
+
+
+def _pair(pkg, cfg, f):
+    gpu = pkg.Solver(cfg)
+    orc = oracle_solver(cfg)
+    pg = gpu.initialize(f)
+    po = orc.initialize(f)
+    return gpu, orc, pg, po
+
+
+def test_moments_match_oracle(pkg):
+    rng = np.random.default_rng(1)
+    cfg, _ = pkg.lid_driven_cavity(nx=5, ny=4, n_v=9, vmax=3.0, samples=100)
+    f = rng.random((cfg.nx * cfg.ny, cfg.num_points)) * 0.01
+    f[3] = 0.0  # an empty cell: all moments zero
+    print("[moments] 5x4 cells, 9^3", flush=True)
+    gpu, orc, pg, po = _pair(pkg, cfg, f)
+    ok, err, scale = close(pg, po, 1e-13)
+    assert ok, (err, scale)
+
+
+TRACE_CASES = {
+    # name: (builder kwargs, config overrides)
+    "hs_21": (dict(n_v=21, samples=6000), {}),
+    "hs_11_nrepl2": (dict(n_v=11, samples=3000), dict(num_repl=2)),
+    "vss_ar": (dict(n_v=13, samples=3000), dict(omega=0.81, alpha=1.40)),
+    "refcount": (dict(n_v=15, samples=0), dict(sample_mode=0, coln_rms=0.02, cdf_tolerance=0.01)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(TRACE_CASES))
+def test_collision_trace_matches_oracle(pkg, case):
+    kw, over = TRACE_CASES[case]
+    cfg, f = pkg.homogeneous_relaxation(**kw)
+    cfg = cfg.with_(**over)
+    print(f"[trace] {case} {cfg.name} overrides={over}", flush=True)
+    gpu, orc, _, _ = _pair(pkg, cfg, f)
+    ig, dg, ng = gpu.collide_trace(1, 0, 200000)
+    io, do, no = orc.collide_trace(1, 0, 200000)
+    assert ng == no and ng > 0
+    # sampled integral, index, nodes, skip flag and both remap stencils
+    assert np.array_equal(ig[:, :19], io[:, :19])
+    ok, err, scale = close(dg, do)
+    assert ok, (err, scale)
+
+
+STEP_CASES = {
+    "homogeneous": lambda pkg: pkg.homogeneous_relaxation(n_v=15, samples=20000),
+    "homogeneous_vss_nrepl3": lambda pkg: (lambda c, f: (c.with_(omega=0.81, alpha=1.4, num_repl=3), f))(
+        *pkg.homogeneous_relaxation(n_v=11, samples=5000)),
+    "cavity_diffuse": lambda pkg: pkg.lid_driven_cavity(nx=6, ny=5, n_v=9, vmax=3.0, samples=800),
+    "cavity_partial_specular": lambda pkg: (lambda c, f: (c.with_(acc_coeff=0.6), f))(
+        *pkg.lid_driven_cavity(nx=5, ny=6, n_v=10, vmax=3.0, samples=500)),
+    "cavity_couette": lambda pkg: (lambda c, f: (c.with_(bc_left=4, bc_right=4), f))(
+        *pkg.lid_driven_cavity(nx=4, ny=4, n_v=9, vmax=3.0, samples=300)),
+    "shock_1d": lambda pkg: pkg.normal_shock_1d(nx=12, n_v=11, vmax=6.0, samples=1500),
+    "shock_1d_zerograd": lambda pkg: (lambda c, f: (c.with_(bc_right=3), f))(
+        *pkg.normal_shock_1d(nx=10, n_v=9, vmax=6.0, samples=800)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(STEP_CASES))
+def test_time_steps_match_oracle(pkg, case):
+    cfg, f = STEP_CASES[case](pkg)
+    print(f"[steps] {case}: {cfg.name}", flush=True)
+    gpu, orc, pg, po = _pair(pkg, cfg, f)
+    for n in range(1, 4):
+        sg = gpu.compute_collision_integral(n)
+        so = orc.compute_collision_integral(n)
+        assert np.array_equal(sg, so), "sample counts differ"
+        ok, bad, worst = cellwise_close(gpu.download_f(), orc.download_f())
+        assert ok, f"step {n} collide: cells {bad[:10]} worst {worst:.3e}"
+        gpu.compute_convection(n)
+        orc.compute_convection(n)
+        ok, bad, worst = cellwise_close(gpu.download_f(), orc.download_f())
+        assert ok, f"step {n} convect: cells {bad[:10]} worst {worst:.3e}"
+        pg = gpu.calculate_physical_properties()
+        po = orc.calculate_physical_properties()
+        ok, err, scale = close(pg, po)
+        assert ok, (n, err, scale)
+
+
+def test_fused_step_equals_phases(pkg):
+    cfg, f = pkg.lid_driven_cavity(nx=5, ny=5, n_v=9, vmax=3.0, samples=400)
+    print("[fused] cavity 5x5 9^3", flush=True)
+    a = pkg.Solver(cfg)
+    b = pkg.Solver(cfg)
+    a.initialize(f)
+    b.initialize(f)
+    for n in range(1, 4):
+        a.step(n)
+        b.compute_collision_integral(n)
+        b.compute_convection(n)
+        b.calculate_physical_properties()
+    ok, bad, worst = cellwise_close(a.download_f(), b.download_f(), 1e-14)
+    assert ok, worst
+
+
+def test_collision_conserves_mass_momentum_energy(pkg):
+    """Each sample depletes and replenishes equal weight with a remap that
+    keeps moments 0-2: per-cell n, n*u and energy are unchanged to round-off."""
+    cfg, f = pkg.homogeneous_relaxation(n_v=21, samples=50000)
+    print("[conservation] homogeneous 21^3, 50k samples", flush=True)
+    v = pkg.node_velocities(cfg)
+    with pkg.Solver(cfg) as s:
+        s.initialize(f)
+        for n in range(1, 6):
+            before = s.download_f()[0]
+            s.compute_collision_integral(n)
+            after = s.download_f()[0]
+            s.calculate_physical_properties()
+            for w in (np.ones(len(v)), v[:, 0], v[:, 1], v[:, 2], (v ** 2).sum(1)):
+                d = abs(np.dot(w, after) - np.dot(w, before))
+                scale = np.dot(np.abs(w), np.abs(after - before)) + 1e-300
+                assert d <= 1e-12 * max(scale, np.dot(np.abs(w), before)), (n, d, scale)
