@@ -15,7 +15,13 @@ struct dvm_ctx {
   dvm_config cfg;
   int ncell = 0, N = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[6] = {};
+  // phase events: per slot, boundaries of (cdf, collide, walls, sweep, moments)
+  static constexpr int kSlots = 16;
+  hipEvent_t ev[kSlots][6] = {};
+  bool slot_used[kSlots] = {};
+  int slot = 0;
+  double phase_ms[5] = {0, 0, 0, 0, 0};
+  long long timed_steps = 0;
   // host copy of the grid
   std::vector<double> ax[3], beta[3];
   Grid g{};                      // device pointers
@@ -117,8 +123,8 @@ static int validate(const dvm_config* cfg) {
   if (cfg->num_repl < 1) return fail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
   if (cfg->coll_solver != DVM_COLL_NONE && cfg->coll_solver != DVM_COLL_VR)
     return fail(nullptr, DVM_E_ARG, "coll_solver must be 0 (none) or 3 (variance reduction)");
-  if (cfg->precision != DVM_PREC_FP64 && cfg->precision != DVM_PREC_FP32)
-    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
+  if (cfg->precision != DVM_PREC_FP64)
+    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 (the FP32-collision kernels are not built)");
   if (cfg->sample_mode != DVM_SAMPLES_REFERENCE && cfg->sample_mode != DVM_SAMPLES_EXPLICIT)
     return fail(nullptr, DVM_E_ARG, "invalid sample_mode");
   if (cfg->sample_mode == DVM_SAMPLES_REFERENCE && !(cfg->coln_rms > 0.0))
@@ -140,8 +146,9 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
                   c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto& e : c->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& row : c->ev)
+    for (auto& e : row)
+      if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -154,7 +161,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   c->N = n[0] * n[1] * n[2];
   const int N = c->N;
   HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& e : c->ev) HIP_TRY(c, hipEventCreate(&e));
+  for (auto& row : c->ev)
+    for (auto& e : row) HIP_TRY(c, hipEventCreate(&e));
   // ---- grid (velocity_grid.f90:271-597)
   std::vector<double> packed;
   for (int d = 0; d < 3; ++d) {
@@ -363,6 +371,33 @@ extern "C" int dvm_download_f(dvm_ctx* c, int32_t species, double* f) {
   return DVM_OK;
 }
 
+// ---- phase timing: a ring of event slots, one slot per entry-point call
+static int timing_drain_slot(dvm_ctx* c, int k) {
+  if (!c->slot_used[k]) return DVM_OK;
+  HIP_TRY(c, hipEventSynchronize(c->ev[k][5]));
+  for (int q = 0; q < 5; ++q) {
+    float ms = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[k][q], c->ev[k][q + 1]));
+    c->phase_ms[q] += ms;
+  }
+  c->slot_used[k] = false;
+  c->timed_steps++;
+  return DVM_OK;
+}
+
+static int begin_slot(dvm_ctx* c) { return timing_drain_slot(c, c->slot); }
+
+static int mark(dvm_ctx* c, int q) {
+  HIP_TRY(c, hipEventRecord(c->ev[c->slot][q], c->stream));
+  return DVM_OK;
+}
+
+static void end_slot(dvm_ctx* c) {
+  c->slot_used[c->slot] = true;
+  c->slot = (c->slot + 1) % dvm_ctx::kSlots;
+}
+
+// ---- phases
 static int moments_async(dvm_ctx* c) {
   hipLaunchKernelGGL(k_moments, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass);
   HIP_TRY(c, hipGetLastError());
@@ -370,30 +405,16 @@ static int moments_async(dvm_ctx* c) {
   return DVM_OK;
 }
 
-extern "C" int dvm_moments(dvm_ctx* c, double* props_out) {
-  if (!c) return fail(c, DVM_E_ARG, "null context");
-  if (!c->f_valid) return fail(c, DVM_E_STATE, "moments before upload");
-  HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
-  if (int rc = moments_async(c)) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
-  if (props_out) {
-    HIP_TRY(c, hipMemcpyAsync(props_out, c->props, (size_t)c->ncell * DVM_NPROP * sizeof(double),
-                              hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-  }
-  return DVM_OK;
-}
-
 static int cdf_async(dvm_ctx* c) {
+  if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
   hipLaunchKernelGGL(k_cdf, dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f, c->props, c->eqv,
                      c->eqc, c->neqc, c->eqtab, c->neqtab, c->eqpar, c->counts, c->factors);
   HIP_TRY(c, hipGetLastError());
   return DVM_OK;
 }
 
-static int collide_async(dvm_ctx* c, int ntime) {
+static int sample_async(dvm_ctx* c, int ntime) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
-  if (int rc = cdf_async(c)) return rc;
   if (c->lds_collide) {
     hipLaunchKernelGGL(k_collide<true>, dim3(c->ncell), dim3(kCollBS), c->coll_lds, c->stream, c->g, c->collp, ntime,
                        c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors, c->err);
@@ -409,13 +430,76 @@ static int collide_async(dvm_ctx* c, int ntime) {
   return DVM_OK;
 }
 
+static bool convects(const dvm_ctx* c) { return c->cfg.enable_convection && c->cfg.dim != DVM_DIM_0D; }
+
+static int walls_async(dvm_ctx* c) {
+  if (!convects(c) || c->cfg.dim != DVM_DIM_2D) return DVM_OK;
+  ConvParams& p = c->convp;
+  const int N = c->N;
+  if (p.acc > 0.0)
+    hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, c->stream, c->g, p, c->f, c->ratio);
+  if (p.acc < 1.0) {
+    const long long ns = (long long)4 * p.max_along * N;
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->spec, ns, 0.0);
+    hipLaunchKernelGGL(k_spec_ghost, dim3((N + 255) / 256, p.max_along, 4), dim3(256), 0, c->stream, c->g, p, c->f,
+                       c->spec, c->err);
+  }
+  HIP_TRY(c, hipGetLastError());
+  return DVM_OK;
+}
+
+static int sweep_async(dvm_ctx* c) {
+  if (!convects(c)) return DVM_OK;
+  const size_t total = (size_t)c->ncell * c->N;
+  const unsigned blocks = (unsigned)((total + kConvBS - 1) / kConvBS);
+  if (c->cfg.dim == DVM_DIM_2D)
+    hipLaunchKernelGGL(k_convect2d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd, c->wallM,
+                       c->ratio, c->spec);
+  else
+    hipLaunchKernelGGL(k_convect1d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd, c->inflow);
+  HIP_TRY(c, hipGetLastError());
+  double* t = c->f; c->f = c->fd; c->fd = t;  // phi := phi_conv
+  return DVM_OK;
+}
+
+// Runs the phases [first, last] of the step (0 cdf, 1 collide, 2 walls,
+// 3 sweep, 4 moments) with timing marks around each.
+static int run_phases(dvm_ctx* c, int ntime, int first, int last) {
+  if (int rc = begin_slot(c)) return rc;
+  int rc = DVM_OK;
+  for (int q = 0; q < 5 && rc == DVM_OK; ++q) {
+    if ((rc = mark(c, q))) break;
+    if (q < first || q > last) continue;
+    switch (q) {
+      case 0: rc = cdf_async(c); break;
+      case 1: rc = sample_async(c, ntime); break;
+      case 2: rc = walls_async(c); break;
+      case 3: rc = sweep_async(c); break;
+      case 4: rc = moments_async(c); break;
+    }
+  }
+  if (rc == DVM_OK) rc = mark(c, 5);
+  if (rc == DVM_OK) end_slot(c);
+  return rc;
+}
+
+extern "C" int dvm_moments(dvm_ctx* c, double* props_out) {
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "moments before upload");
+  if (int rc = run_phases(c, 0, 4, 4)) return rc;
+  if (props_out) {
+    HIP_TRY(c, hipMemcpyAsync(props_out, c->props, (size_t)c->ncell * DVM_NPROP * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return DVM_OK;
+}
+
 extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "collide needs f and its moments");
   if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
-  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-  if (int rc = collide_async(c, ntime)) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  if (int rc = run_phases(c, ntime, 0, 1)) return rc;
   if (samples_out) {
     std::vector<int> h((size_t)c->ncell * 4, 0);
     if (c->cfg.coll_solver == DVM_COLL_VR)
@@ -426,40 +510,11 @@ extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
   return check_err(c);
 }
 
-static int convect_async(dvm_ctx* c) {
-  const dvm_config& cfg = c->cfg;
-  if (!cfg.enable_convection || cfg.dim == DVM_DIM_0D) return DVM_OK;
-  const int N = c->N;
-  const size_t total = (size_t)c->ncell * N;
-  const unsigned blocks = (unsigned)((total + kConvBS - 1) / kConvBS);
-  if (cfg.dim == DVM_DIM_2D) {
-    ConvParams& p = c->convp;
-    if (p.acc > 0.0)
-      hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, c->stream, c->g, p, c->f, c->ratio);
-    if (p.acc < 1.0) {
-      const long long ns = (long long)4 * p.max_along * N;
-      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->spec, ns, 0.0);
-      hipLaunchKernelGGL(k_spec_ghost, dim3((N + 255) / 256, p.max_along, 4), dim3(256), 0, c->stream, c->g, p, c->f,
-                         c->spec, c->err);
-    }
-    hipLaunchKernelGGL(k_convect2d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, p, c->f, c->fd, c->wallM,
-                       c->ratio, c->spec);
-  } else {
-    hipLaunchKernelGGL(k_convect1d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd, c->inflow);
-  }
-  HIP_TRY(c, hipGetLastError());
-  double* t = c->f; c->f = c->fd; c->fd = t;  // phi := phi_conv
-  return DVM_OK;
-}
-
 extern "C" int dvm_convect(dvm_ctx* c, int32_t species, int32_t ntime) {
-  (void)ntime;
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (species != 0) return fail(c, DVM_E_ARG, "only species 0 is supported");
   if (!c->f_valid) return fail(c, DVM_E_STATE, "convect before upload");
-  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
-  if (int rc = convect_async(c)) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  if (int rc = run_phases(c, ntime, 2, 3)) return rc;
   return check_err(c);
 }
 
@@ -467,16 +522,26 @@ extern "C" int dvm_step(dvm_ctx* c, int32_t ntime) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "step needs f and its moments");
   if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
-  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-  if (int rc = collide_async(c, ntime)) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
-  if (int rc = convect_async(c)) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
-  if (int rc = moments_async(c)) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
-  return DVM_OK;
+  return run_phases(c, ntime, 0, 4);
+}
+
+extern "C" int dvm_timing_reset(dvm_ctx* c) {
+  if (!c) return DVM_E_ARG;
+  for (int k = 0; k < dvm_ctx::kSlots; ++k)
+    if (int rc = timing_drain_slot(c, k)) return rc;
+  for (double& v : c->phase_ms) v = 0.0;
+  c->timed_steps = 0;
+  return check_err(c);
+}
+
+extern "C" int dvm_timing_read(dvm_ctx* c, double* phase_ms, int64_t* calls) {
+  if (!c) return DVM_E_ARG;
+  for (int k = 0; k < dvm_ctx::kSlots; ++k)
+    if (int rc = timing_drain_slot(c, k)) return rc;
+  if (phase_ms)
+    for (int q = 0; q < 5; ++q) phase_ms[q] = c->phase_ms[q];
+  if (calls) *calls = c->timed_steps;
+  return check_err(c);
 }
 
 extern "C" int dvm_remap(dvm_ctx* c, int64_t n, const double* uvw, double f_off, int32_t* stencil, double* mass) {
@@ -546,11 +611,3 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
   return DVM_OK;
 }
 
-extern "C" int dvm_last_timings(const dvm_ctx* c, double* collide_ms, double* convect_ms, double* moments_ms) {
-  if (!c) return DVM_E_ARG;
-  float ms = 0.0f;
-  if (collide_ms) { *collide_ms = (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) ? ms : -1.0; }
-  if (convect_ms) { *convect_ms = (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) ? ms : -1.0; }
-  if (moments_ms) { *moments_ms = (hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) ? ms : -1.0; }
-  return DVM_OK;
-}

@@ -284,6 +284,7 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
                           : referenced_search(cv.neqc, N, cv.neqtab, uA);
   const int lB = (t == 1) ? referenced_search(cv.eqc, N, cv.eqtab, uB)
                           : referenced_search(cv.neqc, N, cv.neqtab, uB);
+  if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) return 1;  // unreachable for finite CDFs
   int iA, jA, kA, iB, jB, kB;
   node_ijk(lA, nx, g.n[1], iA, jA, kA);
   node_ijk(lB, nx, g.n[1], iB, jB, kB);
@@ -418,10 +419,14 @@ __global__ void __launch_bounds__(kCollBS) k_collide(Grid g, CollParams cp, int 
     const double factor = factors[cell * 3 + t];
     if (kLds) {
       bad |= vr_sample<false>(g, cp, cv, cell, ntime, t, s, factor,
-                              [&](int node, double val) { atomicAdd(&acc[node], val); }, nullptr);
+                              [&](int node, double val) {
+                                if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], val); else bad = 1;
+                              }, nullptr);
     } else {
       bad |= vr_sample<false>(g, cp, cv, cell, ntime, t, s, factor,
-                              [&](int node, double val) { atomicAdd(&gd[node], val); }, nullptr);
+                              [&](int node, double val) {
+                                if ((unsigned)node < (unsigned)N) atomicAdd(&gd[node], val); else bad = 1;
+                              }, nullptr);
     }
   }
   if (bad) atomicOr(err, 1);
@@ -542,8 +547,12 @@ __global__ void k_spec_ghost(Grid g, ConvParams p, const double* __restrict__ f,
   remap(g, x, y, z, 1.0, s);
   if (s.flag == 0) { atomicOr(err, 2); return; }
   double* gh = spec + ((size_t)side * p.max_along + pos) * g.N;
-  for (int q = 0; q < 7; ++q)
-    if (q < 4 || (s.flag & (1 << (q - 4)))) atomicAdd(&gh[stencil_node(s, q, g.n[0], g.nxy)], dens * s.m[q]);
+  for (int q = 0; q < 7; ++q) {
+    if (q < 4 || (s.flag & (1 << (q - 4)))) {
+      const int node = stencil_node(s, q, g.n[0], g.nxy);
+      if ((unsigned)node < (unsigned)g.N) atomicAdd(&gh[node], dens * s.m[q]); else atomicOr(err, 4);
+    }
+  }
 }
 
 __global__ void k_fill(double* __restrict__ a, long long n, double v) {

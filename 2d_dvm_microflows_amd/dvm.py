@@ -34,9 +34,10 @@ def load_library(path=None):
         raise RuntimeError(
             f"HIP library {p} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
     lib = capi.declare(C.CDLL(p), "dvm")
-    lib.dvm_last_timings.restype = C.c_int
-    lib.dvm_last_timings.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                     C.POINTER(C.c_double)]
+    lib.dvm_timing_reset.restype = C.c_int
+    lib.dvm_timing_reset.argtypes = [C.c_void_p]
+    lib.dvm_timing_read.restype = C.c_int
+    lib.dvm_timing_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     if path is None:
         _LIB = lib
     return lib
@@ -171,8 +172,20 @@ class Solver:
         k = min(n.value, max_records)
         return irec[:k], drec[:k], n.value
 
-    def timings(self):
-        """Device milliseconds of the last collide / convect / moments phases."""
-        a, b, c = C.c_double(), C.c_double(), C.c_double()
-        self.lib.dvm_last_timings(self.h, C.byref(a), C.byref(b), C.byref(c))
-        return a.value, b.value, c.value
+    PHASES = ("cdf", "collide", "walls", "sweep", "moments")
+
+    def timing_reset(self):
+        """Wait for queued work and zero the per-phase device-time totals."""
+        self._check(self.lib.dvm_timing_reset(self.h))
+
+    def timing_read(self):
+        """Per-phase device ms since the last reset (HIP events on the
+        context's stream) and the number of calls they cover."""
+        ms = (C.c_double * 5)()
+        n = C.c_int64(0)
+        self._check(self.lib.dvm_timing_read(self.h, ms, C.byref(n)))
+        return dict(zip(self.PHASES, list(ms))), n.value
+
+    def synchronize(self):
+        """Wait for the context's queued work (drains the timing ring)."""
+        self.timing_read()

@@ -182,10 +182,17 @@ int dvm_remap(dvm_ctx* ctx, int64_t n, const double* uvw, double f_off,
 int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_records,
                       int32_t* irec, double* drec, int64_t* n_out);
 
-/* Device time of the most recent collide / convect / moments phases in ms
-   (HIP events on the context's stream); any pointer may be NULL. */
-int dvm_last_timings(const dvm_ctx* ctx, double* collide_ms, double* convect_ms,
-                     double* moments_ms);
+/* Device time per phase, measured with HIP events on the context's stream
+   around every phase that dvm_step / dvm_collide / dvm_convect / dvm_moments
+   enqueue. Phases: 0 CDF build, 1 collision sampling, 2 wall ghosts,
+   3 upwind sweep, 4 moments. dvm_timing_reset waits for pending work and
+   zeroes the totals; dvm_timing_read waits for pending work and returns the
+   totals in ms since the last reset and the number of calls they cover.
+   (No counterpart in the reference, which times phases with cpu_time at
+   solver/dvm_solver.f90:294-366.) */
+#define DVM_NPHASE 5
+int dvm_timing_reset(dvm_ctx* ctx);
+int dvm_timing_read(dvm_ctx* ctx, double* phase_ms /* [DVM_NPHASE] */, int64_t* calls);
 
 #ifdef __cplusplus
 }
