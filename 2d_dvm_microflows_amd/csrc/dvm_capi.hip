@@ -5,8 +5,10 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 using namespace dvm;
@@ -38,6 +40,8 @@ struct dvm_ctx {
   ConvParams convp{};
   bool lds_collide = false;
   size_t coll_lds = 0;
+  int coll_bs = 512;             // lanes per collision workgroup (256, 512 or 1024)
+  float* dacc = nullptr;         // [C][N] FP32 collision change (FP32 path)
   bool f_valid = false, props_valid = false;
   int trace_ready = -1;
   char msg[512] = {0};
@@ -123,8 +127,8 @@ static int validate(const dvm_config* cfg) {
   if (cfg->num_repl < 1) return fail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
   if (cfg->coll_solver != DVM_COLL_NONE && cfg->coll_solver != DVM_COLL_VR)
     return fail(nullptr, DVM_E_ARG, "coll_solver must be 0 (none) or 3 (variance reduction)");
-  if (cfg->precision != DVM_PREC_FP64)
-    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 (the FP32-collision kernels are not built)");
+  if (cfg->precision != DVM_PREC_FP64 && cfg->precision != DVM_PREC_FP32)
+    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
   if (cfg->sample_mode != DVM_SAMPLES_REFERENCE && cfg->sample_mode != DVM_SAMPLES_EXPLICIT)
     return fail(nullptr, DVM_E_ARG, "invalid sample_mode");
   if (cfg->sample_mode == DVM_SAMPLES_REFERENCE && !(cfg->coln_rms > 0.0))
@@ -143,7 +147,7 @@ extern "C" const char* dvm_last_error(const dvm_ctx* ctx) { return ctx ? ctx->ms
 extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_axes, c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->eqpar, c->props,
-                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow};
+                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& row : c->ev)
@@ -151,6 +155,21 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
       if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+template <int BS>
+static const void* collide_kernel_bs(bool lds, bool f32) {
+  if (!lds) return (const void*)k_collide<false, false, BS>;
+  return f32 ? (const void*)k_collide<true, true, BS> : (const void*)k_collide<true, false, BS>;
+}
+
+static const void* collide_kernel(const dvm_ctx* c) {
+  const bool f32 = c->cfg.precision == DVM_PREC_FP32;
+  switch (c->coll_bs) {
+    case 256: return collide_kernel_bs<256>(c->lds_collide, f32);
+    case 1024: return collide_kernel_bs<1024>(c->lds_collide, f32);
+    default: return collide_kernel_bs<512>(c->lds_collide, f32);
+  }
 }
 
 static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
@@ -238,13 +257,23 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   HIP_TRY(c, hipMemset(c->err, 0, 4 * sizeof(int)));
   HIP_TRY(c, hipMemset(c->counts, 0, (size_t)c->ncell * 4 * sizeof(int)));
   // collision accumulator placement: the cell's slice in LDS when it fits
-  c->coll_lds = (size_t)(kAxisSlots + N) * sizeof(double);
+  const bool f32 = cfg->precision == DVM_PREC_FP32;
+  if (const char* e = getenv("DVM_COLL_BS")) {
+    const int v = atoi(e);
+    if (v == 256 || v == 512 || v == 1024) c->coll_bs = v;
+  }
+  const size_t fixed_lds = (size_t)kAxisSlots * sizeof(double);
+  c->coll_lds = fixed_lds + (size_t)N * (f32 ? sizeof(float) : sizeof(double));
   c->lds_collide = c->coll_lds <= kLdsBytes;
-  if (c->lds_collide) {
-    HIP_TRY(c, hipFuncSetAttribute((const void*)k_collide<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)c->coll_lds));
-  } else {
-    c->coll_lds = (size_t)kAxisSlots * sizeof(double);
+  if (f32 && !c->lds_collide)
+    return fail(c, DVM_E_ARG, "FP32-collision path needs the velocity slice (%d nodes) in LDS", N);
+  if (!c->lds_collide) c->coll_lds = fixed_lds;
+  if (f32) {
+    if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
+  }
+  {
+    const void* kern = collide_kernel(c);
+    HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
   }
   // ---- convection constants
   ConvParams& p = c->convp;
@@ -415,16 +444,14 @@ static int cdf_async(dvm_ctx* c) {
 
 static int sample_async(dvm_ctx* c, int ntime) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
-  if (c->lds_collide) {
-    hipLaunchKernelGGL(k_collide<true>, dim3(c->ncell), dim3(kCollBS), c->coll_lds, c->stream, c->g, c->collp, ntime,
-                       c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors, c->err);
-  } else {
+  if (!c->lds_collide)
     HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
                               c->stream));
-    hipLaunchKernelGGL(k_collide<false>, dim3(c->ncell), dim3(kCollBS), c->coll_lds, c->stream, c->g, c->collp,
-                       ntime, c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors,
-                       c->err);
-  }
+  void* args[] = {&c->g, &c->collp, &ntime, &c->f, &c->fd, &c->dacc, &c->eqv, &c->eqc, &c->neqc, &c->eqtab,
+                  &c->neqtab, &c->counts, &c->factors, &c->err};
+  HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3(c->ncell), dim3(c->coll_bs), args, c->coll_lds, c->stream));
+  if (c->cfg.precision == DVM_PREC_FP32)
+    hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
   return DVM_OK;

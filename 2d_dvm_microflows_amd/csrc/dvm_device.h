@@ -93,23 +93,34 @@ __device__ __forceinline__ int referenced_search(const double* __restrict__ cumu
 
 // ---------------------------------------------------------------- remap
 // binary_search_with_first_guess (sort_and_search.f90:130-183): nearest axis
-// node to val, ties to the lower index, clamped to the axis ends.
+// node to val, ties to the lower index, clamped to the axis ends. The
+// reference's bisection can only stop at the unique index with
+// a[index] < val <= a[index+1] (then it keeps the closer end), so its result
+// does not depend on the first guess; a uniform grid's guess is within one
+// node of that bracket, so the bracket is found by a short walk and the
+// bisection only runs when the walk fails (non-uniform axes).
 __device__ __forceinline__ int nearest_node(const double* __restrict__ a, int mx, double val, int index) {
   if (a[0] > val) return 0;
   if (a[mx] < val) return mx;
   index = index < 0 ? 0 : (index > mx ? mx : index);
-  int left = 0, right = mx;
   if (fabs(val - a[index]) < kDoubleTol) return index;
-  if (index == right) index = index - 1;
-  for (int it = 0; it < kSearchCap; ++it) {
-    const double a0 = a[index], a1 = a[index + 1];
-    if (val > a0 && val <= a1) {
-      if (fabs(val - a0) > fabs(val - a1)) index = index + 1;
-      return index;
-    }
-    if (val > a0) left = index; else right = index;
-    index = left + (right - left) / 2;
+  if (index == mx) index = mx - 1;
+  double a0 = a[index], a1 = a[index + 1];
+#pragma unroll
+  for (int w = 0; w < 2; ++w) {
+    if (val > a1 && index + 1 < mx) { ++index; a0 = a1; a1 = a[index + 1]; }
+    else if (!(val > a0) && index > 0) { --index; a1 = a0; a0 = a[index]; }
   }
+  if (!(val > a0 && val <= a1)) {
+    int left = 0, right = mx;
+    for (int it = 0; it < kSearchCap; ++it) {
+      a0 = a[index]; a1 = a[index + 1];
+      if (val > a0 && val <= a1) break;
+      if (val > a0) left = index; else right = index;
+      index = left + (right - left) / 2;
+    }
+  }
+  if (fabs(val - a0) > fabs(val - a1)) index = index + 1;
   return index;
 }
 

@@ -13,6 +13,7 @@ constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell
 constexpr int kConvBS = 256;     // convection lanes per workgroup
 constexpr int kAxisSlots = 384;  // LDS doubles reserved for the three axes (<= 128 points each)
 constexpr int kMaxAxis = 128;
+constexpr int kRedSlots = 24;     // doubles of reduction scratch per wave in k_collide
 // LDS bytes available to one workgroup on gfx950
 constexpr size_t kLdsBytes = 160 * 1024;
 
@@ -56,12 +57,6 @@ __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const 
                       double* __restrict__ eqv, double* __restrict__ eqc, double* __restrict__ neqc,
                       int2* __restrict__ eqtab, int2* __restrict__ neqtab, double* __restrict__ eqpar,
                       int* __restrict__ counts, double* __restrict__ factors);
-template <bool kLds>
-__global__ void k_collide(Grid g, CollParams cp, int ntime, const double* __restrict__ f, double* __restrict__ fd,
-                          const double* __restrict__ eqv, const double* __restrict__ eqc,
-                          const double* __restrict__ neqc, const int2* __restrict__ eqtab,
-                          const int2* __restrict__ neqtab, const int* __restrict__ counts,
-                          const double* __restrict__ factors, int* __restrict__ err);
 __global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
                         const double* __restrict__ eqv, const double* __restrict__ eqc,
                         const double* __restrict__ neqc, const int2* __restrict__ eqtab,

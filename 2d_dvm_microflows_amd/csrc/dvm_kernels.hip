@@ -22,6 +22,8 @@
 //   k_wall_ratio, k_spec_ghost  diffuse flux balance and specular remap ghosts
 #include "dvm_kernels.h"
 
+#include <type_traits>
+
 namespace dvm {
 
 // ------------------------------------------------------------ reductions
@@ -269,11 +271,26 @@ struct SampleRec {
   double kd1, kd2, m1[7], m2[7];
 };
 
+template <class R>
+__device__ __forceinline__ void sin_cos(R x, R& s, R& c) {
+  if constexpr (sizeof(R) == 4) sincosf(x, &s, &c); else sincos(x, &s, &c);
+}
+template <class R>
+__device__ __forceinline__ R rsqrt_free_sqrt(R x) {
+  if constexpr (sizeof(R) == 4) return sqrtf(x); else return sqrt(x);
+}
+template <class R>
+__device__ __forceinline__ R rpow(R x, R y) {
+  if constexpr (sizeof(R) == 4) return powf(x, y); else return pow(x, y);
+}
+
 // One VR sample: variance_reduction.f90:686-816 / 838-961 / 983-1116 with
 // vr_depletion_routine (:1127-1237) and the elastic branch of
-// replenish_collision (replenishing_collisions.f90:134-354). Scatter(node,val)
-// receives the depletions and the replenishment of every stencil point.
-template <bool kRecord, class Scatter>
+// replenish_collision (replenishing_collisions.f90:134-354).
+// R is the collision arithmetic (double, or float for the FP32-collision path);
+// the partner picks, the deviation signs and the remap weights are FP64 in both.
+// scatter(node, value) receives the two depletions and every stencil point.
+template <bool kRecord, class R, class Scatter>
 __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, const CellView& cv, int cell,
                                          int ntime, int t, int s, double factor, Scatter&& scatter,
                                          SampleRec* rec) {
@@ -288,50 +305,51 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
   int iA, jA, kA, iB, jB, kB;
   node_ijk(lA, nx, g.n[1], iA, jA, kA);
   node_ijk(lB, nx, g.n[1], iB, jB, kB);
-  const double x1 = g.ax[0][iA], y1 = g.ax[1][jA], z1 = g.ax[2][kA];
-  const double x2 = g.ax[0][iB], y2 = g.ax[1][jB], z2 = g.ax[2][kB];
+  const R x1 = (R)g.ax[0][iA], y1 = (R)g.ax[1][jA], z1 = (R)g.ax[2][kA];
+  const R x2 = (R)g.ax[0][iB], y2 = (R)g.ax[1][jB], z2 = (R)g.ax[2][kB];
   if (kRecord) { rec->lA = lA; rec->lB = lB; rec->skipped = 0; }
-  const double gx = x2 - x1, gy = y2 - y1, gz = z2 - z1;
-  const double gg = sqrt(gx * gx + gy * gy + gz * gz);
-  if (gg < kDoubleTol) {
+  const R gx = x2 - x1, gy = y2 - y1, gz = z2 - z1;
+  const R gg = rsqrt_free_sqrt<R>(gx * gx + gy * gy + gz * gz);
+  if (gg < (R)kDoubleTol) {
     if (kRecord) rec->skipped = 1;
     return 0;
   }
-  const double g_sigma = cp.hard_sphere ? gg : pow(gg, cp.vhs_exponent);
-  const double signA = (t == 0) ? 1.0 : dsgn(cv.f[lA] - cv.eqv[lA]);
-  const double signB = (t == 1) ? 1.0 : dsgn(cv.f[lB] - cv.eqv[lB]);
-  const double sumA = signA, sumB = signB;
-  const double dep1 = sumB * 1.0 * factor * g_sigma;
-  const double dep2 = sumA * 1.0 * factor * g_sigma;
-  const double kd1 = sumA * dep1, kd2 = sumB * dep2;
-  if (kRecord) { rec->kd1 = kd1; rec->kd2 = kd2; }
+  const R g_sigma = cp.hard_sphere ? gg : rpow<R>(gg, (R)cp.vhs_exponent);
+  const R signA = (t == 0) ? (R)1 : (R)dsgn(cv.f[lA] - cv.eqv[lA]);
+  const R signB = (t == 1) ? (R)1 : (R)dsgn(cv.f[lB] - cv.eqv[lB]);
+  const R sumA = signA, sumB = signB;
+  const R fac = (R)factor;
+  const R dep1 = sumB * (R)1 * fac * g_sigma;
+  const R dep2 = sumA * (R)1 * fac * g_sigma;
+  const R kd1 = sumA * dep1, kd2 = sumB * dep2;
+  if (kRecord) { rec->kd1 = (double)kd1; rec->kd2 = (double)kd2; }
   if (!kRecord) {
-    scatter(lA, -kd1);
-    scatter(lB, -kd2);
+    scatter(lA, -(double)kd1);
+    scatter(lB, -(double)kd2);
   }
-  const double kr1 = kd1 / (double)cp.num_repl, kr2 = kd2 / (double)cp.num_repl;
-  const double mf = cp.mass / (cp.mass + cp.mass);
-  const double vc0 = mf * x1 + mf * x2, vc1 = mf * y1 + mf * y2, vc2 = mf * z1 + mf * z2;
-  const double gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
-  const double gx2 = fabs(x2 - vc0), gy2 = fabs(y2 - vc1), gz2 = fabs(z2 - vc2);
-  const double ur = gx1 + gx2, vr = gy1 + gy2, wr = gz1 + gz2;
-  const double gp1 = sqrt(gx1 * gx1 + gy1 * gy1 + gz1 * gz1);
-  const double gp2 = sqrt(gx2 * gx2 + gy2 * gy2 + gz2 * gz2);
-  const double gr1 = gp1 / gg, gr2 = gp2 / gg;
+  const R kr1 = kd1 / (R)cp.num_repl, kr2 = kd2 / (R)cp.num_repl;
+  const R mf = (R)(cp.mass / (cp.mass + cp.mass));
+  const R vc0 = mf * x1 + mf * x2, vc1 = mf * y1 + mf * y2, vc2 = mf * z1 + mf * z2;
+  const R gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
+  const R gx2 = fabs(x2 - vc0), gy2 = fabs(y2 - vc1), gz2 = fabs(z2 - vc2);
+  const R ur = gx1 + gx2, vr = gy1 + gy2, wr = gz1 + gz2;
+  const R gp1 = rsqrt_free_sqrt<R>(gx1 * gx1 + gy1 * gy1 + gz1 * gz1);
+  const R gp2 = rsqrt_free_sqrt<R>(gx2 * gx2 + gy2 * gy2 + gz2 * gz2);
+  const R gr1 = gp1 / gg, gr2 = gp2 / gg;
   int bad = 0;
   for (int r = 1; r <= cp.num_repl; ++r) {
     double uphi, uth;
     uniforms2(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, uphi, uth);
     // pick_random_velocity_on_sphere (collision_utilities.f90:284-352)
-    const double ph = 2.0 * kPiRef * uphi;
-    double sph, cph;
-    sincos(ph, &sph, &cph);
-    double rx, ry, rz;
+    const R ph = (R)2 * (R)kPiRef * (R)uphi;
+    R sph, cph;
+    sin_cos<R>(ph, sph, cph);
+    R rx, ry, rz;
     if (cp.vss) {
-      const double ct = 2.0 * pow(uth, cp.ralpha) - 1.0;
-      const double stt = sqrt(1.0 - ct * ct);
-      const double den = sqrt(vr * vr + wr * wr);
-      if (den > kDoubleTol * gg) {
+      const R ct = (R)2 * rpow<R>((R)uth, (R)cp.ralpha) - (R)1;
+      const R stt = rsqrt_free_sqrt<R>((R)1 - ct * ct);
+      const R den = rsqrt_free_sqrt<R>(vr * vr + wr * wr);
+      if (den > (R)kDoubleTol * gg) {
         rx = ct * ur + stt * sph * den;
         ry = ct * vr + stt * (gg * wr * cph - ur * vr * sph) / den;
         rz = ct * wr - stt * (gg * vr * cph + ur * wr * sph) / den;
@@ -341,32 +359,32 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
         rz = stt * sph * ur;
       }
     } else {
-      const double ct = 2.0 * uth - 1.0;
-      const double stt = sqrt(1.0 - ct * ct);
+      const R ct = (R)2 * (R)uth - (R)1;
+      const R stt = rsqrt_free_sqrt<R>((R)1 - ct * ct);
       rx = gg * stt * cph;
       ry = gg * stt * sph;
       rz = gg * ct;
     }
-    Stencil s1, s2;
-    remap(g, vc0 + gr1 * rx, vc1 + gr1 * ry, vc2 + gr1 * rz, 1.0, s1);
-    remap(g, vc0 + -(gr2 * rx), vc1 + -(gr2 * ry), vc2 + -(gr2 * rz), 1.0, s2);
-    if (kRecord && r == 1) {
-      rec->st1[0] = s1.oi; rec->st1[1] = s1.oj; rec->st1[2] = s1.ok; rec->st1[3] = s1.a;
-      rec->st1[4] = s1.b; rec->st1[5] = s1.c; rec->st1[6] = s1.flag;
-      rec->st2[0] = s2.oi; rec->st2[1] = s2.oj; rec->st2[2] = s2.ok; rec->st2[3] = s2.a;
-      rec->st2[4] = s2.b; rec->st2[5] = s2.c; rec->st2[6] = s2.flag;
+    // post-collision velocities, remapped with FP64 weights
+    // (remapping.f90:55-394 via apply_phi_replenishment :356-434)
+    for (int side = 0; side < 2; ++side) {
+      Stencil st;
+      if (side == 0) remap(g, (double)(vc0 + gr1 * rx), (double)(vc1 + gr1 * ry), (double)(vc2 + gr1 * rz), 1.0, st);
+      else remap(g, (double)(vc0 + -(gr2 * rx)), (double)(vc1 + -(gr2 * ry)), (double)(vc2 + -(gr2 * rz)), 1.0, st);
+      if (kRecord && r == 1) {
+        int* o = side == 0 ? rec->st1 : rec->st2;
+        double* m = side == 0 ? rec->m1 : rec->m2;
+        o[0] = st.oi; o[1] = st.oj; o[2] = st.ok; o[3] = st.a; o[4] = st.b; o[5] = st.c; o[6] = st.flag;
 #pragma unroll
-      for (int q = 0; q < 7; ++q) { rec->m1[q] = s1.m[q]; rec->m2[q] = s2.m[q]; }
-    }
-    if (s1.flag == 0 || s2.flag == 0) { bad = 1; continue; }
-    if (!kRecord) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        if (q < 4 || (s1.flag & (1 << (q - 4)))) scatter(stencil_node(s1, q, nx, nxy), kr1 * s1.m[q]);
+        for (int q = 0; q < 7; ++q) m[q] = st.m[q];
       }
+      if (st.flag == 0) { bad = 1; continue; }
+      if (!kRecord) {
+        const double kr = (double)(side == 0 ? kr1 : kr2);
 #pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        if (q < 4 || (s2.flag & (1 << (q - 4)))) scatter(stencil_node(s2, q, nx, nxy), kr2 * s2.m[q]);
+        for (int q = 0; q < 7; ++q) {
+          if (q < 4 || (st.flag & (1 << (q - 4)))) scatter(stencil_node(st, q, nx, nxy), kr * st.m[q]);
+        }
       }
     }
   }
@@ -383,23 +401,57 @@ __device__ __forceinline__ void stage_axes(Grid& g, double* sh) {
   }
 }
 
-// Collision phase of one cell per workgroup. kLds: the cell's increments are
-// accumulated in LDS (dynamic shared memory of N doubles after the axes) and
-// added to f once; otherwise fd (a copy of f) takes global atomics.
-template <bool kLds>
-__global__ void __launch_bounds__(kCollBS) k_collide(Grid g, CollParams cp, int ntime,
-                                                     const double* __restrict__ f, double* __restrict__ fd,
-                                                     const double* __restrict__ eqv, const double* __restrict__ eqc,
-                                                     const double* __restrict__ neqc, const int2* __restrict__ eqtab,
-                                                     const int2* __restrict__ neqtab, const int* __restrict__ counts,
-                                                     const double* __restrict__ factors, int* __restrict__ err) {
+// Symmetric positive-definite 5x5 solve (Cholesky) for the conservation
+// correction; returns false when the matrix is not numerically SPD.
+__device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 5, overwritten */) {
+  for (int j = 0; j < 5; ++j) {
+    double d = M[j * 5 + j];
+    for (int k = 0; k < j; ++k) d -= M[j * 5 + k] * M[j * 5 + k];
+    if (!(d > 0.0)) return false;
+    d = sqrt(d);
+    M[j * 5 + j] = d;
+    for (int i = j + 1; i < 5; ++i) {
+      double v = M[i * 5 + j];
+      for (int k = 0; k < j; ++k) v -= M[i * 5 + k] * M[j * 5 + k];
+      M[i * 5 + j] = v / d;
+    }
+  }
+  for (int i = 0; i < 5; ++i) {
+    double v = b[i];
+    for (int k = 0; k < i; ++k) v -= M[i * 5 + k] * b[k];
+    b[i] = v / M[i * 5 + i];
+  }
+  for (int i = 4; i >= 0; --i) {
+    double v = b[i];
+    for (int k = i + 1; k < 5; ++k) v -= M[k * 5 + i] * b[k];
+    b[i] = v / M[i * 5 + i];
+  }
+  return true;
+}
+
+// Collision phase of one cell per workgroup of BS lanes.
+//   kLds: the cell's increments accumulate in LDS (dynamic shared memory after
+//         the axes); otherwise fd (a copy of f) takes global FP64 atomics.
+//   kF32: FP32 collision arithmetic and FP32 LDS accumulation; the change is
+//         written to dacc and k_conserve adds it to f with the FP64
+//         conservation correction. FP64: f + change is written to fd here.
+template <bool kLds, bool kF32, int BS>
+__global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime,
+                                                const double* __restrict__ f, double* __restrict__ fd,
+                                                float* __restrict__ dacc,
+                                                const double* __restrict__ eqv, const double* __restrict__ eqc,
+                                                const double* __restrict__ neqc, const int2* __restrict__ eqtab,
+                                                const int2* __restrict__ neqtab, const int* __restrict__ counts,
+                                                const double* __restrict__ factors, int* __restrict__ err) {
   extern __shared__ double dyn[];
+  using Acc = typename std::conditional<kF32, float, double>::type;
+  using R = typename std::conditional<kF32, float, double>::type;
   const int cell = blockIdx.x;
   const int N = g.N;
   stage_axes(g, dyn);
-  double* acc = dyn + kAxisSlots;
+  Acc* acc = reinterpret_cast<Acc*>(dyn + kAxisSlots);
   if (kLds) {
-    for (int l = threadIdx.x; l < N; l += kCollBS) acc[l] = 0.0;
+    for (int l = threadIdx.x; l < N; l += BS) acc[l] = (Acc)0;
   }
   __syncthreads();
   CellView cv;
@@ -413,31 +465,103 @@ __global__ void __launch_bounds__(kCollBS) k_collide(Grid g, CollParams cp, int 
   const int total = n0 + n1 + n2;
   double* gd = fd + (size_t)cell * N;
   int bad = 0;
-  for (int q = threadIdx.x; q < total; q += kCollBS) {
+  for (int q = threadIdx.x; q < total; q += BS) {
     const int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
     const int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
     const double factor = factors[cell * 3 + t];
     if (kLds) {
-      bad |= vr_sample<false>(g, cp, cv, cell, ntime, t, s, factor,
-                              [&](int node, double val) {
-                                if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], val); else bad = 1;
-                              }, nullptr);
+      bad |= vr_sample<false, R>(g, cp, cv, cell, ntime, t, s, factor,
+                                 [&](int node, double val) {
+                                   if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (Acc)val); else bad = 1;
+                                 }, nullptr);
     } else {
-      bad |= vr_sample<false>(g, cp, cv, cell, ntime, t, s, factor,
-                              [&](int node, double val) {
-                                if ((unsigned)node < (unsigned)N) atomicAdd(&gd[node], val); else bad = 1;
-                              }, nullptr);
+      bad |= vr_sample<false, R>(g, cp, cv, cell, ntime, t, s, factor,
+                                 [&](int node, double val) {
+                                   if ((unsigned)node < (unsigned)N) atomicAdd(&gd[node], val); else bad = 1;
+                                 }, nullptr);
     }
   }
   if (bad) atomicOr(err, 1);
-  if (kLds) {
-    __syncthreads();
-    for (int l = threadIdx.x; l < N; l += kCollBS) gd[l] = cv.f[l] + acc[l];
+  if (!kLds) return;
+  __syncthreads();
+  if constexpr (!kF32) {
+    for (int l = threadIdx.x; l < N; l += BS) gd[l] = cv.f[l] + acc[l];
+  } else {
+    float* da = dacc + (size_t)cell * N;
+    for (int l = threadIdx.x; l < N; l += BS) da[l] = acc[l];
   }
 }
 
-// Parity trace of one cell: the same samples as k_collide, recorded instead
-// of scattered.
+// FP64 conservation correction of the FP32-accumulated collision change d of
+// each cell: d is replaced by d - |f| (lambda . phi), phi = (1, v, |v|^2), with
+// lambda solving sum |f| phi phi^T lambda = sum d phi. In exact arithmetic
+// the collision step conserves mass, momentum and energy (each sample's
+// depletions equal its replenishments and the remap keeps moments 0-2); the
+// correction removes what FP32 rounding leaves, so the cell's moments change by
+// FP64 round-off only. One workgroup per cell.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restrict__ f,
+                                                 const float* __restrict__ dacc, double* __restrict__ fd) {
+  __shared__ double red[BS / 64][20];
+  __shared__ double lam[5];
+  const int cell = blockIdx.x, N = g.N;
+  const double* fc = f + (size_t)cell * N;
+  const float* da = dacc + (size_t)cell * N;
+  double sums[20];
+#pragma unroll
+  for (int z = 0; z < 20; ++z) sums[z] = 0.0;
+  for (int l = threadIdx.x; l < N; l += BS) {
+    int i, j, k;
+    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+    const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
+    const double d = (double)da[l];
+    const double w = fabs(fc[l]);
+#pragma unroll
+    for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
+    int o = 5;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+      for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int z = 0; z < 20; ++z) {
+    const double v = wave_sum(sums[z]);
+    if (lane == 0) red[wid][z] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[20];
+    for (int z = 0; z < 20; ++z) {
+      double v = 0.0;
+      for (int w = 0; w < BS / 64; ++w) v += red[w][z];
+      tot[z] = v;
+    }
+    double M[25], b[5];
+    int o = 5;
+    for (int a = 0; a < 5; ++a)
+      for (int c = a; c < 5; ++c) { M[a * 5 + c] = tot[o]; M[c * 5 + a] = tot[o]; ++o; }
+    for (int a = 0; a < 5; ++a) b[a] = tot[a];
+    const bool ok = solve5(M, b);
+    for (int a = 0; a < 5; ++a) lam[a] = ok ? b[a] : 0.0;
+  }
+  __syncthreads();
+  const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
+  double* out = fd + (size_t)cell * N;
+  for (int l = threadIdx.x; l < N; l += BS) {
+    int i, j, k;
+    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+    const double fl = fc[l];
+    const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
+    out[l] = fl + ((double)da[l] - corr);
+  }
+}
+
+// Parity trace of one cell: the same samples as the FP64 k_collide, recorded
+// instead of scattered.
 __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int ntime, int cell,
                                                    const double* __restrict__ f, const double* __restrict__ eqv,
                                                    const double* __restrict__ eqc, const double* __restrict__ neqc,
@@ -461,7 +585,7 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     SampleRec rec;
     for (int z = 0; z < 7; ++z) { rec.st1[z] = 0; rec.st2[z] = 0; rec.m1[z] = 0.0; rec.m2[z] = 0.0; }
     rec.kd1 = 0.0; rec.kd2 = 0.0;
-    vr_sample<true>(g, cp, cv, cell, ntime, t, s, factors[cell * 3 + t], [](int, double) {}, &rec);
+    vr_sample<true, double>(g, cp, cv, cell, ntime, t, s, factors[cell * 3 + t], [](int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
     for (int z = 0; z < 32; ++z) ir[z] = 0;

@@ -36,6 +36,8 @@ def hip_sync():
 
 
 def run_gpu(pkg, cfg, f, steps, warmup):
+    """Times `steps` fused steps after `warmup` ones; returns wall seconds, mean
+    per-phase device ms (HIP events on the library's stream) and the moments."""
     s = pkg.Solver(cfg)
     s.initialize(f)
     for n in range(1, warmup + 1):
@@ -72,6 +74,16 @@ def cpu_baseline(pkg, cfg_full, steps=4):
                       f"{cfg.samples_per_cell} samples/cell, {steps} full steps, {dt:.1f} s"}
 
 
+def moment_error(p32, p64):
+    """Max over cells of the relative error of density, velocity (in units of
+    the thermal speed sqrt(T)) and temperature, FP32 path against FP64."""
+    th = np.sqrt(np.abs(p64[:, 4]))
+    errs = {"density": np.abs(p32[:, 0] - p64[:, 0]) / np.abs(p64[:, 0]),
+            "velocity": np.max(np.abs(p32[:, 1:4] - p64[:, 1:4]), axis=1) / th,
+            "temperature": np.abs(p32[:, 4] - p64[:, 4]) / np.abs(p64[:, 4])}
+    return {k: float(np.max(v)) for k, v in errs.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +92,7 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--nv", type=int, default=25)
     ap.add_argument("--samples", type=int, default=20000)
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -87,9 +100,15 @@ def main():
         sys.exit(2)
     pkg = importlib.import_module("2d_dvm_microflows_amd")
     cfg, f = pkg.lid_driven_cavity(nx=args.nx, ny=args.nx, n_v=args.nv, samples=args.samples)
-    wall, phase, props = run_gpu(pkg, cfg, f, args.steps, args.warmup)
     ncell, N = cfg.nx * cfg.ny, cfg.num_points
     updates = ncell * N * args.steps
+    runs = {}
+    for prec in (("fp64", "fp32") if args.precision == "both" else (args.precision,)):
+        c = cfg.with_(precision=pkg.capi.DVM_PREC_FP32 if prec == "fp32" else pkg.capi.DVM_PREC_FP64)
+        runs[prec] = run_gpu(pkg, c, f, args.steps, args.warmup)
+        print(f"[bench] {prec}: {runs[prec][0] / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    head = "fp32" if "fp32" in runs else "fp64"
+    wall, phase, props = runs[head]
     value = updates / wall
     sweep_ms = phase["sweep"]
     alg_bytes = 2 * 8 * ncell * N
@@ -106,21 +125,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": f"2-D lid-driven cavity {cfg.nx}x{cfg.ny} cells, {cfg.nvx}^3 velocity nodes on +-5, "
                                f"{cfg.samples_per_cell} VR samples/cell/step, hard sphere, Kn {cfg.kn}, lid 0.3, "
-                               f"diffuse walls, FP64 collision",
+                               f"diffuse walls, {head.upper()} collision arithmetic (FP64 remap, moments"
+                               f"{', conservation correction' if head == 'fp32' else ''})",
                    "cells": ncell, "velocity_nodes": N, "samples_per_cell": cfg.samples_per_cell,
-                   "precision": "fp64"},
+                   "precision": head},
         "collision_samples_per_s": samples_per_step * args.steps / wall,
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         "roofline": {"bound": "hbm", "kernel": "k_convect2d (upwind sweep)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "algorithmic_bytes_per_launch": alg_bytes},
-        "mass_total": float(props[:, 0].sum()),
     }
+    if "fp64" in runs and "fp32" in runs:
+        w64 = runs["fp64"][0]
+        out["fp64_path"] = {"value": updates / w64, "ms_per_step": w64 / args.steps * 1e3,
+                            "phase_ms": {k: round(v, 4) for k, v in runs["fp64"][1].items()}}
+        out["speedup_fp32_vs_fp64"] = w64 / wall
+        out["fp32_moment_rel_error_vs_fp64"] = moment_error(runs["fp32"][2], runs["fp64"][2])
+        out["fp32_moment_error_after_steps"] = args.warmup + args.steps
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pkg, cfg)
     print(json.dumps(out))
