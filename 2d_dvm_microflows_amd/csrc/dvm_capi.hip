@@ -29,8 +29,8 @@ struct dvm_ctx {
   Grid g{};                      // device pointers
   double* d_axes = nullptr;      // [ax0 ax1 ax2 beta0 beta1 beta2]
   // resident state
-  double *f = nullptr, *fd = nullptr, *eqv = nullptr, *eqc = nullptr, *neqc = nullptr;
-  int2 *eqtab = nullptr, *neqtab = nullptr;
+  double *f = nullptr, *fd = nullptr, *eqv = nullptr, *cdf = nullptr;  // cdf: [C][2][N+1]
+  int2* tabs = nullptr;                  // [C][2][1000]
   double *eqpar = nullptr, *props = nullptr, *factors = nullptr;
   int *counts = nullptr, *err = nullptr;
   // walls
@@ -40,8 +40,9 @@ struct dvm_ctx {
   ConvParams convp{};
   bool lds_collide = false;
   size_t coll_lds = 0;
-  int coll_bs = 512;             // lanes per collision workgroup (256, 512 or 1024)
-  float* dacc = nullptr;         // [C][N] FP32 collision change (FP32 path)
+  int coll_bs = 1024;            // lanes per collision workgroup (256, 512 or 1024)
+  int pick_k = 1;                // samples per lane picked in lockstep (1, 2 or 4)
+  double* dacc = nullptr;        // [C][N] collision change of the FP32 path
   bool f_valid = false, props_valid = false;
   int trace_ready = -1;
   char msg[512] = {0};
@@ -146,7 +147,7 @@ extern "C" const char* dvm_last_error(const dvm_ctx* ctx) { return ctx ? ctx->ms
 
 extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   if (!c) return;
-  void* ptrs[] = {c->d_axes, c->f, c->fd, c->eqv, c->eqc, c->neqc, c->eqtab, c->neqtab, c->eqpar, c->props,
+  void* ptrs[] = {c->d_axes, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->eqpar, c->props,
                   c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -157,18 +158,27 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   delete c;
 }
 
+template <int BS, int K>
+static const void* collide_kernel_bsk(bool lds, bool f32) {
+  if (!lds) return f32 ? (const void*)k_collide<false, true, BS, K> : (const void*)k_collide<false, false, BS, K>;
+  return f32 ? (const void*)k_collide<true, true, BS, K> : (const void*)k_collide<true, false, BS, K>;
+}
+
 template <int BS>
-static const void* collide_kernel_bs(bool lds, bool f32) {
-  if (!lds) return (const void*)k_collide<false, false, BS>;
-  return f32 ? (const void*)k_collide<true, true, BS> : (const void*)k_collide<true, false, BS>;
+static const void* collide_kernel_bs(int k, bool lds, bool f32) {
+  switch (k) {
+    case 1: return collide_kernel_bsk<BS, 1>(lds, f32);
+    case 4: return collide_kernel_bsk<BS, 4>(lds, f32);
+    default: return collide_kernel_bsk<BS, 2>(lds, f32);
+  }
 }
 
 static const void* collide_kernel(const dvm_ctx* c) {
   const bool f32 = c->cfg.precision == DVM_PREC_FP32;
   switch (c->coll_bs) {
-    case 256: return collide_kernel_bs<256>(c->lds_collide, f32);
-    case 1024: return collide_kernel_bs<1024>(c->lds_collide, f32);
-    default: return collide_kernel_bs<512>(c->lds_collide, f32);
+    case 256: return collide_kernel_bs<256>(c->pick_k, c->lds_collide, f32);
+    case 1024: return collide_kernel_bs<1024>(c->pick_k, c->lds_collide, f32);
+    default: return collide_kernel_bs<512>(c->pick_k, c->lds_collide, f32);
   }
 }
 
@@ -248,9 +258,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   const size_t CN = (size_t)c->ncell * N, CN1 = (size_t)c->ncell * (N + 1);
   int rc;
   if ((rc = dalloc(c, &c->f, CN)) || (rc = dalloc(c, &c->fd, CN)) || (rc = dalloc(c, &c->eqv, CN)) ||
-      (rc = dalloc(c, &c->eqc, CN1)) || (rc = dalloc(c, &c->neqc, CN1)) ||
-      (rc = dalloc(c, &c->eqtab, (size_t)c->ncell * kNBucket)) ||
-      (rc = dalloc(c, &c->neqtab, (size_t)c->ncell * kNBucket)) || (rc = dalloc(c, &c->eqpar, (size_t)c->ncell * 5)) ||
+      (rc = dalloc(c, &c->cdf, 2 * CN1)) || (rc = dalloc(c, &c->tabs, (size_t)c->ncell * 2 * kNBucket)) ||
+      (rc = dalloc(c, &c->eqpar, (size_t)c->ncell * 5)) ||
       (rc = dalloc(c, &c->props, (size_t)c->ncell * DVM_NPROP)) || (rc = dalloc(c, &c->factors, (size_t)c->ncell * 3)) ||
       (rc = dalloc(c, &c->counts, (size_t)c->ncell * 4)) || (rc = dalloc(c, &c->err, 4)))
     return rc;
@@ -262,11 +271,13 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     const int v = atoi(e);
     if (v == 256 || v == 512 || v == 1024) c->coll_bs = v;
   }
+  if (const char* e = getenv("DVM_PICK_K")) {
+    const int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4) c->pick_k = v;
+  }
   const size_t fixed_lds = (size_t)kAxisSlots * sizeof(double);
-  c->coll_lds = fixed_lds + (size_t)N * (f32 ? sizeof(float) : sizeof(double));
+  c->coll_lds = fixed_lds + (size_t)N * sizeof(double);
   c->lds_collide = c->coll_lds <= kLdsBytes;
-  if (f32 && !c->lds_collide)
-    return fail(c, DVM_E_ARG, "FP32-collision path needs the velocity slice (%d nodes) in LDS", N);
   if (!c->lds_collide) c->coll_lds = fixed_lds;
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
@@ -437,20 +448,25 @@ static int moments_async(dvm_ctx* c) {
 static int cdf_async(dvm_ctx* c) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
   hipLaunchKernelGGL(k_cdf, dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f, c->props, c->eqv,
-                     c->eqc, c->neqc, c->eqtab, c->neqtab, c->eqpar, c->counts, c->factors);
+                     c->cdf, c->tabs, c->eqpar, c->counts, c->factors);
   HIP_TRY(c, hipGetLastError());
   return DVM_OK;
 }
 
 static int sample_async(dvm_ctx* c, int ntime) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
-  if (!c->lds_collide)
-    HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
-                              c->stream));
-  void* args[] = {&c->g, &c->collp, &ntime, &c->f, &c->fd, &c->dacc, &c->eqv, &c->eqc, &c->neqc, &c->eqtab,
-                  &c->neqtab, &c->counts, &c->factors, &c->err};
+  const bool f32 = c->cfg.precision == DVM_PREC_FP32;
+  if (!c->lds_collide) {
+    if (f32)
+      hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->dacc, (long long)c->ncell * c->N, 0.0);
+    else
+      HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+  }
+  void* args[] = {&c->g, &c->collp, &ntime, &c->f, &c->fd, &c->dacc, &c->eqv, &c->cdf, &c->tabs,
+                  &c->counts, &c->factors, &c->err};
   HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3(c->ncell), dim3(c->coll_bs), args, c->coll_lds, c->stream));
-  if (c->cfg.precision == DVM_PREC_FP32)
+  if (f32)
     hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
@@ -625,7 +641,7 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
     }
     const unsigned blocks = (unsigned)((nrec + kCollBS - 1) / kCollBS);
     hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kCollBS), 0, c->stream, c->g, c->collp, ntime, cell, c->f, c->eqv,
-                       c->eqc, c->neqc, c->eqtab, c->neqtab, c->counts, c->factors, (long long)nrec, d_i, d_d);
+                       c->cdf, c->tabs, c->counts, c->factors, (long long)nrec, d_i, d_d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(irec, d_i, (size_t)32 * nrec * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(drec, d_d, (size_t)16 * nrec * sizeof(double), hipMemcpyDeviceToHost, c->stream);

@@ -13,7 +13,6 @@ constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell
 constexpr int kConvBS = 256;     // convection lanes per workgroup
 constexpr int kAxisSlots = 384;  // LDS doubles reserved for the three axes (<= 128 points each)
 constexpr int kMaxAxis = 128;
-constexpr int kRedSlots = 24;     // doubles of reduction scratch per wave in k_collide
 // LDS bytes available to one workgroup on gfx950
 constexpr size_t kLdsBytes = 160 * 1024;
 
@@ -39,6 +38,7 @@ struct CellView {
   const double* neqc;
   const int2* eqtab;
   const int2* neqtab;
+  double eq_total, neq_total;    // eqc[N], neqc[N]
 };
 
 struct ConvParams {
@@ -54,13 +54,12 @@ struct ConvParams {
 
 __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass);
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
-                      double* __restrict__ eqv, double* __restrict__ eqc, double* __restrict__ neqc,
-                      int2* __restrict__ eqtab, int2* __restrict__ neqtab, double* __restrict__ eqpar,
+                      double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
+                      double* __restrict__ eqpar,
                       int* __restrict__ counts, double* __restrict__ factors);
 __global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
-                        const double* __restrict__ eqv, const double* __restrict__ eqc,
-                        const double* __restrict__ neqc, const int2* __restrict__ eqtab,
-                        const int2* __restrict__ neqtab, const int* __restrict__ counts,
+                        const double* __restrict__ eqv, const double* __restrict__ cdf,
+                        const int2* __restrict__ tabs, const int* __restrict__ counts,
                         const double* __restrict__ factors, long long max_records, int* __restrict__ irec,
                         double* __restrict__ drec);
 __global__ void k_remap_batch(Grid g, long long n, const double* __restrict__ uvw, double f_off,

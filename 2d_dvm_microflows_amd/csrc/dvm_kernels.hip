@@ -151,8 +151,7 @@ __device__ __forceinline__ void build_buckets(const double* __restrict__ cumul, 
 // depletion factors (variance_reduction.f90:598-604,679,831,976).
 __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
                                                 const double* __restrict__ props, double* __restrict__ eqv,
-                                                double* __restrict__ eqc, double* __restrict__ neqc,
-                                                int2* __restrict__ eqtab, int2* __restrict__ neqtab,
+                                                double* __restrict__ cdf, int2* __restrict__ tabs,
                                                 double* __restrict__ eqpar, int* __restrict__ counts,
                                                 double* __restrict__ factors) {
   __shared__ double red[kCdfBS / 64];
@@ -164,8 +163,8 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
   const double* p = props + (size_t)cell * DVM_NPROP;
   const double* fc = f + (size_t)cell * N;
   double* ev = eqv + (size_t)cell * N;
-  double* ec = eqc + (size_t)cell * (N + 1);
-  double* nc = neqc + (size_t)cell * (N + 1);
+  double* ec = cdf + (size_t)cell * 2 * (N + 1);  // equilibrium CDF, then the deviation CDF
+  double* nc = ec + (N + 1);
   double* par = eqpar + (size_t)cell * 5;
   const double dens = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W];
   double temp = p[DVM_P_TEMP];
@@ -220,7 +219,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
       }
     }
     __syncthreads();
-    build_buckets(ec, N, eqtab + (size_t)cell * kNBucket, rights);
+    build_buckets(ec, N, tabs + (size_t)cell * 2 * kNBucket, rights);
     if (threadIdx.x == 0) {
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
     }
@@ -237,7 +236,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
     if (l < N) nc[l + 1] = pre;
   }
   __syncthreads();
-  build_buckets(nc, N, neqtab + (size_t)cell * kNBucket, rights);
+  build_buckets(nc, N, tabs + (size_t)cell * 2 * kNBucket + kNBucket, rights);
   if (threadIdx.x == 0) {
     const double neq_dens = nc[N], eq_dens = ec[N];
     const double h = 0.5 * neq_dens / dens;
@@ -271,17 +270,101 @@ struct SampleRec {
   double kd1, kd2, m1[7], m2[7];
 };
 
+// FP32 collision arithmetic uses the hardware transcendental and square-root
+// instructions (v_sin_f32 / v_cos_f32 / v_sqrt_f32, ~1e-6 relative); FP64
+// uses the correctly rounded library functions the reference's results need.
 template <class R>
 __device__ __forceinline__ void sin_cos(R x, R& s, R& c) {
-  if constexpr (sizeof(R) == 4) sincosf(x, &s, &c); else sincos(x, &s, &c);
+  if constexpr (sizeof(R) == 4) { s = __sinf(x); c = __cosf(x); } else sincos(x, &s, &c);
 }
 template <class R>
 __device__ __forceinline__ R rsqrt_free_sqrt(R x) {
-  if constexpr (sizeof(R) == 4) return sqrtf(x); else return sqrt(x);
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_sqrtf(x); else return sqrt(x);
+}
+template <class R>
+__device__ __forceinline__ R rdiv(R a, R b) {
+  if constexpr (sizeof(R) == 4) return __fdividef(a, b); else return a / b;
 }
 template <class R>
 __device__ __forceinline__ R rpow(R x, R y) {
   if constexpr (sizeof(R) == 4) return powf(x, y); else return pow(x, y);
+}
+
+// Partner picks of K samples (pick_collision_partner, collision_utilities.f90:
+// 142-186, with referenced_search, sort_and_search.f90:40-86). The 2K
+// inverse-CDF searches advance in lockstep so that their dependent loads from
+// the cell's CDFs (L2/MALL resident, not LDS) overlap; each search follows the
+// reference's bucket bracket and bisection exactly. Also returns the sign of
+// the deviation at each deviation partner (+1 for an equilibrium partner).
+template <int K>
+__device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView& cv, int N, int cell, int ntime,
+                                           const int* t, const int* s, const bool* valid, int* lA, int* lB,
+                                           double* sgA, double* sgB) {
+  constexpr int M = 2 * K;
+  // the cell's two CDFs are adjacent: eq at cv.eqc, deviation at cv.eqc + N + 1
+  int off[M], left[M], right[M], idx[M];
+  double adj[M];
+  bool active[M];
+  int2 lr[M];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double uA = 0.0, uB = 0.0;
+    if (valid[k]) uniforms2(cp.seed, (uint32_t)s[k], (uint32_t)(t[k] + 1), (uint32_t)cell, (uint32_t)ntime, uA, uB);
+    const bool eqA = t[k] == 0, eqB = t[k] == 1;
+    off[2 * k] = eqA ? 0 : N + 1;
+    off[2 * k + 1] = eqB ? 0 : N + 1;
+    adj[2 * k] = uA;
+    adj[2 * k + 1] = uB;
+    int bA = (int)ceil(uA * 1000.0), bB = (int)ceil(uB * 1000.0);
+    bA = bA < 1 ? 1 : (bA > kNBucket ? kNBucket : bA);
+    bB = bB < 1 ? 1 : (bB > kNBucket ? kNBucket : bB);
+    lr[2 * k] = cv.eqtab[(eqA ? 0 : kNBucket) + bA - 1];
+    lr[2 * k + 1] = cv.eqtab[(eqB ? 0 : kNBucket) + bB - 1];
+  }
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const double u = adj[j];
+    left[j] = lr[j].x;
+    right[j] = lr[j].y;
+    idx[j] = left[j] + (right[j] - left[j]) / 2;
+    active[j] = valid[j >> 1] && !(u < kDoubleTol) && left[j] != right[j];
+    if (valid[j >> 1] && u < kDoubleTol) idx[j] = 0;
+    adj[j] = u * (off[j] == 0 ? cv.eq_total : cv.neq_total);
+  }
+  for (int it = 0; it < kSearchCap; ++it) {
+    double c0[M], c1[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const double* a = cv.eqc + off[j] + (active[j] ? idx[j] : 0);
+      c0[j] = a[0];
+      c1[j] = a[1];
+    }
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      if (!active[j]) continue;
+      if (adj[j] > c0[j] && adj[j] <= c1[j]) { active[j] = false; continue; }
+      if (adj[j] > c0[j]) left[j] = idx[j]; else right[j] = idx[j];
+      idx[j] = left[j] + (right[j] - left[j]) / 2;
+      any = true;
+    }
+    if (!any) break;
+  }
+  double fa[K], ea[K], fb[K], eb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    lA[k] = idx[2 * k];
+    lB[k] = idx[2 * k + 1];
+    const int a = ((unsigned)lA[k] < (unsigned)N) ? lA[k] : 0;
+    const int b = ((unsigned)lB[k] < (unsigned)N) ? lB[k] : 0;
+    fa[k] = cv.f[a]; ea[k] = cv.eqv[a];
+    fb[k] = cv.f[b]; eb[k] = cv.eqv[b];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    sgA[k] = (t[k] == 0) ? 1.0 : dsgn(fa[k] - ea[k]);
+    sgB[k] = (t[k] == 1) ? 1.0 : dsgn(fb[k] - eb[k]);
+  }
 }
 
 // One VR sample: variance_reduction.f90:686-816 / 838-961 / 983-1116 with
@@ -291,16 +374,10 @@ __device__ __forceinline__ R rpow(R x, R y) {
 // the partner picks, the deviation signs and the remap weights are FP64 in both.
 // scatter(node, value) receives the two depletions and every stencil point.
 template <bool kRecord, class R, class Scatter>
-__device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, const CellView& cv, int cell,
-                                         int ntime, int t, int s, double factor, Scatter&& scatter,
-                                         SampleRec* rec) {
+__device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, int cell, int ntime, int t, int s,
+                                         double factor, int lA, int lB, double signA_raw, double signB_raw,
+                                         Scatter&& scatter, SampleRec* rec) {
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
-  double uA, uB;
-  uniforms2(cp.seed, (uint32_t)s, (uint32_t)(t + 1), (uint32_t)cell, (uint32_t)ntime, uA, uB);
-  const int lA = (t == 0) ? referenced_search(cv.eqc, N, cv.eqtab, uA)
-                          : referenced_search(cv.neqc, N, cv.neqtab, uA);
-  const int lB = (t == 1) ? referenced_search(cv.eqc, N, cv.eqtab, uB)
-                          : referenced_search(cv.neqc, N, cv.neqtab, uB);
   if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) return 1;  // unreachable for finite CDFs
   int iA, jA, kA, iB, jB, kB;
   node_ijk(lA, nx, g.n[1], iA, jA, kA);
@@ -315,8 +392,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
     return 0;
   }
   const R g_sigma = cp.hard_sphere ? gg : rpow<R>(gg, (R)cp.vhs_exponent);
-  const R signA = (t == 0) ? (R)1 : (R)dsgn(cv.f[lA] - cv.eqv[lA]);
-  const R signB = (t == 1) ? (R)1 : (R)dsgn(cv.f[lB] - cv.eqv[lB]);
+  const R signA = (R)signA_raw;
+  const R signB = (R)signB_raw;
   const R sumA = signA, sumB = signB;
   const R fac = (R)factor;
   const R dep1 = sumB * (R)1 * fac * g_sigma;
@@ -327,7 +404,9 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
     scatter(lA, -(double)kd1);
     scatter(lB, -(double)kd2);
   }
-  const R kr1 = kd1 / (R)cp.num_repl, kr2 = kd2 / (R)cp.num_repl;
+  // kd / NRepl; NRepl = 1 in the reference's decks, where the quotient is kd exactly
+  const R kr1 = (cp.num_repl == 1) ? kd1 : kd1 / (R)cp.num_repl;
+  const R kr2 = (cp.num_repl == 1) ? kd2 : kd2 / (R)cp.num_repl;
   const R mf = (R)(cp.mass / (cp.mass + cp.mass));
   const R vc0 = mf * x1 + mf * x2, vc1 = mf * y1 + mf * y2, vc2 = mf * z1 + mf * z2;
   const R gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
@@ -335,7 +414,7 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
   const R ur = gx1 + gx2, vr = gy1 + gy2, wr = gz1 + gz2;
   const R gp1 = rsqrt_free_sqrt<R>(gx1 * gx1 + gy1 * gy1 + gz1 * gz1);
   const R gp2 = rsqrt_free_sqrt<R>(gx2 * gx2 + gy2 * gy2 + gz2 * gz2);
-  const R gr1 = gp1 / gg, gr2 = gp2 / gg;
+  const R gr1 = rdiv<R>(gp1, gg), gr2 = rdiv<R>(gp2, gg);
   int bad = 0;
   for (int r = 1; r <= cp.num_repl; ++r) {
     double uphi, uth;
@@ -351,8 +430,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, co
       const R den = rsqrt_free_sqrt<R>(vr * vr + wr * wr);
       if (den > (R)kDoubleTol * gg) {
         rx = ct * ur + stt * sph * den;
-        ry = ct * vr + stt * (gg * wr * cph - ur * vr * sph) / den;
-        rz = ct * wr - stt * (gg * vr * cph + ur * wr * sph) / den;
+        ry = ct * vr + rdiv<R>(stt * (gg * wr * cph - ur * vr * sph), den);
+        rz = ct * wr - rdiv<R>(stt * (gg * vr * cph + ur * wr * sph), den);
       } else {
         rx = ct * ur;
         ry = stt * cph * ur;
@@ -432,19 +511,20 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
 // Collision phase of one cell per workgroup of BS lanes.
 //   kLds: the cell's increments accumulate in LDS (dynamic shared memory after
 //         the axes); otherwise fd (a copy of f) takes global FP64 atomics.
-//   kF32: FP32 collision arithmetic and FP32 LDS accumulation; the change is
-//         written to dacc and k_conserve adds it to f with the FP64
-//         conservation correction. FP64: f + change is written to fd here.
-template <bool kLds, bool kF32, int BS>
+//   kF32: FP32 collision arithmetic; the FP64-accumulated change is written to
+//         dacc and k_conserve adds it to f with the FP64 conservation
+//         correction. FP64: f + change is written to fd here. (LDS atomics
+//         accumulate in FP64 in both paths: ds_add_f32 stalls far longer on
+//         the repeated addresses of a cell's distribution peak.)
+template <bool kLds, bool kF32, int BS, int K>
 __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime,
                                                 const double* __restrict__ f, double* __restrict__ fd,
-                                                float* __restrict__ dacc,
-                                                const double* __restrict__ eqv, const double* __restrict__ eqc,
-                                                const double* __restrict__ neqc, const int2* __restrict__ eqtab,
-                                                const int2* __restrict__ neqtab, const int* __restrict__ counts,
+                                                double* __restrict__ dacc,
+                                                const double* __restrict__ eqv, const double* __restrict__ cdf,
+                                                const int2* __restrict__ tabs, const int* __restrict__ counts,
                                                 const double* __restrict__ factors, int* __restrict__ err) {
   extern __shared__ double dyn[];
-  using Acc = typename std::conditional<kF32, float, double>::type;
+  using Acc = double;
   using R = typename std::conditional<kF32, float, double>::type;
   const int cell = blockIdx.x;
   const int N = g.N;
@@ -457,28 +537,46 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   CellView cv;
   cv.f = f + (size_t)cell * N;
   cv.eqv = eqv + (size_t)cell * N;
-  cv.eqc = eqc + (size_t)cell * (N + 1);
-  cv.neqc = neqc + (size_t)cell * (N + 1);
-  cv.eqtab = eqtab + (size_t)cell * kNBucket;
-  cv.neqtab = neqtab + (size_t)cell * kNBucket;
+  cv.eqc = cdf + (size_t)cell * 2 * (N + 1);
+  cv.neqc = cv.eqc + (N + 1);
+  cv.eqtab = tabs + (size_t)cell * 2 * kNBucket;
+  cv.neqtab = cv.eqtab + kNBucket;
+  cv.eq_total = cv.eqc[N];
+  cv.neq_total = cv.neqc[N];
   const int n0 = counts[cell * 4 + 0], n1 = counts[cell * 4 + 1], n2 = counts[cell * 4 + 2];
   const int total = n0 + n1 + n2;
   double* gd = fd + (size_t)cell * N;
   int bad = 0;
-  for (int q = threadIdx.x; q < total; q += BS) {
-    const int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
-    const int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
-    const double factor = factors[cell * 3 + t];
-    if (kLds) {
-      bad |= vr_sample<false, R>(g, cp, cv, cell, ntime, t, s, factor,
-                                 [&](int node, double val) {
-                                   if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (Acc)val); else bad = 1;
-                                 }, nullptr);
-    } else {
-      bad |= vr_sample<false, R>(g, cp, cv, cell, ntime, t, s, factor,
-                                 [&](int node, double val) {
-                                   if ((unsigned)node < (unsigned)N) atomicAdd(&gd[node], val); else bad = 1;
-                                 }, nullptr);
+  for (int base = 0; base < total; base += BS * K) {
+    int tt[K], ss[K], lA[K], lB[K];
+    bool valid[K];
+    double sgA[K], sgB[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = base + k * BS + (int)threadIdx.x;
+      valid[k] = q < total;
+      tt[k] = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
+      ss[k] = (tt[k] == 0) ? q : (tt[k] == 1 ? q - n0 : q - n0 - n1);
+    }
+    pick_batch<K>(cp, cv, N, cell, ntime, tt, ss, valid, lA, lB, sgA, sgB);
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+      if (!valid[k]) continue;
+      const double factor = factors[cell * 3 + tt[k]];
+      if (kLds) {
+        bad |= vr_sample<false, R>(g, cp, cell, ntime, tt[k], ss[k], factor, lA[k], lB[k], sgA[k], sgB[k],
+                                   [&](int node, double val) {
+                                     if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (Acc)val); else bad = 1;
+                                   }, nullptr);
+      } else {
+        // global FP64 atomics: into fd (a copy of f) in FP64, into the zeroed
+        // change buffer dacc in FP32 (k_conserve adds it to f)
+        double* tgt = kF32 ? dacc + (size_t)cell * N : gd;
+        bad |= vr_sample<false, R>(g, cp, cell, ntime, tt[k], ss[k], factor, lA[k], lB[k], sgA[k], sgB[k],
+                                   [&](int node, double val) {
+                                     if ((unsigned)node < (unsigned)N) atomicAdd(&tgt[node], val); else bad = 1;
+                                   }, nullptr);
+      }
     }
   }
   if (bad) atomicOr(err, 1);
@@ -487,13 +585,13 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   if constexpr (!kF32) {
     for (int l = threadIdx.x; l < N; l += BS) gd[l] = cv.f[l] + acc[l];
   } else {
-    float* da = dacc + (size_t)cell * N;
+    double* da = dacc + (size_t)cell * N;
     for (int l = threadIdx.x; l < N; l += BS) da[l] = acc[l];
   }
 }
 
-// FP64 conservation correction of the FP32-accumulated collision change d of
-// each cell: d is replaced by d - |f| (lambda . phi), phi = (1, v, |v|^2), with
+// FP64 conservation correction of the collision change d of each cell computed
+// with FP32 arithmetic: d is replaced by d - |f| (lambda . phi), phi = (1, v, |v|^2), with
 // lambda solving sum |f| phi phi^T lambda = sum d phi. In exact arithmetic
 // the collision step conserves mass, momentum and energy (each sample's
 // depletions equal its replenishments and the remap keeps moments 0-2); the
@@ -501,12 +599,12 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
 // FP64 round-off only. One workgroup per cell.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restrict__ f,
-                                                 const float* __restrict__ dacc, double* __restrict__ fd) {
+                                                 const double* __restrict__ dacc, double* __restrict__ fd) {
   __shared__ double red[BS / 64][20];
   __shared__ double lam[5];
   const int cell = blockIdx.x, N = g.N;
   const double* fc = f + (size_t)cell * N;
-  const float* da = dacc + (size_t)cell * N;
+  const double* da = dacc + (size_t)cell * N;
   double sums[20];
 #pragma unroll
   for (int z = 0; z < 20; ++z) sums[z] = 0.0;
@@ -515,7 +613,7 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
     node_ijk(l, g.n[0], g.n[1], i, j, k);
     const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
     const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
-    const double d = (double)da[l];
+    const double d = da[l];
     const double w = fabs(fc[l]);
 #pragma unroll
     for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
@@ -556,7 +654,7 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
     const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
     const double fl = fc[l];
     const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
-    out[l] = fl + ((double)da[l] - corr);
+    out[l] = fl + (da[l] - corr);
   }
 }
 
@@ -564,8 +662,7 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
 // instead of scattered.
 __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int ntime, int cell,
                                                    const double* __restrict__ f, const double* __restrict__ eqv,
-                                                   const double* __restrict__ eqc, const double* __restrict__ neqc,
-                                                   const int2* __restrict__ eqtab, const int2* __restrict__ neqtab,
+                                                   const double* __restrict__ cdf, const int2* __restrict__ tabs,
                                                    const int* __restrict__ counts, const double* __restrict__ factors,
                                                    long long max_records, int* __restrict__ irec,
                                                    double* __restrict__ drec) {
@@ -573,19 +670,27 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
   CellView cv;
   cv.f = f + (size_t)cell * N;
   cv.eqv = eqv + (size_t)cell * N;
-  cv.eqc = eqc + (size_t)cell * (N + 1);
-  cv.neqc = neqc + (size_t)cell * (N + 1);
-  cv.eqtab = eqtab + (size_t)cell * kNBucket;
-  cv.neqtab = neqtab + (size_t)cell * kNBucket;
+  cv.eqc = cdf + (size_t)cell * 2 * (N + 1);
+  cv.neqc = cv.eqc + (N + 1);
+  cv.eqtab = tabs + (size_t)cell * 2 * kNBucket;
+  cv.neqtab = cv.eqtab + kNBucket;
+  cv.eq_total = cv.eqc[N];
+  cv.neq_total = cv.neqc[N];
   const int n0 = counts[cell * 4 + 0], n1 = counts[cell * 4 + 1], n2 = counts[cell * 4 + 2];
   const int total = n0 + n1 + n2;
   for (int q = threadIdx.x + blockIdx.x * blockDim.x; q < total && q < max_records; q += blockDim.x * gridDim.x) {
-    const int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
-    const int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
+    int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
+    int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
     SampleRec rec;
     for (int z = 0; z < 7; ++z) { rec.st1[z] = 0; rec.st2[z] = 0; rec.m1[z] = 0.0; rec.m2[z] = 0.0; }
     rec.kd1 = 0.0; rec.kd2 = 0.0;
-    vr_sample<true, double>(g, cp, cv, cell, ntime, t, s, factors[cell * 3 + t], [](int, double) {}, &rec);
+    const bool valid = true;
+    int lA, lB;
+    double sgA, sgB;
+    pick_batch<1>(cp, cv, N, cell, ntime, &t, &s, &valid, &lA, &lB, &sgA, &sgB);
+    rec.lA = lA; rec.lB = lB; rec.skipped = 0;
+    vr_sample<true, double>(g, cp, cell, ntime, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB,
+                            [](int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
     for (int z = 0; z < 32; ++z) ir[z] = 0;

@@ -35,6 +35,20 @@ def hip_sync():
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
+def copy_calibration(nbytes=512 << 20):
+    """One device-to-device copy of a known size (read nbytes, write nbytes),
+    so that the HBM byte counters of a profile can be scaled against it."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def run_gpu(pkg, cfg, f, steps, warmup):
     """Times `steps` fused steps after `warmup` ones; returns wall seconds, mean
     per-phase device ms (HIP events on the library's stream) and the moments."""
@@ -94,11 +108,14 @@ def main():
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true", help="also run a 512 MiB device copy (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     pkg = importlib.import_module("2d_dvm_microflows_amd")
+    if args.copy_calib:
+        copy_calibration()
     cfg, f = pkg.lid_driven_cavity(nx=args.nx, ny=args.nx, n_v=args.nv, samples=args.samples)
     ncell, N = cfg.nx * cfg.ny, cfg.num_points
     updates = ncell * N * args.steps

@@ -91,6 +91,8 @@ def test_collision_trace_matches_oracle(pkg, case):
 
 STEP_CASES = {
     "homogeneous": lambda pkg: pkg.homogeneous_relaxation(n_v=15, samples=20000),
+    # 29^3 nodes: the FP64 slice exceeds LDS, increments go to global atomics
+    "homogeneous_global_atomics": lambda pkg: pkg.homogeneous_relaxation(n_v=29, samples=20000),
     "homogeneous_vss_nrepl3": lambda pkg: (lambda c, f: (c.with_(omega=0.81, alpha=1.4, num_repl=3), f))(
         *pkg.homogeneous_relaxation(n_v=11, samples=5000)),
     "cavity_diffuse": lambda pkg: pkg.lid_driven_cavity(nx=6, ny=5, n_v=9, vmax=3.0, samples=800),
