@@ -218,6 +218,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   }
   c->g.N = N;
   c->g.nxy = n[0] * n[1];
+  c->g.inv_n0 = 1.0 / (double)n[0];
+  c->g.inv_nxy = 1.0 / (double)(n[0] * n[1]);
   double b3min = c->beta[0][0] * c->beta[1][0] * c->beta[2][0];
   for (int k = 0; k < n[2]; ++k)
     for (int j = 0; j < n[1]; ++j)
@@ -493,13 +495,15 @@ static int walls_async(dvm_ctx* c) {
 
 static int sweep_async(dvm_ctx* c) {
   if (!convects(c)) return DVM_OK;
-  const size_t total = (size_t)c->ncell * c->N;
-  const unsigned blocks = (unsigned)((total + kConvBS - 1) / kConvBS);
+  // node chunks padded to a multiple of 8 (one XCD per chunk index, see k_convect2d)
+  const unsigned chunks = (unsigned)(((c->N + kConvBS - 1) / kConvBS + 7) / 8 * 8);
+  const unsigned ys = (unsigned)(c->ncell < 65535 ? c->ncell : 65535);
   if (c->cfg.dim == DVM_DIM_2D)
-    hipLaunchKernelGGL(k_convect2d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd, c->wallM,
-                       c->ratio, c->spec);
+    hipLaunchKernelGGL(k_convect2d, dim3(chunks, ys), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd,
+                       c->wallM, c->ratio, c->spec);
   else
-    hipLaunchKernelGGL(k_convect1d, dim3(blocks), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd, c->inflow);
+    hipLaunchKernelGGL(k_convect1d, dim3(chunks, ys), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd,
+                       c->inflow);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;  // phi := phi_conv
   return DVM_OK;

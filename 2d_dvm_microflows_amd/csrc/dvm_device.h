@@ -28,6 +28,7 @@ struct Grid {
   double vmin[3];  // first coordinate of each axis
   double factor[3];// (n-1)/(vmax-vmin): first guess of the nearest-node search
   int zero[3], neg[3], pos[3];  // find_zero_point results per axis
+  double inv_n0, inv_nxy;       // 1/n0 and 1/(n0*n1): node index decoding without integer division
   const double* ax[3];
   const double* beta[3];
 };

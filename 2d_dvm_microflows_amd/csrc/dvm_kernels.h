@@ -9,6 +9,7 @@ namespace dvm {
 
 constexpr int kMomBS = 256;      // moments / wall-flux workgroup
 constexpr int kCdfBS = 256;      // CDF workgroup
+constexpr int kCdfV = 8;         // consecutive nodes per lane in the CDF scan
 constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell's LDS slice
 constexpr int kConvBS = 256;     // convection lanes per workgroup
 constexpr int kAxisSlots = 384;  // LDS doubles reserved for the three axes (<= 128 points each)

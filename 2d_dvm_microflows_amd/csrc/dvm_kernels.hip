@@ -60,11 +60,15 @@ __device__ __forceinline__ double block_scan(double v, double* red, double& tota
   return inc + off;
 }
 
-__device__ __forceinline__ void node_ijk(int l, int nx, int ny, int& i, int& j, int& k) {
-  k = l / (nx * ny);
-  const int r = l - k * nx * ny;
-  j = r / nx;
-  i = r - j * nx;
+// Node l -> (i, j, k), i fastest. floor((l + 0.5) / n) is computed as
+// (l + 0.5) * (1/n) in FP64: for l < 2^24 the product is within 1e-9 of the
+// quotient, which itself is at least 0.5/n away from an integer, so the floor
+// is exact (no integer division, which is a long instruction sequence).
+__device__ __forceinline__ void node_ijk(const Grid& g, int l, int& i, int& j, int& k) {
+  k = (int)(((double)l + 0.5) * g.inv_nxy);
+  const int r = l - k * g.nxy;
+  j = (int)(((double)r + 0.5) * g.inv_n0);
+  i = r - j * g.n[0];
 }
 
 // --------------------------------------------------------------- moments
@@ -79,7 +83,7 @@ __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __rest
   double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, ss = 0.0;
   for (int l = threadIdx.x; l < g.N; l += kMomBS) {
     int i, j, k;
-    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    node_ijk(g, l, i, j, k);
     const double v = fc[l], x = X[i], y = Y[j], z = Z[k];
     s0 += v;
     sx += v * x;
@@ -101,7 +105,7 @@ __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __rest
   double st = 0.0;
   for (int l = threadIdx.x; l < g.N; l += kMomBS) {
     int i, j, k;
-    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    node_ijk(g, l, i, j, k);
     double cx = X[i] - u; cx = cx * cx;
     double cy = Y[j] - v; cy = cy * cy;
     double cz = Z[k] - w; cz = cz * cz;
@@ -122,23 +126,36 @@ __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __rest
 }
 
 // ------------------------------------------------------------------- CDF
-// Bucket brackets of the inverse-CDF search (variance_reduction.f90:468-493):
-// right[r] is the first l >= 1 with cumul[l] >= r*(0.001*total) (N+1 if none),
-// left[r] = right[r-1] - 1, left[1] = 0, right[1000] = N.
+// Bucket brackets of the inverse-CDF search (variance_reduction.f90:468-493).
+// The reference walks l upward over thresholds r*(0.001*total), so that
+// right[r] = first l >= 1 with cumul[l] >= r*mx (N+1 if none), left[r] =
+// right[r-1] - 1, left[1] = 0, right[1000] = N. For a non-decreasing CDF each
+// threshold is crossed at exactly one node, found here in one pass over the
+// nodes (each node tests the few thresholds in (cumul[l-1], cumul[l]]).
+template <int BS>
 __device__ __forceinline__ void build_buckets(const double* __restrict__ cumul, int N, int2* __restrict__ tab,
                                               int* __restrict__ rights) {
+  for (int r = threadIdx.x; r < kNBucket; r += BS) rights[r] = N + 1;
+  __syncthreads();
   const double mx = 0.001 * cumul[N];
-  for (int r = threadIdx.x + 1; r <= kNBucket; r += kCdfBS) {
-    const double t = (double)r * mx;
-    int lo = 1, hi = N + 1;  // first l in [1,N] with cumul[l] >= t, else N+1
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (t > cumul[mid]) lo = mid + 1; else hi = mid;
+  if (mx > 0.0) {
+    const double inv = 1.0 / mx;
+    for (int l = threadIdx.x + 1; l <= N; l += BS) {
+      const double c0 = cumul[l - 1], c1 = cumul[l];
+      if (!(c1 > c0)) continue;
+      int r0 = (int)fmin(c0 * inv, 2000.0), r1 = (int)fmin(c1 * inv, 2000.0) + 1;
+      r0 = r0 < 1 ? 1 : r0;
+      r1 = r1 > kNBucket ? kNBucket : r1;
+      for (int r = r0; r <= r1; ++r) {
+        const double t = (double)r * mx;
+        if (t > c0 && t <= c1) rights[r - 1] = l;
+      }
     }
-    rights[r - 1] = lo;
+  } else {
+    for (int r = threadIdx.x; r < kNBucket; r += BS) rights[r] = 1;
   }
   __syncthreads();
-  for (int r = threadIdx.x + 1; r <= kNBucket; r += kCdfBS) {
+  for (int r = threadIdx.x + 1; r <= kNBucket; r += BS) {
     const int left = (r == 1) ? 0 : rights[r - 2] - 1;
     const int right = (r == kNBucket) ? N : rights[r - 1];
     tab[r - 1] = make_int2(left, right);
@@ -146,16 +163,33 @@ __device__ __forceinline__ void build_buckets(const double* __restrict__ cumul, 
   __syncthreads();
 }
 
+// Workgroup-wide scan of V consecutive values per lane: v[q] becomes
+// carry + (prefix of everything before this lane in the chunk) + the lane's
+// own running sum. Returns the chunk total.
+template <int BS, int V>
+__device__ __forceinline__ double chunk_scan(double (&v)[V], double* red, double carry) {
+#pragma unroll
+  for (int q = 1; q < V; ++q) v[q] = v[q - 1] + v[q];
+  double tot;
+  const double inc = block_scan<BS>(v[V - 1], red, tot);
+  const double excl = carry + (inc - v[V - 1]);
+#pragma unroll
+  for (int q = 0; q < V; ++q) v[q] = excl + v[q];
+  return tot;
+}
+
 // compute_cdf_vr (variance_reduction.f90:234-496) for the like-species pair,
 // plus compute_num_coll_pairs (collision_utilities.f90:106-140) and the
 // depletion factors (variance_reduction.f90:598-604,679,831,976).
+// One workgroup per cell; lane t owns kCdfV consecutive nodes of each chunk.
 __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
                                                 const double* __restrict__ props, double* __restrict__ eqv,
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
-                                                double* __restrict__ eqpar, int* __restrict__ counts,
-                                                double* __restrict__ factors) {
+                                                double* __restrict__ eqpar,
+                                                int* __restrict__ counts, double* __restrict__ factors) {
+  constexpr int V = kCdfV;
   __shared__ double red[kCdfBS / 64];
-  __shared__ double pref[kCdfBS + 1];
+  __shared__ double lastp[kCdfBS];
   __shared__ int rights[kNBucket];
   __shared__ int s_new;
   const int cell = blockIdx.x;
@@ -171,7 +205,8 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
   const double mass = cp.mass;
   if (threadIdx.x == 0) {
     const double tol = cp.cdf_tolerance;
-    // Q4: w is not compared; a negative u or v makes its test always true
+    // the equilibrium cache test: w is not compared; a negative u or v makes
+    // its test always true
     s_new = (fabs(dens - par[0]) > tol * dens || fabs(u - par[1]) > tol * u ||
              fabs(v - par[2]) > tol * v || fabs(temp - par[4]) > tol * temp) ? 1 : 0;
   }
@@ -181,45 +216,54 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
   if (new_eq) {
     double coeff = 0.0;
     if (dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
-    // eq_df = prefix of f_eq*beta^3; eqv = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
-    double carry = 0.0, carry_c = 0.0;
+    // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
+    double carry = 0.0, carry_c = 0.0, prev_last = 0.0;
     if (threadIdx.x == 0) ec[0] = 0.0;
-    for (int base = 0; base < N; base += kCdfBS) {
-      const int l = base + threadIdx.x;
-      double val = 0.0;
-      if (l < N) {
-        int i, j, k;
-        node_ijk(l, g.n[0], g.n[1], i, j, k);
-        const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-        const double beta3 = g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
-        double phi_eq = 0.0;
-        if (temp > 0.0) {
-          double cx = (x - u); cx = cx * cx;
-          double cy = (y - v); cy = cy * cy;
-          double cz = (z - w); cz = cz * cz;
-          const double csq = cx + cy + cz;
-          phi_eq = coeff * exp(-csq * mass / temp);
+    for (int base = 0; base < N; base += kCdfBS * V) {
+      const int l0 = base + threadIdx.x * V;
+      double pv[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const int l = l0 + q;
+        pv[q] = 0.0;
+        if (l < N) {
+          int i, j, k;
+          node_ijk(g, l, i, j, k);
+          const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+          const double beta3 = g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
+          double phi_eq = 0.0;
+          if (temp > 0.0) {
+            double cx = (x - u); cx = cx * cx;
+            double cy = (y - v); cy = cy * cy;
+            double cz = (z - w); cz = cz * cz;
+            const double csq = cx + cy + cz;
+            phi_eq = coeff * exp(-csq * mass / temp);
+          }
+          pv[q] = phi_eq * beta3;
         }
-        val = phi_eq * beta3;
       }
-      double tot;
-      const double pre = carry + block_scan<kCdfBS>(val, red, tot);
-      if (threadIdx.x == 0) pref[0] = carry;
-      pref[threadIdx.x + 1] = pre;
-      __syncthreads();
-      const double e = (l < N) ? pref[threadIdx.x + 1] - pref[threadIdx.x] : 0.0;
-      __syncthreads();
+      const double tot = chunk_scan<kCdfBS, V>(pv, red, carry);
       carry = carry + tot;
-      double tot_c;
-      const double prc = carry_c + block_scan<kCdfBS>(e, red, tot_c);
+      lastp[threadIdx.x] = pv[V - 1];
+      __syncthreads();
+      const double before = (threadIdx.x == 0) ? prev_last : lastp[threadIdx.x - 1];
+      const double chunk_last = lastp[kCdfBS - 1];
+      __syncthreads();
+      prev_last = chunk_last;
+      double e[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) e[q] = (l0 + q < N) ? pv[q] - (q == 0 ? before : pv[q - 1]) : 0.0;
+#pragma unroll
+      for (int q = 0; q < V; ++q)
+        if (l0 + q < N) ev[l0 + q] = e[q];
+      const double tot_c = chunk_scan<kCdfBS, V>(e, red, carry_c);
       carry_c = carry_c + tot_c;
-      if (l < N) {
-        ev[l] = e;
-        ec[l + 1] = prc;
-      }
+#pragma unroll
+      for (int q = 0; q < V; ++q)
+        if (l0 + q < N) ec[l0 + q + 1] = e[q];
     }
     __syncthreads();
-    build_buckets(ec, N, tabs + (size_t)cell * 2 * kNBucket, rights);
+    build_buckets<kCdfBS>(ec, N, tabs + (size_t)cell * 2 * kNBucket, rights);
     if (threadIdx.x == 0) {
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
     }
@@ -227,16 +271,19 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
   // |f - f_eq| prefix
   double carry = 0.0;
   if (threadIdx.x == 0) nc[0] = 0.0;
-  for (int base = 0; base < N; base += kCdfBS) {
-    const int l = base + threadIdx.x;
-    const double d = (l < N) ? fabs(fc[l] - ev[l]) : 0.0;
-    double tot;
-    const double pre = carry + block_scan<kCdfBS>(d, red, tot);
+  for (int base = 0; base < N; base += kCdfBS * V) {
+    const int l0 = base + threadIdx.x * V;
+    double d[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) d[q] = (l0 + q < N) ? fabs(fc[l0 + q] - ev[l0 + q]) : 0.0;
+    const double tot = chunk_scan<kCdfBS, V>(d, red, carry);
     carry = carry + tot;
-    if (l < N) nc[l + 1] = pre;
+#pragma unroll
+    for (int q = 0; q < V; ++q)
+      if (l0 + q < N) nc[l0 + q + 1] = d[q];
   }
   __syncthreads();
-  build_buckets(nc, N, tabs + (size_t)cell * 2 * kNBucket + kNBucket, rights);
+  build_buckets<kCdfBS>(nc, N, tabs + (size_t)cell * 2 * kNBucket + kNBucket, rights);
   if (threadIdx.x == 0) {
     const double neq_dens = nc[N], eq_dens = ec[N];
     const double h = 0.5 * neq_dens / dens;
@@ -380,8 +427,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
   if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) return 1;  // unreachable for finite CDFs
   int iA, jA, kA, iB, jB, kB;
-  node_ijk(lA, nx, g.n[1], iA, jA, kA);
-  node_ijk(lB, nx, g.n[1], iB, jB, kB);
+  node_ijk(g, lA, iA, jA, kA);
+  node_ijk(g, lB, iB, jB, kB);
   const R x1 = (R)g.ax[0][iA], y1 = (R)g.ax[1][jA], z1 = (R)g.ax[2][kA];
   const R x2 = (R)g.ax[0][iB], y2 = (R)g.ax[1][jB], z2 = (R)g.ax[2][kB];
   if (kRecord) { rec->lA = lA; rec->lB = lB; rec->skipped = 0; }
@@ -610,7 +657,7 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
   for (int z = 0; z < 20; ++z) sums[z] = 0.0;
   for (int l = threadIdx.x; l < N; l += BS) {
     int i, j, k;
-    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    node_ijk(g, l, i, j, k);
     const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
     const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
     const double d = da[l];
@@ -650,7 +697,7 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
   double* out = fd + (size_t)cell * N;
   for (int l = threadIdx.x; l < N; l += BS) {
     int i, j, k;
-    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    node_ijk(g, l, i, j, k);
     const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
     const double fl = fc[l];
     const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
@@ -735,7 +782,7 @@ __global__ void __launch_bounds__(kMomBS) k_wall_ratio(Grid g, ConvParams p, con
   double s = 0.0;
   for (int l = threadIdx.x; l < g.N; l += kMomBS) {
     int i, j, k;
-    node_ijk(l, g.n[0], g.n[1], i, j, k);
+    node_ijk(g, l, i, j, k);
     bool in;
     double vn;
     if (side == 0) { in = i <= g.neg[0]; vn = g.ax[0][i]; }
@@ -759,7 +806,7 @@ __global__ void k_spec_ghost(Grid g, ConvParams p, const double* __restrict__ f,
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= g.N) return;
   int i, j, k;
-  node_ijk(l, g.n[0], g.n[1], i, j, k);
+  node_ijk(g, l, i, j, k);
   bool in;
   if (side == 0) in = i <= g.neg[0];
   else if (side == 1) in = i >= g.pos[0];
@@ -815,20 +862,31 @@ __device__ __forceinline__ double ghost(const ConvParams& p, const double* __res
 
 // 2-D upwind sweep: interior cells (convection.f90:119-276) and boundary cells
 // (boundary_conditions.f90:1548-1643) in one pass, one lane per (cell, node).
+// Grid: x = node chunks of kConvBS nodes, padded to a multiple of 8, y = cells
+// (strided when there are more than gridDim.y). Workgroups are dispatched to
+// the 8 XCDs round-robin, so with a chunk count divisible by 8 every cell's
+// chunk j runs on XCD j % 8 and the neighbour re-reads of that chunk (x and y
+// upwind cells) hit that XCD's L2.
 __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, const double* __restrict__ f,
                                                        double* __restrict__ fo, const double* __restrict__ wallM,
                                                        const double* __restrict__ ratio,
                                                        const double* __restrict__ spec) {
   const int N = g.N;
-  const size_t total = (size_t)p.nx * p.ny * N;
-  for (size_t idx = (size_t)blockIdx.x * kConvBS + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kConvBS) {
-    const int cell = (int)(idx / N);
-    const int l = (int)(idx - (size_t)cell * N);
+  const int l = blockIdx.x * kConvBS + threadIdx.x;
+  if (l >= N) return;
+  int i, j, k;
+  node_ijk(g, l, i, j, k);
+  const double xv = g.ax[0][i], yv = g.ax[1][j];
+  const double cfl_x = p.fx * xv, cfl_y = p.fy * yv;
+  // x region: 0 upwind from the right ghost, 1 no x term (zero plane), 2 from the left ghost
+  int xr = -1, yr = -1;
+  if (i <= g.neg[0]) xr = 0; else if (p.izflag && i == g.zero[0]) xr = 1; else if (i >= g.pos[0]) xr = 2;
+  if (j <= g.neg[1]) yr = 0; else if (p.jzflag && j == g.zero[1]) yr = 1; else if (j >= g.pos[1]) yr = 2;
+  const int ncell = p.nx * p.ny;
+  for (int cell = blockIdx.y; cell < ncell; cell += gridDim.y) {
     const int ix = cell % p.nx, iy = cell / p.nx;
-    const int i = l % g.n[0], j = (l / g.n[0]) % g.n[1];
-    const double xv = g.ax[0][i], yv = g.ax[1][j];
+    const size_t idx = (size_t)cell * N + l;
     const double f0 = f[idx];
-    const double cfl_x = p.fx * xv, cfl_y = p.fy * yv;
     const bool boundary = ix == 0 || iy == 0 || ix == p.nx - 1 || iy == p.ny - 1;
     if (!boundary) {
       const double px = (xv < 0.0) ? f[idx + N] : f[idx - N];
@@ -836,10 +894,6 @@ __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, con
       fo[idx] = f0 + fabs(cfl_x) * (px - f0) + fabs(cfl_y) * (py - f0);
       continue;
     }
-    // x region: 0 upwind from the right ghost, 1 no x term (zero plane), 2 from the left ghost
-    int xr = -1, yr = -1;
-    if (i <= g.neg[0]) xr = 0; else if (p.izflag && i == g.zero[0]) xr = 1; else if (i >= g.pos[0]) xr = 2;
-    if (j <= g.neg[1]) yr = 0; else if (p.jzflag && j == g.zero[1]) yr = 1; else if (j >= g.pos[1]) yr = 2;
     if (xr < 0 || yr < 0) { fo[idx] = 0.0; continue; }
     double out = f0;
     if (xr != 1) {
@@ -866,12 +920,14 @@ __device__ __forceinline__ double upwind1d(const double* __restrict__ f, size_t 
 __global__ void __launch_bounds__(kConvBS) k_convect1d(Grid g, ConvParams p, const double* __restrict__ f,
                                                        double* __restrict__ fo, const double* __restrict__ inflow) {
   const int N = g.N;
-  const size_t total = (size_t)p.nx * N;
-  for (size_t idx = (size_t)blockIdx.x * kConvBS + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kConvBS) {
-    const int ix = (int)(idx / N);
-    const int l = (int)(idx - (size_t)ix * N);
-    const double xv = g.ax[0][l % g.n[0]];
-    const double cfl = p.fx * xv;
+  const int l = blockIdx.x * kConvBS + threadIdx.x;
+  if (l >= N) return;
+  int i, j, k;
+  node_ijk(g, l, i, j, k);
+  const double xv = g.ax[0][i];
+  const double cfl = p.fx * xv;
+  for (int ix = blockIdx.y; ix < p.nx; ix += gridDim.y) {
+    const size_t idx = (size_t)ix * N + l;
     if (ix > 0 && ix < p.nx - 1) { fo[idx] = upwind1d(f, idx, N, cfl, xv); continue; }
     const int side = (ix == 0) ? 0 : 1;
     const int bc = p.bc[side];

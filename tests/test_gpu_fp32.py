@@ -50,29 +50,41 @@ def _builders(pkg, case):
 
 @pytest.mark.parametrize("case", ["homogeneous", "cavity"])
 def test_fp32_step_moments_within_1e5_of_fp64(pkg, case):
-    """Same state, same seed: FP32-collision steps track the FP64 path to
-    1e-5 in every moment field while both draw the same sample counts. The
-    per-integral counts nint(S*w_t/sum w) are recomputed from each path's own
-    f, so after enough steps a count can differ by one in some cell, which
-    rescales that integral's weights by ~1/n; the comparison stops there."""
+    """Per-step precision of the FP32-collision path: from the same state and
+    seed, one FP32 step and one FP64 step agree in every moment field to far
+    below 1e-5 (the partner picks are FP64 in both, so the samples coincide;
+    a pick can differ only when a uniform lands within rounding of a CDF
+    boundary). The FP64 trajectory is advanced and re-shared every step."""
     cfg, f = _builders(pkg, case)
-    print(f"[fp32 vs fp64] {cfg.name}", flush=True)
+    print(f"[fp32 vs fp64, per step] {cfg.name}", flush=True)
+    a = pkg.Solver(cfg)
+    b = pkg.Solver(cfg.with_(precision=FP32))
+    a.initialize(f)
+    worst = 0.0
+    for n in range(1, 9):
+        state = a.download_f()
+        b.initialize(state)
+        a.step(n)
+        b.step(n)
+        err = _moment_err(a.calculate_physical_properties(), b.calculate_physical_properties())
+        print(f"  step {n}: max moment rel err {err:.2e}", flush=True)
+        worst = max(worst, err)
+    assert worst <= 1e-6, worst
+
+
+def test_fp32_trajectory_tracks_fp64(pkg):
+    """Same seed over several steps on the cavity: the FP32 run stays within
+    1e-5 of the FP64 run in every moment field (pick flips near CDF
+    boundaries make the two Monte Carlo trajectories drift apart slowly)."""
+    cfg, f = pkg.lid_driven_cavity(nx=16, ny=16, n_v=15, vmax=4.0, samples=20000)
+    print(f"[fp32 vs fp64, trajectory] {cfg.name}", flush=True)
     a = pkg.Solver(cfg)
     b = pkg.Solver(cfg.with_(precision=FP32))
     a.initialize(f)
     b.initialize(f)
-    same_counts_steps = 0
-    for n in range(1, 11):
-        ca = a.compute_collision_integral(n)
-        cb = b.compute_collision_integral(n)
-        a.compute_convection(n)
-        b.compute_convection(n)
-        pa = a.calculate_physical_properties()
-        pb = b.calculate_physical_properties()
-        if not np.array_equal(ca, cb):
-            break
-        same_counts_steps = n
-        err = _moment_err(pa, pb)
+    for n in range(1, 6):
+        a.step(n)
+        b.step(n)
+        err = _moment_err(a.calculate_physical_properties(), b.calculate_physical_properties())
         print(f"  step {n}: max moment rel err {err:.2e}", flush=True)
         assert err <= 1e-5, (n, err)
-    assert same_counts_steps >= 2
