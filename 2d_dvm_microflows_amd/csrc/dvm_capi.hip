@@ -115,6 +115,7 @@ static int validate(const dvm_config* cfg) {
   if (cfg->dim == DVM_DIM_0D && (cfg->nx != 1 || cfg->ny != 1))
     return fail(nullptr, DVM_E_ARG, "space-homogeneous runs have one cell");
   if (cfg->dim == DVM_DIM_1D && cfg->ny != 1) return fail(nullptr, DVM_E_ARG, "1-D runs need ny = 1");
+  if (cfg->nx > 65535) return fail(nullptr, DVM_E_ARG, "nx must be <= 65535");
   if (cfg->dim == DVM_DIM_1D && cfg->enable_convection && cfg->nx < 3)
     return fail(nullptr, DVM_E_ARG, "1-D convection needs nx >= 3");
   const int n[3] = {cfg->nvx, cfg->nvy, cfg->nvz};
@@ -499,8 +500,8 @@ static int sweep_async(dvm_ctx* c) {
   const unsigned chunks = (unsigned)(((c->N + kConvBS - 1) / kConvBS + 7) / 8 * 8);
   const unsigned ys = (unsigned)(c->ncell < 65535 ? c->ncell : 65535);
   if (c->cfg.dim == DVM_DIM_2D)
-    hipLaunchKernelGGL(k_convect2d, dim3(chunks, ys), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd,
-                       c->wallM, c->ratio, c->spec);
+    hipLaunchKernelGGL(k_convect2d, dim3(chunks, c->cfg.nx), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f,
+                       c->fd, c->wallM, c->ratio, c->spec);
   else
     hipLaunchKernelGGL(k_convect1d, dim3(chunks, ys), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd,
                        c->inflow);

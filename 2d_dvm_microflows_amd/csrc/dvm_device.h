@@ -50,18 +50,16 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// Two FP64 uniforms in [0,1) with 53 random bits each, for counter
-// (sample, tag, cell, ntime): tag = integral (1..3) | slot << 2, slot 0 for
-// the partner picks and r = 1..NRepl for the scattering angles of location r.
-__device__ __forceinline__ void uniforms2(uint64_t seed, uint32_t sample, uint32_t tag, uint32_t cell,
-                                          uint32_t ntime, double& ua, double& ub) {
-  uint32_t o[4];
-  philox4x32_10(sample, tag, cell, ntime, (uint32_t)seed, (uint32_t)(seed >> 32), o);
-  const uint64_t a = ((uint64_t)o[1] << 32) | o[0];
-  const uint64_t b = ((uint64_t)o[3] << 32) | o[2];
-  ua = (double)(a >> 11) * 0x1.0p-53;
-  ub = (double)(b >> 11) * 0x1.0p-53;
+// Philox block of sample `sample` for counter (sample, tag, cell, ntime):
+// tag = integral (1..3) | slot << 2. Slot 0 gives (u_A, u_B, u_phi, u_theta)
+// of replenish location 1, slot r gives (u_phi, u_theta) of location r >= 2.
+// A uniform is word * 2^-32 in [0,1), the resolution of the reference's
+// gsl_rng_uniform on mt19937.
+__device__ __forceinline__ void philox_block(uint64_t seed, uint32_t sample, uint32_t tag, uint32_t cell,
+                                             uint32_t ntime, uint32_t out[4]) {
+  philox4x32_10(sample, tag, cell, ntime, (uint32_t)seed, (uint32_t)(seed >> 32), out);
 }
+__device__ __forceinline__ double u32_to_uniform(uint32_t x) { return (double)x * 0x1.0p-32; }
 
 // ---------------------------------------------------------------- signs
 // isgn / dsgn (math_utils.f90:38-60): sign(1, x) with sign(1, -0.0) = -1.

@@ -73,46 +73,50 @@ __device__ __forceinline__ void node_ijk(const Grid& g, int l, int& i, int& j, i
 
 // --------------------------------------------------------------- moments
 // calculate_physical_properties (physical_props.f90:285-485) for one
-// monatomic species: n, u, v, w, <|v|>, T and the derived fields.
+// monatomic species: n, u, v, w, <|v|>, T and the derived fields, in one pass
+// over f. The reference's second pass sum f |c - u|^2 is formed as
+// sum f |c|^2 - u . sum f c (algebraically equal; the distribution is read once).
 __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
                                                     double* __restrict__ props, double mass) {
-  __shared__ double red[kMomBS / 64];
+  __shared__ double red[6][kMomBS / 64];
   const int cell = blockIdx.x;
   const double* fc = f + (size_t)cell * g.N;
   const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
-  double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, ss = 0.0;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int l = threadIdx.x; l < g.N; l += kMomBS) {
     int i, j, k;
     node_ijk(g, l, i, j, k);
     const double v = fc[l], x = X[i], y = Y[j], z = Z[k];
-    s0 += v;
-    sx += v * x;
-    sy += v * y;
-    sz += v * z;
-    ss += v * sqrt(x * x + y * y + z * z);
+    const double c2 = x * x + y * y + z * z;
+    acc[0] += v;
+    acc[1] += v * x;
+    acc[2] += v * y;
+    acc[3] += v * z;
+    acc[4] += v * c2;
+    acc[5] += v * sqrt(c2);
   }
-  const double dens = block_sum<kMomBS>(s0, red);
-  sx = block_sum<kMomBS>(sx, red);
-  sy = block_sum<kMomBS>(sy, red);
-  sz = block_sum<kMomBS>(sz, red);
-  ss = block_sum<kMomBS>(ss, red);
-  double* p = props + (size_t)cell * DVM_NPROP;
-  if (dens == 0.0) {
-    if (threadIdx.x < DVM_NPROP) p[threadIdx.x] = 0.0;
-    return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double t = wave_sum(acc[q]);
+    if (lane == 0) red[q][wid] = t;
   }
-  const double u = sx / dens, v = sy / dens, w = sz / dens;
-  double st = 0.0;
-  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    double cx = X[i] - u; cx = cx * cx;
-    double cy = Y[j] - v; cy = cy * cy;
-    double cz = Z[k] - w; cz = cz * cz;
-    st += fc[l] * (cx + cy + cz);
-  }
-  st = block_sum<kMomBS>(st, red);
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double tot[6];
+    for (int q = 0; q < 6; ++q) {
+      double t = 0.0;
+      for (int w = 0; w < kMomBS / 64; ++w) t += red[q][w];
+      tot[q] = t;
+    }
+    double* p = props + (size_t)cell * DVM_NPROP;
+    const double dens = tot[0];
+    if (dens == 0.0) {
+      for (int q = 0; q < DVM_NPROP; ++q) p[q] = 0.0;
+      return;
+    }
+    const double u = tot[1] / dens, v = tot[2] / dens, w = tot[3] / dens;
+    const double st = tot[4] - (u * tot[1] + v * tot[2] + w * tot[3]);
     const double t = 2.0 * mass * st / (3.0 * dens);
     p[DVM_P_DENS] = dens;
     p[DVM_P_U] = u;
@@ -121,7 +125,7 @@ __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __rest
     p[DVM_P_TRTEMP] = t;
     p[DVM_P_TEMP] = (3.0 * t + 0.0 + 0.0) / 3.0;  // compute_overall_temperature, no internal dof
     p[DVM_P_PRESSURE] = 0.5 * dens * t;
-    p[DVM_P_AVGSPEED] = ss / dens;
+    p[DVM_P_AVGSPEED] = tot[5] / dens;
   }
 }
 
@@ -178,22 +182,31 @@ __device__ __forceinline__ double chunk_scan(double (&v)[V], double* red, double
   return tot;
 }
 
+// Position in the LDS chunk tile of node m of a chunk: lane m / V owns V
+// consecutive nodes, rows padded to V + 1 doubles against bank conflicts.
+__device__ __forceinline__ int tile_pos(int m) { return (m / kCdfV) * (kCdfV + 1) + (m % kCdfV); }
+
 // compute_cdf_vr (variance_reduction.f90:234-496) for the like-species pair,
 // plus compute_num_coll_pairs (collision_utilities.f90:106-140) and the
 // depletion factors (variance_reduction.f90:598-604,679,831,976).
-// One workgroup per cell; lane t owns kCdfV consecutive nodes of each chunk.
+// One workgroup per cell. Each chunk of kCdfBS*kCdfV nodes is loaded and
+// stored coalesced through an LDS tile in which lane t owns kCdfV consecutive
+// nodes for the scan.
 __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
                                                 const double* __restrict__ props, double* __restrict__ eqv,
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
                                                 double* __restrict__ eqpar,
                                                 int* __restrict__ counts, double* __restrict__ factors) {
   constexpr int V = kCdfV;
+  constexpr int CH = kCdfBS * V;
   __shared__ double red[kCdfBS / 64];
   __shared__ double lastp[kCdfBS];
+  __shared__ double tile[kCdfBS * (V + 1)];
   __shared__ int rights[kNBucket];
   __shared__ int s_new;
   const int cell = blockIdx.x;
   const int N = g.N;
+  const int own = threadIdx.x * (V + 1);
   const double* p = props + (size_t)cell * DVM_NPROP;
   const double* fc = f + (size_t)cell * N;
   double* ev = eqv + (size_t)cell * N;
@@ -219,13 +232,11 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
     // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
     double carry = 0.0, carry_c = 0.0, prev_last = 0.0;
     if (threadIdx.x == 0) ec[0] = 0.0;
-    for (int base = 0; base < N; base += kCdfBS * V) {
-      const int l0 = base + threadIdx.x * V;
-      double pv[V];
+    for (int base = 0; base < N; base += CH) {
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        const int l = l0 + q;
-        pv[q] = 0.0;
+        const int m = q * kCdfBS + threadIdx.x, l = base + m;
+        double val = 0.0;
         if (l < N) {
           int i, j, k;
           node_ijk(g, l, i, j, k);
@@ -239,30 +250,42 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
             const double csq = cx + cy + cz;
             phi_eq = coeff * exp(-csq * mass / temp);
           }
-          pv[q] = phi_eq * beta3;
+          val = phi_eq * beta3;
         }
+        tile[tile_pos(m)] = val;
       }
-      const double tot = chunk_scan<kCdfBS, V>(pv, red, carry);
-      carry = carry + tot;
+      __syncthreads();
+      double pv[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) pv[q] = tile[own + q];
+      carry = carry + chunk_scan<kCdfBS, V>(pv, red, carry);
       lastp[threadIdx.x] = pv[V - 1];
       __syncthreads();
       const double before = (threadIdx.x == 0) ? prev_last : lastp[threadIdx.x - 1];
-      const double chunk_last = lastp[kCdfBS - 1];
-      __syncthreads();
-      prev_last = chunk_last;
+      prev_last = lastp[kCdfBS - 1];
       double e[V];
 #pragma unroll
-      for (int q = 0; q < V; ++q) e[q] = (l0 + q < N) ? pv[q] - (q == 0 ? before : pv[q - 1]) : 0.0;
+      for (int q = 0; q < V; ++q) {
+        e[q] = pv[q] - (q == 0 ? before : pv[q - 1]);
+        tile[own + q] = e[q];
+      }
+      __syncthreads();
 #pragma unroll
-      for (int q = 0; q < V; ++q)
-        if (l0 + q < N) ev[l0 + q] = e[q];
-      const double tot_c = chunk_scan<kCdfBS, V>(e, red, carry_c);
-      carry_c = carry_c + tot_c;
+      for (int q = 0; q < V; ++q) {
+        const int m = q * kCdfBS + threadIdx.x, l = base + m;
+        if (l < N) ev[l] = tile[tile_pos(m)];
+      }
+      carry_c = carry_c + chunk_scan<kCdfBS, V>(e, red, carry_c);
 #pragma unroll
-      for (int q = 0; q < V; ++q)
-        if (l0 + q < N) ec[l0 + q + 1] = e[q];
+      for (int q = 0; q < V; ++q) tile[own + q] = e[q];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const int m = q * kCdfBS + threadIdx.x, l = base + m;
+        if (l < N) ec[l + 1] = tile[tile_pos(m)];
+      }
+      __syncthreads();
     }
-    __syncthreads();
     build_buckets<kCdfBS>(ec, N, tabs + (size_t)cell * 2 * kNBucket, rights);
     if (threadIdx.x == 0) {
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
@@ -271,18 +294,27 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
   // |f - f_eq| prefix
   double carry = 0.0;
   if (threadIdx.x == 0) nc[0] = 0.0;
-  for (int base = 0; base < N; base += kCdfBS * V) {
-    const int l0 = base + threadIdx.x * V;
+  for (int base = 0; base < N; base += CH) {
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int m = q * kCdfBS + threadIdx.x, l = base + m;
+      tile[tile_pos(m)] = (l < N) ? fabs(fc[l] - ev[l]) : 0.0;
+    }
+    __syncthreads();
     double d[V];
 #pragma unroll
-    for (int q = 0; q < V; ++q) d[q] = (l0 + q < N) ? fabs(fc[l0 + q] - ev[l0 + q]) : 0.0;
-    const double tot = chunk_scan<kCdfBS, V>(d, red, carry);
-    carry = carry + tot;
+    for (int q = 0; q < V; ++q) d[q] = tile[own + q];
+    carry = carry + chunk_scan<kCdfBS, V>(d, red, carry);
 #pragma unroll
-    for (int q = 0; q < V; ++q)
-      if (l0 + q < N) nc[l0 + q + 1] = d[q];
+    for (int q = 0; q < V; ++q) tile[own + q] = d[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int m = q * kCdfBS + threadIdx.x, l = base + m;
+      if (l < N) nc[l + 1] = tile[tile_pos(m)];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   build_buckets<kCdfBS>(nc, N, tabs + (size_t)cell * 2 * kNBucket + kNBucket, rights);
   if (threadIdx.x == 0) {
     const double neq_dens = nc[N], eq_dens = ec[N];
@@ -346,7 +378,7 @@ __device__ __forceinline__ R rpow(R x, R y) {
 template <int K>
 __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView& cv, int N, int cell, int ntime,
                                            const int* t, const int* s, const bool* valid, int* lA, int* lB,
-                                           double* sgA, double* sgB) {
+                                           double* sgA, double* sgB, uint32_t* wphi, uint32_t* wth) {
   constexpr int M = 2 * K;
   // the cell's two CDFs are adjacent: eq at cv.eqc, deviation at cv.eqc + N + 1
   int off[M], left[M], right[M], idx[M];
@@ -355,8 +387,11 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
   int2 lr[M];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    double uA = 0.0, uB = 0.0;
-    if (valid[k]) uniforms2(cp.seed, (uint32_t)s[k], (uint32_t)(t[k] + 1), (uint32_t)cell, (uint32_t)ntime, uA, uB);
+    uint32_t w4[4] = {0u, 0u, 0u, 0u};
+    if (valid[k]) philox_block(cp.seed, (uint32_t)s[k], (uint32_t)(t[k] + 1), (uint32_t)cell, (uint32_t)ntime, w4);
+    const double uA = u32_to_uniform(w4[0]), uB = u32_to_uniform(w4[1]);
+    wphi[k] = w4[2];
+    wth[k] = w4[3];
     const bool eqA = t[k] == 0, eqB = t[k] == 1;
     off[2 * k] = eqA ? 0 : N + 1;
     off[2 * k + 1] = eqB ? 0 : N + 1;
@@ -423,7 +458,7 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
 template <bool kRecord, class R, class Scatter>
 __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, int cell, int ntime, int t, int s,
                                          double factor, int lA, int lB, double signA_raw, double signB_raw,
-                                         Scatter&& scatter, SampleRec* rec) {
+                                         uint32_t wphi, uint32_t wth, Scatter&& scatter, SampleRec* rec) {
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
   if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) return 1;  // unreachable for finite CDFs
   int iA, jA, kA, iB, jB, kB;
@@ -464,8 +499,13 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
   const R gr1 = rdiv<R>(gp1, gg), gr2 = rdiv<R>(gp2, gg);
   int bad = 0;
   for (int r = 1; r <= cp.num_repl; ++r) {
-    double uphi, uth;
-    uniforms2(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, uphi, uth);
+    double uphi = u32_to_uniform(wphi), uth = u32_to_uniform(wth);
+    if (r > 1) {
+      uint32_t w4[4];
+      philox_block(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, w4);
+      uphi = u32_to_uniform(w4[0]);
+      uth = u32_to_uniform(w4[1]);
+    }
     // pick_random_velocity_on_sphere (collision_utilities.f90:284-352)
     const R ph = (R)2 * (R)kPiRef * (R)uphi;
     R sph, cph;
@@ -598,6 +638,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
     int tt[K], ss[K], lA[K], lB[K];
     bool valid[K];
     double sgA[K], sgB[K];
+    uint32_t wphi[K], wth[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int q = base + k * BS + (int)threadIdx.x;
@@ -605,14 +646,14 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
       tt[k] = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
       ss[k] = (tt[k] == 0) ? q : (tt[k] == 1 ? q - n0 : q - n0 - n1);
     }
-    pick_batch<K>(cp, cv, N, cell, ntime, tt, ss, valid, lA, lB, sgA, sgB);
+    pick_batch<K>(cp, cv, N, cell, ntime, tt, ss, valid, lA, lB, sgA, sgB, wphi, wth);
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
       if (!valid[k]) continue;
       const double factor = factors[cell * 3 + tt[k]];
       if (kLds) {
         bad |= vr_sample<false, R>(g, cp, cell, ntime, tt[k], ss[k], factor, lA[k], lB[k], sgA[k], sgB[k],
-                                   [&](int node, double val) {
+                                   wphi[k], wth[k], [&](int node, double val) {
                                      if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (Acc)val); else bad = 1;
                                    }, nullptr);
       } else {
@@ -620,7 +661,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
         // change buffer dacc in FP32 (k_conserve adds it to f)
         double* tgt = kF32 ? dacc + (size_t)cell * N : gd;
         bad |= vr_sample<false, R>(g, cp, cell, ntime, tt[k], ss[k], factor, lA[k], lB[k], sgA[k], sgB[k],
-                                   [&](int node, double val) {
+                                   wphi[k], wth[k], [&](int node, double val) {
                                      if ((unsigned)node < (unsigned)N) atomicAdd(&tgt[node], val); else bad = 1;
                                    }, nullptr);
       }
@@ -734,9 +775,10 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     const bool valid = true;
     int lA, lB;
     double sgA, sgB;
-    pick_batch<1>(cp, cv, N, cell, ntime, &t, &s, &valid, &lA, &lB, &sgA, &sgB);
+    uint32_t wphi, wth;
+    pick_batch<1>(cp, cv, N, cell, ntime, &t, &s, &valid, &lA, &lB, &sgA, &sgB, &wphi, &wth);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
-    vr_sample<true, double>(g, cp, cell, ntime, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB,
+    vr_sample<true, double>(g, cp, cell, ntime, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth,
                             [](int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
@@ -861,50 +903,60 @@ __device__ __forceinline__ double ghost(const ConvParams& p, const double* __res
 }
 
 // 2-D upwind sweep: interior cells (convection.f90:119-276) and boundary cells
-// (boundary_conditions.f90:1548-1643) in one pass, one lane per (cell, node).
-// Grid: x = node chunks of kConvBS nodes, padded to a multiple of 8, y = cells
-// (strided when there are more than gridDim.y). Workgroups are dispatched to
-// the 8 XCDs round-robin, so with a chunk count divisible by 8 every cell's
-// chunk j runs on XCD j % 8 and the neighbour re-reads of that chunk (x and y
-// upwind cells) hit that XCD's L2.
+// (boundary_conditions.f90:1548-1643) in one pass. Each lane owns one velocity
+// node of one column ix and walks the column in y, keeping the rows iy-1, iy,
+// iy+1 of its node in registers, so the y-upwind value is never re-read; the
+// x-upwind value comes from the neighbouring column's lane of the same node.
+// Grid: x = node chunks of kConvBS nodes, padded to a multiple of 8, y = column.
+// Workgroups are dispatched to the 8 XCDs round-robin, so with a chunk count
+// divisible by 8 every column's chunk j runs on XCD j % 8, next to its
+// neighbouring columns, and the x-upwind re-read hits that XCD's L2.
 __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, const double* __restrict__ f,
                                                        double* __restrict__ fo, const double* __restrict__ wallM,
                                                        const double* __restrict__ ratio,
                                                        const double* __restrict__ spec) {
   const int N = g.N;
   const int l = blockIdx.x * kConvBS + threadIdx.x;
-  if (l >= N) return;
+  const int ix = blockIdx.y;
+  if (l >= N || ix >= p.nx) return;
   int i, j, k;
   node_ijk(g, l, i, j, k);
   const double xv = g.ax[0][i], yv = g.ax[1][j];
   const double cfl_x = p.fx * xv, cfl_y = p.fy * yv;
+  const double ax_ = fabs(cfl_x), ay_ = fabs(cfl_y);
   // x region: 0 upwind from the right ghost, 1 no x term (zero plane), 2 from the left ghost
   int xr = -1, yr = -1;
   if (i <= g.neg[0]) xr = 0; else if (p.izflag && i == g.zero[0]) xr = 1; else if (i >= g.pos[0]) xr = 2;
   if (j <= g.neg[1]) yr = 0; else if (p.jzflag && j == g.zero[1]) yr = 1; else if (j >= g.pos[1]) yr = 2;
-  const int ncell = p.nx * p.ny;
-  for (int cell = blockIdx.y; cell < ncell; cell += gridDim.y) {
-    const int ix = cell % p.nx, iy = cell / p.nx;
-    const size_t idx = (size_t)cell * N + l;
-    const double f0 = f[idx];
-    const bool boundary = ix == 0 || iy == 0 || ix == p.nx - 1 || iy == p.ny - 1;
-    if (!boundary) {
-      const double px = (xv < 0.0) ? f[idx + N] : f[idx - N];
-      const double py = (yv < 0.0) ? f[idx + (size_t)p.nx * N] : f[idx - (size_t)p.nx * N];
-      fo[idx] = f0 + fabs(cfl_x) * (px - f0) + fabs(cfl_y) * (py - f0);
-      continue;
+  const size_t row = (size_t)p.nx * N;  // distance between rows iy and iy+1
+  const long long xoff = (xv < 0.0) ? (long long)N : -(long long)N;
+  const bool xin = (ix > 0 && ix < p.nx - 1);
+  size_t idx = (size_t)ix * N + l;
+  double fm = 0.0, fc = f[idx];
+  for (int iy = 0; iy < p.ny; ++iy, idx += row) {
+    const double fp = (iy + 1 < p.ny) ? f[idx + row] : 0.0;
+    if (xin && iy > 0 && iy < p.ny - 1) {
+      const double px = f[(long long)idx + xoff];
+      const double py = (yv < 0.0) ? fp : fm;
+      fo[idx] = fc + ax_ * (px - fc) + ay_ * (py - fc);
+    } else if (xr < 0 || yr < 0) {
+      fo[idx] = 0.0;
+    } else {
+      double out = fc;
+      if (xr != 1) {
+        const double gx = ghost(p, f, wallM, ratio, spec, N, xr == 0 ? 1 : 0, ix, iy, l, idx);
+        out = out + ax_ * (gx - fc);
+      }
+      if (yr != 1) {
+        const int side = yr == 0 ? 3 : 2;
+        const bool wall = (side == 2) ? iy == 0 : iy == p.ny - 1;
+        const double gy = wall ? ghost(p, f, wallM, ratio, spec, N, side, ix, iy, l, idx) : (side == 2 ? fm : fp);
+        out = out + ay_ * (gy - fc);
+      }
+      fo[idx] = out;
     }
-    if (xr < 0 || yr < 0) { fo[idx] = 0.0; continue; }
-    double out = f0;
-    if (xr != 1) {
-      const double gx = ghost(p, f, wallM, ratio, spec, N, xr == 0 ? 1 : 0, ix, iy, l, idx);
-      out = out + fabs(cfl_x) * (gx - f0);
-    }
-    if (yr != 1) {
-      const double gy = ghost(p, f, wallM, ratio, spec, N, yr == 0 ? 3 : 2, ix, iy, l, idx);
-      out = out + fabs(cfl_y) * (gy - f0);
-    }
-    fo[idx] = out;
+    fm = fc;
+    fc = fp;
   }
 }
 

@@ -26,7 +26,9 @@
  *
  * The reference draws uniforms from one sequential GSL stream; this oracle,
  * like the HIP library, draws them from counter-based Philox4x32-10 keyed by
- * the seed with counter (sample, integral | slot<<2, cell, ntime).
+ * the seed with counter (sample, integral | slot<<2, cell, ntime): block slot 0
+ * gives (u_A, u_B, u_phi, u_theta) of replenish location 1, block slot r gives
+ * (u_phi, u_theta) of location r >= 2.
  */
 #include "dvm_oracle.h"
 
@@ -219,17 +221,15 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* Two uniforms in [0,1) with 53 random bits each. */
-static void uniforms2(uint64_t seed, uint32_t sample, uint32_t tag, uint32_t cell, uint32_t ntime,
-                      double* ua, double* ub) {
+/* Four uniforms in [0,1) with 32 random bits each (x * 2^-32, the resolution
+   of the reference's gsl_rng_uniform on mt19937) from one Philox4x32-10 block. */
+static void uniforms4(uint64_t seed, uint32_t sample, uint32_t tag, uint32_t cell, uint32_t ntime,
+                      double u[4]) {
   uint32_t ctr[4] = {sample, tag, cell, ntime};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t o[4];
   orc_philox4x32_10(ctr, key, o);
-  uint64_t a = ((uint64_t)o[1] << 32) | o[0];
-  uint64_t b = ((uint64_t)o[3] << 32) | o[2];
-  *ua = (double)(a >> 11) * 0x1.0p-53;
-  *ub = (double)(b >> 11) * 0x1.0p-53;
+  for (int q = 0; q < 4; ++q) u[q] = (double)o[q] * 0x1.0p-32;
 }
 
 /* ----------------------------------------------------------------- remap */
@@ -602,8 +602,9 @@ static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_c
   int nx = g->n[0], nxy = g->n[0] * g->n[1];
   const double* f = c->f + (size_t)cell * N;
   const double* eq_df = c->eq_df + (size_t)cell * (N + 1);
-  double uA, uB;
-  uniforms2(cfg->seed, (uint32_t)s, (uint32_t)(t + 1), (uint32_t)cell, (uint32_t)ntime, &uA, &uB);
+  double u0[4];
+  uniforms4(cfg->seed, (uint32_t)s, (uint32_t)(t + 1), (uint32_t)cell, (uint32_t)ntime, u0);
+  const double uA = u0[0], uB = u0[1];
   /* pick_collision_partner (collision_utilities.f90:142-186) */
   int lA, lB;
   if (t == 0) lA = orc_referenced_search(c->eqc, N, c->eql, c->eqr, uA);
@@ -649,9 +650,13 @@ static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_c
   double alpha_ab = 0.5 * (cfg->alpha + cfg->alpha);
   if (irec) { drec[0] = kd1; drec[1] = kd2; }
   for (int r = 1; r <= cfg->num_repl; ++r) {
-    double uphi, uth;
-    uniforms2(cfg->seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell,
-              (uint32_t)ntime, &uphi, &uth);
+    double uphi = u0[2], uth = u0[3];
+    if (r > 1) {
+      double ur[4];
+      uniforms4(cfg->seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, ur);
+      uphi = ur[0];
+      uth = ur[1];
+    }
     /* find_post_collision_velocities (collision_utilities.f90:240-282), g_ratio = 1 */
     double gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
     double gx2 = fabs(x2 - vc0), gy2 = fabs(y2 - vc1), gz2 = fabs(z2 - vc2);
