@@ -76,11 +76,8 @@ __device__ __forceinline__ void node_ijk(const Grid& g, int l, int& i, int& j, i
 // monatomic species: n, u, v, w, <|v|>, T and the derived fields, in one pass
 // over f. The reference's second pass sum f |c - u|^2 is formed as
 // sum f |c|^2 - u . sum f c (algebraically equal; the distribution is read once).
-__global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
-                                                    double* __restrict__ props, double mass) {
-  __shared__ double red[6][kMomBS / 64];
-  const int cell = blockIdx.x;
-  const double* fc = f + (size_t)cell * g.N;
+__device__ __forceinline__ void moments_cell(const Grid& g, const double* __restrict__ fc, double* __restrict__ p,
+                                             double mass, double (*red)[kMomBS / 64]) {
   const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int l = threadIdx.x; l < g.N; l += kMomBS) {
@@ -109,7 +106,6 @@ __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __rest
       for (int w = 0; w < kMomBS / 64; ++w) t += red[q][w];
       tot[q] = t;
     }
-    double* p = props + (size_t)cell * DVM_NPROP;
     const double dens = tot[0];
     if (dens == 0.0) {
       for (int q = 0; q < DVM_NPROP; ++q) p[q] = 0.0;
@@ -127,6 +123,13 @@ __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __rest
     p[DVM_P_PRESSURE] = 0.5 * dens * t;
     p[DVM_P_AVGSPEED] = tot[5] / dens;
   }
+}
+
+__global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
+                                                    double* __restrict__ props, double mass) {
+  __shared__ double red[6][kMomBS / 64];
+  const int cell = blockIdx.x;
+  moments_cell(g, f + (size_t)cell * g.N, props + (size_t)cell * DVM_NPROP, mass, red);
 }
 
 // ------------------------------------------------------------------- CDF
@@ -192,27 +195,31 @@ __device__ __forceinline__ int tile_pos(int m) { return (m / kCdfV) * (kCdfV + 1
 // One workgroup per cell. Each chunk of kCdfBS*kCdfV nodes is loaded and
 // stored coalesced through an LDS tile in which lane t owns kCdfV consecutive
 // nodes for the scan.
-__global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
-                                                const double* __restrict__ props, double* __restrict__ eqv,
-                                                double* __restrict__ cdf, int2* __restrict__ tabs,
-                                                double* __restrict__ eqpar,
-                                                int* __restrict__ counts, double* __restrict__ factors) {
+struct CdfShared {
+  double red[kCdfBS / 64];
+  double lastp[kCdfBS];
+  double tile[kCdfBS * (kCdfV + 1)];
+  int rights[kNBucket];
+  int s_new;
+};
+
+// One cell: fc its distribution, p its moments, ev / ec / tab2 / par / cnt4 /
+// fac3 its slices of eqv, cdf ([2][N+1]), tabs ([2][1000]), eqpar, counts and
+// factors.
+__device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, const double* __restrict__ fc,
+                                         const double* __restrict__ p, double* __restrict__ ev,
+                                         double* __restrict__ ec, int2* __restrict__ tab2, double* __restrict__ par,
+                                         int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared& sh) {
   constexpr int V = kCdfV;
   constexpr int CH = kCdfBS * V;
-  __shared__ double red[kCdfBS / 64];
-  __shared__ double lastp[kCdfBS];
-  __shared__ double tile[kCdfBS * (V + 1)];
-  __shared__ int rights[kNBucket];
-  __shared__ int s_new;
-  const int cell = blockIdx.x;
+  double* red = sh.red;
+  double* lastp = sh.lastp;
+  double* tile = sh.tile;
+  int* rights = sh.rights;
+  int& s_new = sh.s_new;
   const int N = g.N;
   const int own = threadIdx.x * (V + 1);
-  const double* p = props + (size_t)cell * DVM_NPROP;
-  const double* fc = f + (size_t)cell * N;
-  double* ev = eqv + (size_t)cell * N;
-  double* ec = cdf + (size_t)cell * 2 * (N + 1);  // equilibrium CDF, then the deviation CDF
   double* nc = ec + (N + 1);
-  double* par = eqpar + (size_t)cell * 5;
   const double dens = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W];
   double temp = p[DVM_P_TEMP];
   const double mass = cp.mass;
@@ -286,7 +293,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
       }
       __syncthreads();
     }
-    build_buckets<kCdfBS>(ec, N, tabs + (size_t)cell * 2 * kNBucket, rights);
+    build_buckets<kCdfBS>(ec, N, tab2, rights);
     if (threadIdx.x == 0) {
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
     }
@@ -315,7 +322,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
     }
     __syncthreads();
   }
-  build_buckets<kCdfBS>(nc, N, tabs + (size_t)cell * 2 * kNBucket + kNBucket, rights);
+  build_buckets<kCdfBS>(nc, N, tab2 + kNBucket, rights);
   if (threadIdx.x == 0) {
     const double neq_dens = nc[N], eq_dens = ec[N];
     const double h = 0.5 * neq_dens / dens;
@@ -335,12 +342,25 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const doub
       if (!(num > 0.0)) num = 0.0;
       if (num > 2147483647.0) num = 2147483647.0;
       const int n = (int)num;
-      counts[cell * 4 + t] = n;
+      cnt4[t] = n;
       sum += n;
-      factors[cell * 3 + t] = (n > 0) ? factor_coeff * cp.sigma * wA[t] * wB[t] / (double)n : 0.0;
+      fac3[t] = (n > 0) ? factor_coeff * cp.sigma * wA[t] * wB[t] / (double)n : 0.0;
     }
-    counts[cell * 4 + 3] = sum;
+    cnt4[3] = sum;
   }
+}
+
+__global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
+                                                const double* __restrict__ props, double* __restrict__ eqv,
+                                                double* __restrict__ cdf, int2* __restrict__ tabs,
+                                                double* __restrict__ eqpar,
+                                                int* __restrict__ counts, double* __restrict__ factors) {
+  __shared__ CdfShared sh;
+  const int cell = blockIdx.x;
+  const int N = g.N;
+  cdf_cell(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
+           cdf + (size_t)cell * 2 * (N + 1), tabs + (size_t)cell * 2 * kNBucket, eqpar + (size_t)cell * 5,
+           counts + cell * 4, factors + cell * 3, sh);
 }
 
 // --------------------------------------------------------------- collide
@@ -603,6 +623,60 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
 //         correction. FP64: f + change is written to fd here. (LDS atomics
 //         accumulate in FP64 in both paths: ds_add_f32 stalls far longer on
 //         the repeated addresses of a cell's distribution peak.)
+// Samples q in [q0, q1) of one cell (q indexes integral 1, then 2, then 3);
+// every depletion and stencil point goes to scatter(node, value).
+template <bool kF32, int BS, int K, class Scatter>
+__device__ __forceinline__ int collide_samples(const Grid& g, const CollParams& cp, const CellView& cv,
+                                               int rng_cell, int ntime, int n0, int n1, int q0, int q1,
+                                               const double* __restrict__ fac3, Scatter&& scatter) {
+  using R = typename std::conditional<kF32, float, double>::type;
+  const int N = g.N;
+  int bad = 0;
+  for (int base = q0; base < q1; base += BS * K) {
+    int tt[K], ss[K], lA[K], lB[K];
+    bool valid[K];
+    double sgA[K], sgB[K];
+    uint32_t wphi[K], wth[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = base + k * BS + (int)threadIdx.x;
+      valid[k] = q < q1;
+      tt[k] = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
+      ss[k] = (tt[k] == 0) ? q : (tt[k] == 1 ? q - n0 : q - n0 - n1);
+    }
+    pick_batch<K>(cp, cv, N, rng_cell, ntime, tt, ss, valid, lA, lB, sgA, sgB, wphi, wth);
+#pragma unroll 1
+    for (int k = 0; k < K; ++k) {
+      if (!valid[k]) continue;
+      bad |= vr_sample<false, R>(g, cp, rng_cell, ntime, tt[k], ss[k], fac3[tt[k]], lA[k], lB[k], sgA[k], sgB[k],
+                                 wphi[k], wth[k], scatter, nullptr);
+    }
+  }
+  return bad;
+}
+
+__device__ __forceinline__ CellView cell_view(const double* f, const double* eqv, const double* cdf2,
+                                              const int2* tab2, int N) {
+  CellView cv;
+  cv.f = f;
+  cv.eqv = eqv;
+  cv.eqc = cdf2;
+  cv.neqc = cdf2 + (N + 1);
+  cv.eqtab = tab2;
+  cv.neqtab = tab2 + kNBucket;
+  cv.eq_total = cv.eqc[N];
+  cv.neq_total = cv.neqc[N];
+  return cv;
+}
+
+// Collision phase of one cell per workgroup of BS lanes.
+//   kLds: the cell's increments accumulate in LDS (dynamic shared memory after
+//         the axes); otherwise fd (a copy of f) takes global FP64 atomics.
+//   kF32: FP32 collision arithmetic; the FP64-accumulated change is written to
+//         dacc and k_conserve adds it to f with the FP64 conservation
+//         correction. FP64: f + change is written to fd here. (LDS atomics
+//         accumulate in FP64 in both paths: ds_add_f32 stalls far longer on
+//         the repeated addresses of a cell's distribution peak.)
 template <bool kLds, bool kF32, int BS, int K>
 __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime,
                                                 const double* __restrict__ f, double* __restrict__ fd,
@@ -611,61 +685,34 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                                 const int2* __restrict__ tabs, const int* __restrict__ counts,
                                                 const double* __restrict__ factors, int* __restrict__ err) {
   extern __shared__ double dyn[];
-  using Acc = double;
-  using R = typename std::conditional<kF32, float, double>::type;
   const int cell = blockIdx.x;
   const int N = g.N;
   stage_axes(g, dyn);
-  Acc* acc = reinterpret_cast<Acc*>(dyn + kAxisSlots);
+  double* acc = dyn + kAxisSlots;
   if (kLds) {
-    for (int l = threadIdx.x; l < N; l += BS) acc[l] = (Acc)0;
+    for (int l = threadIdx.x; l < N; l += BS) acc[l] = 0.0;
   }
   __syncthreads();
-  CellView cv;
-  cv.f = f + (size_t)cell * N;
-  cv.eqv = eqv + (size_t)cell * N;
-  cv.eqc = cdf + (size_t)cell * 2 * (N + 1);
-  cv.neqc = cv.eqc + (N + 1);
-  cv.eqtab = tabs + (size_t)cell * 2 * kNBucket;
-  cv.neqtab = cv.eqtab + kNBucket;
-  cv.eq_total = cv.eqc[N];
-  cv.neq_total = cv.neqc[N];
-  const int n0 = counts[cell * 4 + 0], n1 = counts[cell * 4 + 1], n2 = counts[cell * 4 + 2];
-  const int total = n0 + n1 + n2;
+  const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
+                                tabs + (size_t)cell * 2 * kNBucket, N);
+  const int* cnt = counts + cell * 4;
+  const int total = cnt[0] + cnt[1] + cnt[2];
   double* gd = fd + (size_t)cell * N;
-  int bad = 0;
-  for (int base = 0; base < total; base += BS * K) {
-    int tt[K], ss[K], lA[K], lB[K];
-    bool valid[K];
-    double sgA[K], sgB[K];
-    uint32_t wphi[K], wth[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = base + k * BS + (int)threadIdx.x;
-      valid[k] = q < total;
-      tt[k] = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
-      ss[k] = (tt[k] == 0) ? q : (tt[k] == 1 ? q - n0 : q - n0 - n1);
-    }
-    pick_batch<K>(cp, cv, N, cell, ntime, tt, ss, valid, lA, lB, sgA, sgB, wphi, wth);
-#pragma unroll 1
-    for (int k = 0; k < K; ++k) {
-      if (!valid[k]) continue;
-      const double factor = factors[cell * 3 + tt[k]];
-      if (kLds) {
-        bad |= vr_sample<false, R>(g, cp, cell, ntime, tt[k], ss[k], factor, lA[k], lB[k], sgA[k], sgB[k],
-                                   wphi[k], wth[k], [&](int node, double val) {
-                                     if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (Acc)val); else bad = 1;
-                                   }, nullptr);
-      } else {
-        // global FP64 atomics: into fd (a copy of f) in FP64, into the zeroed
-        // change buffer dacc in FP32 (k_conserve adds it to f)
-        double* tgt = kF32 ? dacc + (size_t)cell * N : gd;
-        bad |= vr_sample<false, R>(g, cp, cell, ntime, tt[k], ss[k], factor, lA[k], lB[k], sgA[k], sgB[k],
-                                   wphi[k], wth[k], [&](int node, double val) {
-                                     if ((unsigned)node < (unsigned)N) atomicAdd(&tgt[node], val); else bad = 1;
-                                   }, nullptr);
-      }
-    }
+  double* da = dacc + (size_t)cell * N;
+  int bad;
+  if (kLds) {
+    bad = collide_samples<kF32, BS, K>(g, cp, cv, cell, ntime, cnt[0], cnt[1], 0, total, factors + cell * 3,
+                                       [&](int node, double val) {
+                                         if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], val);
+                                       });
+  } else {
+    // global FP64 atomics: into fd (a copy of f) in FP64, into the zeroed
+    // change buffer dacc in FP32 (k_conserve adds it to f)
+    double* tgt = kF32 ? da : gd;
+    bad = collide_samples<kF32, BS, K>(g, cp, cv, cell, ntime, cnt[0], cnt[1], 0, total, factors + cell * 3,
+                                       [&](int node, double val) {
+                                         if ((unsigned)node < (unsigned)N) atomicAdd(&tgt[node], val);
+                                       });
   }
   if (bad) atomicOr(err, 1);
   if (!kLds) return;
@@ -673,7 +720,6 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   if constexpr (!kF32) {
     for (int l = threadIdx.x; l < N; l += BS) gd[l] = cv.f[l] + acc[l];
   } else {
-    double* da = dacc + (size_t)cell * N;
     for (int l = threadIdx.x; l < N; l += BS) da[l] = acc[l];
   }
 }
@@ -686,13 +732,18 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
 // correction removes what FP32 rounding leaves, so the cell's moments change by
 // FP64 round-off only. One workgroup per cell.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restrict__ f,
-                                                 const double* __restrict__ dacc, double* __restrict__ fd) {
-  __shared__ double red[BS / 64][20];
-  __shared__ double lam[5];
-  const int cell = blockIdx.x, N = g.N;
-  const double* fc = f + (size_t)cell * N;
-  const double* da = dacc + (size_t)cell * N;
+struct ConserveShared {
+  double red[BS / 64][20];
+  double lam[5];
+};
+
+template <int BS>
+__device__ __forceinline__ void conserve_cell(const Grid& g, const double* __restrict__ fc,
+                                              const double* __restrict__ da, double* __restrict__ out,
+                                              ConserveShared<BS>& sh) {
+  const int N = g.N;
+  auto& red = sh.red;
+  double* lam = sh.lam;
   double sums[20];
 #pragma unroll
   for (int z = 0; z < 20; ++z) sums[z] = 0.0;
@@ -735,7 +786,6 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
   }
   __syncthreads();
   const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
-  double* out = fd + (size_t)cell * N;
   for (int l = threadIdx.x; l < N; l += BS) {
     int i, j, k;
     node_ijk(g, l, i, j, k);
@@ -744,6 +794,14 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
     const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
     out[l] = fl + (da[l] - corr);
   }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restrict__ f,
+                                                 const double* __restrict__ dacc, double* __restrict__ fd) {
+  __shared__ ConserveShared<BS> sh;
+  const size_t off = (size_t)blockIdx.x * g.N;
+  conserve_cell<BS>(g, f + off, dacc + off, fd + off, sh);
 }
 
 // Parity trace of one cell: the same samples as the FP64 k_collide, recorded
@@ -755,15 +813,8 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
                                                    long long max_records, int* __restrict__ irec,
                                                    double* __restrict__ drec) {
   const int N = g.N;
-  CellView cv;
-  cv.f = f + (size_t)cell * N;
-  cv.eqv = eqv + (size_t)cell * N;
-  cv.eqc = cdf + (size_t)cell * 2 * (N + 1);
-  cv.neqc = cv.eqc + (N + 1);
-  cv.eqtab = tabs + (size_t)cell * 2 * kNBucket;
-  cv.neqtab = cv.eqtab + kNBucket;
-  cv.eq_total = cv.eqc[N];
-  cv.neq_total = cv.neqc[N];
+  const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
+                                tabs + (size_t)cell * 2 * kNBucket, N);
   const int n0 = counts[cell * 4 + 0], n1 = counts[cell * 4 + 1], n2 = counts[cell * 4 + 2];
   const int total = n0 + n1 + n2;
   for (int q = threadIdx.x + blockIdx.x * blockDim.x; q < total && q < max_records; q += blockDim.x * gridDim.x) {
