@@ -3,10 +3,12 @@ conservative remap, upwind convection, wall / specular / inflow boundaries and
 FP64 moments, as hand-written HIP kernels behind the C-ABI of include/dvm_hip.h.
 """
 from . import capi
-from .config import (SolverConfig, homogeneous_relaxation, lid_driven_cavity, maxwellian,
-                     normal_shock_1d, node_velocities, rankine_hugoniot, uniform_axis)
+from .batch import Batch
+from .config import (SolverConfig, batched_homogeneous, homogeneous_relaxation, lid_driven_cavity, maxwellian,
+                     mixture_instance, normal_shock_1d, node_velocities, rankine_hugoniot, uniform_axis)
 from .dvm import DvmError, Solver, load_library
 
-__all__ = ["capi", "SolverConfig", "Solver", "DvmError", "load_library", "homogeneous_relaxation",
+__all__ = ["capi", "SolverConfig", "Solver", "Batch", "DvmError", "load_library", "homogeneous_relaxation",
+           "batched_homogeneous", "mixture_instance",
            "lid_driven_cavity", "normal_shock_1d", "maxwellian", "node_velocities", "rankine_hugoniot",
            "uniform_axis"]

@@ -71,3 +71,33 @@ def declare(lib, prefix):
         fn.restype = res
         fn.argtypes = args
     return lib
+
+
+def declare_batch(lib):
+    """Attach argtypes/restype for the dvm_batch_* entry points."""
+    P, I32, I64 = C.c_void_p, C.c_int32, C.c_int64
+    pd, pi32, pi64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    pcfg = C.POINTER(DvmConfig)
+    sig = {
+        "create": (C.c_int, [pcfg, I32, I64, I32, C.POINTER(C.c_void_p)]),
+        "destroy": (None, [P]),
+        "last_error": (C.c_char_p, [P]),
+        "footprint": (C.c_int, [pcfg, I32, pi64, pi64]),
+        "admit": (C.c_int, [P, pcfg, pd, pi32]),
+        "retire": (C.c_int, [P, I32]),
+        "step": (C.c_int, [P, I32]),
+        "synchronize": (C.c_int, [P]),
+        "download_f": (C.c_int, [P, I32, pd]),
+        "moments": (C.c_int, [P, I32, pd]),
+        "l1": (C.c_int, [P, pd]),
+        "slot_info": (C.c_int, [P, I32, pi32, pi64, pi32, pi32]),
+        "capacity": (C.c_int, [P, pi64, pi64, pi32, pi32]),
+        "set_tiles": (C.c_int, [P, I32, I32, I32]),
+        "get_tiles": (C.c_int, [P, pi32]),
+        "tune": (C.c_int, [P, I32, pi32, pd]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, f"dvm_batch_{name}")
+        fn.restype = res
+        fn.argtypes = args
+    return lib

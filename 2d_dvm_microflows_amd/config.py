@@ -201,3 +201,42 @@ def normal_shock_1d(nx=200, n_v=31, vmax=6.0, mach=2.0, samples=50_000, kn=1.0, 
         t = 1.0 + (rt - 1.0) * s[c]
         f[c] = maxwellian(cfg, d, u, 0.0, 0.0, t)
     return cfg, f
+
+
+def mixture_instance(rng, edge, samples_per_node=50, dt=0.05, kn=1.0, seed=0, precision=capi.DVM_PREC_FP64,
+                     vpad=4.0):
+    """One space-homogeneous instance of the batched workload: a grid of
+    edge^3 nodes and a sum of 1-3 Maxwellians with T ~ U[0.5, 3],
+    u ~ U[-1.5, 1.5]^3 and positive weights (total density 1). The grid spans
+    +-(1.5 + vpad sqrt(T_max)) so every component's tails are resolved; hard
+    spheres, explicit sample count samples_per_node * edge^3."""
+    k = int(rng.integers(1, 4))
+    temps = rng.uniform(0.5, 3.0, size=k)
+    vels = rng.uniform(-1.5, 1.5, size=(k, 3))
+    w = rng.uniform(0.2, 1.0, size=k)
+    w = w / w.sum()
+    vmax = 1.5 + vpad * math.sqrt(float(temps.max()))
+    ax = uniform_axis(edge, -vmax, vmax)
+    cfg = SolverConfig(dim=capi.DVM_DIM_0D, nx=1, ny=1, vx=ax, vy=ax.copy(), vz=ax.copy(), dt=dt, kn=kn,
+                       omega=0.5, alpha=1.0, sample_mode=capi.DVM_SAMPLES_EXPLICIT,
+                       samples_per_cell=int(samples_per_node * edge ** 3), seed=seed, precision=precision,
+                       enable_convection=0, name=f"mixture {edge}^3 x{k}")
+    f = np.zeros(cfg.num_points)
+    for q in range(k):
+        f += maxwellian(cfg, w[q], *vels[q], temps[q])
+    return cfg, f.reshape(1, -1)
+
+
+def batched_homogeneous(count=4096, seed=11, edges=tuple(range(15, 42, 2)), samples_per_node=50, dt=0.05,
+                        precision=capi.DVM_PREC_FP64):
+    """The batched workload: `count` space-homogeneous instances, edge drawn
+    uniformly from `edges` (odd 15..41: mean nodes 27 412), each relaxing a
+    mixture of Maxwellians with 50 samples per node per step. Instance i has
+    seed seed * 1_000_003 + i."""
+    rng = np.random.default_rng(seed)
+    es = rng.choice(np.asarray(edges), size=count)
+    out = []
+    for i, e in enumerate(es):
+        out.append(mixture_instance(rng, int(e), samples_per_node=samples_per_node, dt=dt,
+                                    seed=seed * 1_000_003 + i, precision=precision))
+    return out

@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -38,7 +39,8 @@ struct dvm_ctx {
   CdfParams cdfp{};
   CollParams collp{};
   ConvParams convp{};
-  bool lds_collide = false;
+  bool lds_collide = false;      // the FP64 slice fits LDS (window = grid)
+  WinDims wd{};                  // LDS window extents
   size_t coll_lds = 0;
   int coll_bs = 1024;            // lanes per collision workgroup (256, 512 or 1024)
   int pick_k = 1;                // samples per lane picked in lockstep (1, 2 or 4)
@@ -183,52 +185,83 @@ static const void* collide_kernel(const dvm_ctx* c) {
   }
 }
 
-static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
-  c->cfg = *cfg;
-  c->ncell = cfg->nx * cfg->ny;
+// LDS window nodes available to a collision workgroup next to the staged axes
+// (64 bytes kept for the kernels' static LDS).
+constexpr int kWindowCap = (int)((kLdsBytes - 64) / sizeof(double)) - kAxisSlots;
+
+// Extents of the collision LDS window for a grid of n[0] x n[1] x n[2] nodes:
+// the grid itself when it fits (returns true), else the largest box, near
+// cubic, within kWindowCap nodes.
+static bool window_dims(const int n[3], int w[3]) {
+  const long long N = (long long)n[0] * n[1] * n[2];
+  if (N <= kWindowCap) {
+    for (int d = 0; d < 3; ++d) w[d] = n[d];
+    return true;
+  }
+  int e = 1;
+  while ((long long)(e + 1) * (e + 1) * (e + 1) <= kWindowCap) ++e;
+  for (int d = 0; d < 3; ++d) w[d] = n[d] < e ? n[d] : e;
+  for (bool grew = true; grew;) {
+    grew = false;
+    for (int d = 0; d < 3; ++d) {
+      const long long rest = (long long)w[(d + 1) % 3] * w[(d + 2) % 3];
+      if (w[d] < n[d] && (w[d] + 1) * rest <= kWindowCap) { ++w[d]; grew = true; }
+    }
+  }
+  return false;
+}
+
+// Host copy of one velocity grid (velocity_grid.f90:271-597): axes, the
+// trapezoid weights beta, the zero points, and the packed device image
+// [ax0 ax1 ax2 beta0 beta1 beta2]. Device pointers are set by the caller.
+struct HostGrid {
+  std::vector<double> ax[3], beta[3], packed;
+  Grid g{};
+  double b3min = 0.0;
+};
+
+static void build_host_grid(const dvm_config* cfg, HostGrid& h) {
   const int n[3] = {cfg->nvx, cfg->nvy, cfg->nvz};
   const double* src[3] = {cfg->vx, cfg->vy, cfg->vz};
-  c->N = n[0] * n[1] * n[2];
-  const int N = c->N;
-  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& row : c->ev)
-    for (auto& e : row) HIP_TRY(c, hipEventCreate(&e));
-  // ---- grid (velocity_grid.f90:271-597)
-  std::vector<double> packed;
+  h.packed.clear();
   for (int d = 0; d < 3; ++d) {
-    c->ax[d].assign(src[d], src[d] + n[d]);
-    auto& a = c->ax[d];
-    auto& b = c->beta[d];
+    h.ax[d].assign(src[d], src[d] + n[d]);
+    auto& a = h.ax[d];
+    auto& b = h.beta[d];
     b.resize(n[d]);
     b[0] = a[1] - a[0];
     b[n[d] - 1] = a[n[d] - 1] - a[n[d] - 2];
     for (int i = 1; i < n[d] - 1; ++i) b[i] = (a[i + 1] - a[i]) / 2.0 + (a[i] - a[i - 1]) / 2.0;
-    c->g.n[d] = n[d];
-    c->g.vmin[d] = a[0];
-    c->g.factor[d] = (double)(n[d] - 1) / (a[n[d] - 1] - a[0]);
-    zero_point(a, c->g.zero[d], c->g.pos[d], c->g.neg[d]);
+    h.g.n[d] = n[d];
+    h.g.vmin[d] = a[0];
+    h.g.factor[d] = (double)(n[d] - 1) / (a[n[d] - 1] - a[0]);
+    zero_point(a, h.g.zero[d], h.g.pos[d], h.g.neg[d]);
   }
-  for (int d = 0; d < 3; ++d) packed.insert(packed.end(), c->ax[d].begin(), c->ax[d].end());
-  for (int d = 0; d < 3; ++d) packed.insert(packed.end(), c->beta[d].begin(), c->beta[d].end());
-  if (int rc = dalloc(c, &c->d_axes, packed.size())) return rc;
-  HIP_TRY(c, hipMemcpy(c->d_axes, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
-  {
-    double* p = c->d_axes;
-    for (int d = 0; d < 3; ++d) { c->g.ax[d] = p; p += n[d]; }
-    for (int d = 0; d < 3; ++d) { c->g.beta[d] = p; p += n[d]; }
-  }
-  c->g.N = N;
-  c->g.nxy = n[0] * n[1];
-  c->g.inv_n0 = 1.0 / (double)n[0];
-  c->g.inv_nxy = 1.0 / (double)(n[0] * n[1]);
-  double b3min = c->beta[0][0] * c->beta[1][0] * c->beta[2][0];
+  for (int d = 0; d < 3; ++d) h.packed.insert(h.packed.end(), h.ax[d].begin(), h.ax[d].end());
+  for (int d = 0; d < 3; ++d) h.packed.insert(h.packed.end(), h.beta[d].begin(), h.beta[d].end());
+  h.g.N = n[0] * n[1] * n[2];
+  h.g.nxy = n[0] * n[1];
+  h.g.inv_n0 = 1.0 / (double)n[0];
+  h.g.inv_nxy = 1.0 / (double)(n[0] * n[1]);
+  double b3min = h.beta[0][0] * h.beta[1][0] * h.beta[2][0];
   for (int k = 0; k < n[2]; ++k)
     for (int j = 0; j < n[1]; ++j)
       for (int i = 0; i < n[0]; ++i) {
-        const double b3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
+        const double b3 = h.beta[0][i] * h.beta[1][j] * h.beta[2][k];
         if (b3 < b3min) b3min = b3;
       }
-  // ---- species constants (scaling.f90:163-201 calculate_cross_section)
+  h.b3min = b3min;
+}
+
+// Point the grid at its packed image at device address p.
+static void place_grid(Grid& g, double* p) {
+  for (int d = 0; d < 3; ++d) { g.ax[d] = p; p += g.n[d]; }
+  for (int d = 0; d < 3; ++d) { g.beta[d] = p; p += g.n[d]; }
+}
+
+// Species constants (scaling.f90:163-201 calculate_cross_section) and the
+// sample-count / collision parameters derived from them.
+static void species_params(const dvm_config* cfg, double b3min, CdfParams& cp, CollParams& co) {
   const double m = cfg->mass, d = cfg->diam;
   const double m_red = m * m / (m + m);
   const double omega_ab = 0.5 * (cfg->omega + cfg->omega);
@@ -239,24 +272,64 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   else if (cfg->csection_coeff == DVM_CS_VKI)
     sigma = (3.32335097045 * pow(m_red, 1.0 - omega_ab) * pow(cfg->t_viscosity, omega_ab)) /
             (gamma_ref(4.5 - omega_ab) * cfg->viscosity);
-  c->cdfp.mass = m;
-  c->cdfp.cdf_tolerance = cfg->cdf_tolerance;
-  c->cdfp.dt = cfg->dt;
-  c->cdfp.kn = cfg->kn;
-  c->cdfp.sigma = sigma;
-  c->cdfp.rms_sq = cfg->coln_rms * cfg->coln_rms;
+  cp.mass = m;
+  cp.cdf_tolerance = cfg->cdf_tolerance;
+  cp.dt = cfg->dt;
+  cp.kn = cfg->kn;
+  cp.sigma = sigma;
+  cp.rms_sq = cfg->coln_rms * cfg->coln_rms;
   const double b3a = b3min * pow(m, 3.0 / 2.0);
-  c->cdfp.beta3 = 0.5 * (b3a + b3a);
-  c->cdfp.sample_mode = cfg->sample_mode;
-  c->cdfp.samples_per_cell = cfg->samples_per_cell;
-  c->collp.seed = cfg->seed;
-  c->collp.mass = m;
-  c->collp.vhs_exponent = 2.0 - 2.0 * omega_ab;
-  c->collp.hard_sphere = (c->collp.vhs_exponent == 1.0);
+  cp.beta3 = 0.5 * (b3a + b3a);
+  cp.sample_mode = cfg->sample_mode;
+  cp.samples_per_cell = cfg->samples_per_cell;
+  co.seed = cfg->seed;
+  co.mass = m;
+  co.vhs_exponent = 2.0 - 2.0 * omega_ab;
+  co.hard_sphere = (co.vhs_exponent == 1.0);
   const double alpha_ab = 0.5 * (cfg->alpha + cfg->alpha);
-  c->collp.ralpha = 1.0 / alpha_ab;
-  c->collp.vss = fabs(c->collp.ralpha - 1.0) > kDoubleTol;
-  c->collp.num_repl = cfg->num_repl;
+  co.ralpha = 1.0 / alpha_ab;
+  co.vss = fabs(co.ralpha - 1.0) > kDoubleTol;
+  co.num_repl = cfg->num_repl;
+}
+
+// apply_fixed_in_out_flow (grid_ref_code boundary_conditions.f90:783-913):
+// the wall Maxwellian (n, u, T; v = w = 0) times beta^3 on both ends, [2][N].
+static void inflow_maxwellians(const dvm_config* cfg, const HostGrid& h, std::vector<double>& inf) {
+  const int n[3] = {h.g.n[0], h.g.n[1], h.g.n[2]};
+  const int N = h.g.N;
+  const double m = cfg->mass;
+  inf.assign((size_t)2 * N, 0.0);
+  for (int s = 0; s < 2; ++s) {
+    const double* wl = cfg->wall[s];
+    const double dens = wl[DVM_W_DENS], temp = wl[DVM_W_TEMP], u = wl[DVM_W_U];
+    const double mc = dens * sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
+    for (int k = 0, l = 0; k < n[2]; ++k)
+      for (int j = 0; j < n[1]; ++j)
+        for (int i = 0; i < n[0]; ++i, ++l) {
+          const double beta3 = h.beta[0][i] * h.beta[1][j] * h.beta[2][k];
+          inf[(size_t)s * N + l] = maxwell_host(h.ax[0][i], h.ax[1][j], h.ax[2][k], mc, m, u, 0.0, 0.0, temp) * beta3;
+        }
+  }
+}
+
+static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
+  c->cfg = *cfg;
+  c->ncell = cfg->nx * cfg->ny;
+  const int n[3] = {cfg->nvx, cfg->nvy, cfg->nvz};
+  c->N = n[0] * n[1] * n[2];
+  const int N = c->N;
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& row : c->ev)
+    for (auto& e : row) HIP_TRY(c, hipEventCreate(&e));
+  HostGrid hg;
+  build_host_grid(cfg, hg);
+  for (int d = 0; d < 3; ++d) { c->ax[d] = hg.ax[d]; c->beta[d] = hg.beta[d]; }
+  c->g = hg.g;
+  if (int rc = dalloc(c, &c->d_axes, hg.packed.size())) return rc;
+  HIP_TRY(c, hipMemcpy(c->d_axes, hg.packed.data(), hg.packed.size() * sizeof(double), hipMemcpyHostToDevice));
+  place_grid(c->g, c->d_axes);
+  species_params(cfg, hg.b3min, c->cdfp, c->collp);
+  const double m = cfg->mass;
   // ---- resident buffers
   const size_t CN = (size_t)c->ncell * N, CN1 = (size_t)c->ncell * (N + 1);
   int rc;
@@ -278,10 +351,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     const int v = atoi(e);
     if (v == 1 || v == 2 || v == 4) c->pick_k = v;
   }
-  const size_t fixed_lds = (size_t)kAxisSlots * sizeof(double);
-  c->coll_lds = fixed_lds + (size_t)N * sizeof(double);
-  c->lds_collide = c->coll_lds <= kLdsBytes;
-  if (!c->lds_collide) c->coll_lds = fixed_lds;
+  c->lds_collide = window_dims(n, c->wd.w);
+  c->coll_lds = (size_t)(kAxisSlots + c->wd.w[0] * c->wd.w[1] * c->wd.w[2]) * sizeof(double);
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
   }
@@ -343,20 +414,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     }
   }
   if (cfg->dim == DVM_DIM_1D && cfg->enable_convection) {
-    // apply_fixed_in_out_flow (grid_ref_code boundary_conditions.f90:783-913):
-    // the wall Maxwellian (n, u, T; v = w = 0) times beta^3
-    std::vector<double> inf((size_t)2 * N, 0.0);
-    for (int s = 0; s < 2; ++s) {
-      const double* wl = cfg->wall[s];
-      const double dens = wl[DVM_W_DENS], temp = wl[DVM_W_TEMP], u = wl[DVM_W_U];
-      const double mc = dens * sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
-      for (int k = 0, l = 0; k < n[2]; ++k)
-        for (int j = 0; j < n[1]; ++j)
-          for (int i = 0; i < n[0]; ++i, ++l) {
-            const double beta3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
-            inf[(size_t)s * N + l] = maxwell_host(c->ax[0][i], c->ax[1][j], c->ax[2][k], mc, m, u, 0.0, 0.0, temp) * beta3;
-          }
-    }
+    std::vector<double> inf;
+    inflow_maxwellians(cfg, hg, inf);
     if ((rc = dalloc(c, &c->inflow, (size_t)2 * N))) return rc;
     HIP_TRY(c, hipMemcpy(c->inflow, inf.data(), inf.size() * sizeof(double), hipMemcpyHostToDevice));
   }
@@ -466,8 +525,8 @@ static int sample_async(dvm_ctx* c, int ntime) {
       HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
                                 c->stream));
   }
-  void* args[] = {&c->g, &c->collp, &ntime, &c->f, &c->fd, &c->dacc, &c->eqv, &c->cdf, &c->tabs,
-                  &c->counts, &c->factors, &c->err};
+  void* args[] = {&c->g,   &c->collp, &ntime,  &c->wd,     &c->props,   &c->f,   &c->fd,
+                  &c->dacc, &c->eqv, &c->cdf, &c->tabs, &c->counts, &c->factors, &c->err};
   HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3(c->ncell), dim3(c->coll_bs), args, c->coll_lds, c->stream));
   if (f32)
     hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
@@ -659,3 +718,5 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
   return DVM_OK;
 }
 
+
+#include "dvm_batch.hip"

@@ -211,8 +211,10 @@ __device__ __forceinline__ void remap(const Grid& g, double u, double v, double 
 }
 
 // Node index of stencil point q (0 origin, 1..3 interior, 4..6 exterior).
-__device__ __forceinline__ int stencil_node(const Stencil& s, int q, int nx, int nxy) {
-  int i = s.oi, j = s.oj, k = s.ok;
+__device__ __forceinline__ void stencil_ijk(const Stencil& s, int q, int& i, int& j, int& k) {
+  i = s.oi;
+  j = s.oj;
+  k = s.ok;
   switch (q) {
     case 1: i += s.a; break;
     case 2: j += s.b; break;
@@ -222,6 +224,11 @@ __device__ __forceinline__ int stencil_node(const Stencil& s, int q, int nx, int
     case 6: k -= s.c; break;
     default: break;
   }
+}
+
+__device__ __forceinline__ int stencil_node(const Stencil& s, int q, int nx, int nxy) {
+  int i, j, k;
+  stencil_ijk(s, q, i, j, k);
   return i + nx * j + nxy * k;
 }
 

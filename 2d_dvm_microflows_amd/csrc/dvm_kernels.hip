@@ -503,8 +503,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
   const R kd1 = sumA * dep1, kd2 = sumB * dep2;
   if (kRecord) { rec->kd1 = (double)kd1; rec->kd2 = (double)kd2; }
   if (!kRecord) {
-    scatter(lA, -(double)kd1);
-    scatter(lB, -(double)kd2);
+    scatter(iA, jA, kA, lA, -(double)kd1);
+    scatter(iB, jB, kB, lB, -(double)kd2);
   }
   // kd / NRepl; NRepl = 1 in the reference's decks, where the quotient is kd exactly
   const R kr1 = (cp.num_repl == 1) ? kd1 : kd1 / (R)cp.num_repl;
@@ -569,7 +569,11 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
         const double kr = (double)(side == 0 ? kr1 : kr2);
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
-          if (q < 4 || (st.flag & (1 << (q - 4)))) scatter(stencil_node(st, q, nx, nxy), kr * st.m[q]);
+          if (q < 4 || (st.flag & (1 << (q - 4)))) {
+            int si, sj, sk;
+            stencil_ijk(st, q, si, sj, sk);
+            scatter(si, sj, sk, si + nx * sj + nxy * sk, kr * st.m[q]);
+          }
         }
       }
     }
@@ -669,16 +673,103 @@ __device__ __forceinline__ CellView cell_view(const double* f, const double* eqv
   return cv;
 }
 
+// LDS accumulation window of a cell: a box of w[0] x w[1] x w[2] nodes, the
+// whole grid when the cell's FP64 slice fits LDS, otherwise centred on the
+// node nearest to the cell's bulk velocity (where a relaxing distribution and
+// its post-collision velocities concentrate) and clamped to the grid.
+struct Window {
+  int o[3], w[3], wxy, wn;
+};
+
+__device__ __forceinline__ Window make_window(const Grid& g, const int* wd, const double* __restrict__ props) {
+  Window win;
+  const double c[3] = {props[DVM_P_U], props[DVM_P_V], props[DVM_P_W]};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int n = g.n[d];
+    win.w[d] = wd[d];
+    double x = (c[d] - g.vmin[d]) * g.factor[d] + 0.5;
+    x = fmin(fmax(x, 0.0), (double)(n - 1));  // also maps NaN to 0
+    int o = (int)x - wd[d] / 2;
+    o = o < 0 ? 0 : o;
+    o = o > n - wd[d] ? n - wd[d] : o;
+    win.o[d] = o;
+  }
+  win.wxy = win.w[0] * win.w[1];
+  win.wn = win.wxy * win.w[2];
+  return win;
+}
+
+// One work item of the collision phase: samples [q0, q1) of one cell
+// (q1 clamped to the cell's count) on BS lanes, accumulated in FP64 in the LDS
+// window acc[] (LDS atomics accumulate in FP64 in both precisions: ds_add_f32
+// stalls far longer on the repeated addresses of a distribution's peak).
+//   full: the window is the whole grid; otherwise scatters outside it go to
+//         global FP64 atomics on tgt (fd prefilled with f in FP64, dacc
+//         zeroed in FP32).
+//   sole: the item is the cell's only one and writes its window directly
+//         (FP64: fd = f + change; FP32: dacc = change, which the conservation
+//         kernel adds to f); otherwise it adds its window to tgt atomically.
+template <bool kF32, int BS, int K>
+__device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp, const CellView& cv, int rng_cell,
+                                            int ntime, int q0, int q1, const int* __restrict__ cnt,
+                                            const double* __restrict__ fac, const Window& win, bool full, bool sole,
+                                            double* acc, double* __restrict__ fd, double* __restrict__ da) {
+  const int N = g.N;
+  const int total = cnt[0] + cnt[1] + cnt[2];
+  q1 = q1 < total ? q1 : total;
+  const int wn = full ? N : win.wn;
+  for (int q = threadIdx.x; q < wn; q += BS) acc[q] = 0.0;
+  __syncthreads();
+  double* tgt = kF32 ? da : fd;
+  int bad;
+  if (full) {
+    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac,
+                                       [&](int, int, int, int node, double v) {
+                                         if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], v);
+                                       });
+  } else {
+    const int o0 = win.o[0], o1 = win.o[1], o2 = win.o[2], wx = win.w[0], wxy = win.wxy;
+    const unsigned w0 = win.w[0], w1 = win.w[1], w2 = win.w[2];
+    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac,
+                                       [&](int i, int j, int k, int node, double v) {
+                                         const unsigned a = i - o0, b = j - o1, c = k - o2;
+                                         if (a < w0 && b < w1 && c < w2) atomicAdd(&acc[a + wx * b + wxy * c], v);
+                                         else if ((unsigned)node < (unsigned)N) atomicAdd(&tgt[node], v);
+                                       });
+  }
+  __syncthreads();
+  const int nx = g.n[0], nxy = g.nxy;
+  for (int q = threadIdx.x; q < wn; q += BS) {
+    int l = q;
+    if (!full) {
+      const int c = q / win.wxy, r = q - c * win.wxy;
+      const int b = r / win.w[0], a = r - b * win.w[0];
+      l = (win.o[0] + a) + nx * (win.o[1] + b) + nxy * (win.o[2] + c);
+    }
+    const double v = acc[q];
+    if (sole) {
+      if (kF32) da[l] = v;
+      else fd[l] = cv.f[l] + v;
+    } else if (v != 0.0) {
+      atomicAdd(&tgt[l], v);
+    }
+  }
+  return bad;
+}
+
 // Collision phase of one cell per workgroup of BS lanes.
-//   kLds: the cell's increments accumulate in LDS (dynamic shared memory after
-//         the axes); otherwise fd (a copy of f) takes global FP64 atomics.
-//   kF32: FP32 collision arithmetic; the FP64-accumulated change is written to
-//         dacc and k_conserve adds it to f with the FP64 conservation
-//         correction. FP64: f + change is written to fd here. (LDS atomics
-//         accumulate in FP64 in both paths: ds_add_f32 stalls far longer on
-//         the repeated addresses of a cell's distribution peak.)
+//   kLds: the cell's FP64 slice fits LDS (window = grid, no prefill);
+//         otherwise the window of extents wd around the cell's bulk velocity,
+//         with fd prefilled with f (FP64) or dacc zeroed (FP32).
+//   kF32: FP32 collision arithmetic (dacc, then k_conserve), else FP64 (fd).
+struct WinDims {
+  int w[3];
+};
+
 template <bool kLds, bool kF32, int BS, int K>
-__global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime,
+__global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, WinDims wd,
+                                                const double* __restrict__ props,
                                                 const double* __restrict__ f, double* __restrict__ fd,
                                                 double* __restrict__ dacc,
                                                 const double* __restrict__ eqv, const double* __restrict__ cdf,
@@ -687,41 +778,15 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   extern __shared__ double dyn[];
   const int cell = blockIdx.x;
   const int N = g.N;
+  Window win{};
+  if (!kLds) win = make_window(g, wd.w, props + (size_t)cell * DVM_NPROP);
   stage_axes(g, dyn);
-  double* acc = dyn + kAxisSlots;
-  if (kLds) {
-    for (int l = threadIdx.x; l < N; l += BS) acc[l] = 0.0;
-  }
-  __syncthreads();
   const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
                                 tabs + (size_t)cell * 2 * kNBucket, N);
-  const int* cnt = counts + cell * 4;
-  const int total = cnt[0] + cnt[1] + cnt[2];
-  double* gd = fd + (size_t)cell * N;
-  double* da = dacc + (size_t)cell * N;
-  int bad;
-  if (kLds) {
-    bad = collide_samples<kF32, BS, K>(g, cp, cv, cell, ntime, cnt[0], cnt[1], 0, total, factors + cell * 3,
-                                       [&](int node, double val) {
-                                         if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], val);
-                                       });
-  } else {
-    // global FP64 atomics: into fd (a copy of f) in FP64, into the zeroed
-    // change buffer dacc in FP32 (k_conserve adds it to f)
-    double* tgt = kF32 ? da : gd;
-    bad = collide_samples<kF32, BS, K>(g, cp, cv, cell, ntime, cnt[0], cnt[1], 0, total, factors + cell * 3,
-                                       [&](int node, double val) {
-                                         if ((unsigned)node < (unsigned)N) atomicAdd(&tgt[node], val);
-                                       });
-  }
+  const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, 0, 0x7fffffff, counts + cell * 4,
+                                            factors + cell * 3, win, kLds, true, dyn + kAxisSlots,
+                                            fd + (size_t)cell * N, dacc + (size_t)cell * N);
   if (bad) atomicOr(err, 1);
-  if (!kLds) return;
-  __syncthreads();
-  if constexpr (!kF32) {
-    for (int l = threadIdx.x; l < N; l += BS) gd[l] = cv.f[l] + acc[l];
-  } else {
-    for (int l = threadIdx.x; l < N; l += BS) da[l] = acc[l];
-  }
 }
 
 // FP64 conservation correction of the collision change d of each cell computed
@@ -830,7 +895,7 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     pick_batch<1>(cp, cv, N, cell, ntime, &t, &s, &valid, &lA, &lB, &sgA, &sgB, &wphi, &wth);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
     vr_sample<true, double>(g, cp, cell, ntime, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth,
-                            [](int, double) {}, &rec);
+                            [](int, int, int, int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
     for (int z = 0; z < 32; ++z) ir[z] = 0;
@@ -1020,6 +1085,19 @@ __device__ __forceinline__ double upwind1d(const double* __restrict__ f, size_t 
   return p0 - cfl * (p0 - f[idx - N]);
 }
 
+// One node l of 1-D cell ix (of nx) from the post-collision f into fo.
+__device__ __forceinline__ void convect1d_point(int N, int nx, const int* bc, double cfl, double xv,
+                                                const double* __restrict__ f, double* __restrict__ fo,
+                                                const double* __restrict__ inflow, int ix, int l) {
+  const size_t idx = (size_t)ix * N + l;
+  if (ix > 0 && ix < nx - 1) { fo[idx] = upwind1d(f, idx, N, cfl, xv); return; }
+  const int side = (ix == 0) ? 0 : 1;
+  const int b = bc[side];
+  if (b == DVM_BC_INOUT) fo[idx] = inflow[(size_t)side * N + l];
+  else if (b == DVM_BC_ZEROGRAD) fo[idx] = upwind1d(f, (side == 0) ? idx + N : idx - N, N, cfl, xv);
+  else fo[idx] = f[idx];
+}
+
 __global__ void __launch_bounds__(kConvBS) k_convect1d(Grid g, ConvParams p, const double* __restrict__ f,
                                                        double* __restrict__ fo, const double* __restrict__ inflow) {
   const int N = g.N;
@@ -1029,15 +1107,7 @@ __global__ void __launch_bounds__(kConvBS) k_convect1d(Grid g, ConvParams p, con
   node_ijk(g, l, i, j, k);
   const double xv = g.ax[0][i];
   const double cfl = p.fx * xv;
-  for (int ix = blockIdx.y; ix < p.nx; ix += gridDim.y) {
-    const size_t idx = (size_t)ix * N + l;
-    if (ix > 0 && ix < p.nx - 1) { fo[idx] = upwind1d(f, idx, N, cfl, xv); continue; }
-    const int side = (ix == 0) ? 0 : 1;
-    const int bc = p.bc[side];
-    if (bc == DVM_BC_INOUT) fo[idx] = inflow[(size_t)side * N + l];
-    else if (bc == DVM_BC_ZEROGRAD) fo[idx] = upwind1d(f, (side == 0) ? idx + N : idx - N, N, cfl, xv);
-    else fo[idx] = f[idx];
-  }
+  for (int ix = blockIdx.y; ix < p.nx; ix += gridDim.y) convect1d_point(N, p.nx, p.bc, cfl, xv, f, fo, inflow, ix, l);
 }
 
 }  // namespace dvm

@@ -33,7 +33,7 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise RuntimeError(
             f"HIP library {p} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
-    lib = capi.declare(C.CDLL(p), "dvm")
+    lib = capi.declare_batch(capi.declare(C.CDLL(p), "dvm"))
     lib.dvm_timing_reset.restype = C.c_int
     lib.dvm_timing_reset.argtypes = [C.c_void_p]
     lib.dvm_timing_read.restype = C.c_int

@@ -194,6 +194,74 @@ int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_rec
 int dvm_timing_reset(dvm_ctx* ctx);
 int dvm_timing_read(dvm_ctx* ctx, double* phase_ms /* [DVM_NPHASE] */, int64_t* calls);
 
+/* ------------------------------------------------------------------------
+   Batched solves. Many independent space-homogeneous (dim 0) and 1-D solves,
+   each with its own velocity grid, cell count, dt, Kn, sample count, seed and
+   step counter, resident in shared device pools and advanced together by one
+   replayed HIP graph per step. Each instance behaves as the loop body of
+   solve() (solver/dvm_solver.f90:299-357) for its own deck; the batch itself
+   has no counterpart in the reference, which runs one solve per process.
+   The Philox samples of an instance depend only on its seed, its own cell
+   indices and its own step count, so an instance's trajectory is the one a
+   dvm_ctx with the same config produces, whatever else is resident. */
+typedef struct dvm_batch dvm_batch;
+
+/* Creates the pools: node_capacity velocity nodes (summed over the cells of
+   all resident instances), cell_capacity cells, max_slots instances. The
+   species / collision fields of `species` (mass, diam, omega, alpha,
+   csection_coeff, viscosity, t_viscosity, coll_solver = DVM_COLL_VR,
+   cdf_tolerance, num_repl, coln_rms, sample_mode, precision) are shared by
+   every instance admitted later. */
+int dvm_batch_create(const dvm_config* species, int32_t max_slots, int64_t node_capacity,
+                     int32_t cell_capacity, dvm_batch** out);
+void dvm_batch_destroy(dvm_batch* b);
+/* Message of the last failing call on b (or of the last failing
+   dvm_batch_create / dvm_batch_footprint when b is NULL). */
+const char* dvm_batch_last_error(const dvm_batch* b);
+
+/* Pool nodes and HBM bytes an instance would occupy (admission by footprint). */
+int dvm_batch_footprint(const dvm_config* inst, int32_t precision, int64_t* nodes, int64_t* bytes);
+
+/* Admits one instance (dim 0 or 1; its species fields must equal the batch's)
+   with initial f0[nx][nnode], computes its moments and returns its slot.
+   DVM_E_NOMEM with a message when no slot or pool range is free: the caller
+   waits for a retirement and tries again.
+   Replaces, per instance: initialize_solver + initialize_dist_func
+   (solver/dvm_solver.f90:92-229). */
+int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const double* f0, int32_t* slot);
+/* Releases a slot's pool ranges (waits for queued steps first). */
+int dvm_batch_retire(dvm_batch* b, int32_t slot);
+
+/* Enqueues nsteps steps of every resident instance (no host wait). */
+int dvm_batch_step(dvm_batch* b, int32_t nsteps);
+/* Waits for queued steps; reports a remap failure flagged by any of them. */
+int dvm_batch_synchronize(dvm_batch* b);
+
+/* Current f[nx][nnode] / moments [nx][DVM_NPROP] of one instance. */
+int dvm_batch_download_f(dvm_batch* b, int32_t slot, double* f);
+int dvm_batch_moments(dvm_batch* b, int32_t slot, double* props);
+/* l1[max_slots]: for each resident space-homogeneous instance
+   sum |f - M| / n, M the discrete Maxwellian (times beta^3) at the
+   instance's own n, u, T scaled to sum M = n; -1 for other slots. A
+   convergence measure of this library; the reference has none. */
+int dvm_batch_l1(dvm_batch* b, double* l1);
+/* Steps enqueued since admission, pool nodes, cells, and whether the
+   instance's cells accumulate in LDS (1) or with global atomics (0). */
+int dvm_batch_slot_info(dvm_batch* b, int32_t slot, int32_t* steps, int64_t* nodes, int32_t* ncell,
+                        int32_t* in_lds);
+int dvm_batch_capacity(dvm_batch* b, int64_t* free_nodes, int64_t* largest_free, int32_t* free_cells,
+                       int32_t* free_slots);
+
+/* Collision tiles: lanes per workgroup (256, 512, 1024), samples per lane
+   picked in lockstep (1, 2) and samples per global-atomic work item.
+   dvm_batch_get_tiles returns those three and the persistent grid size. */
+int dvm_batch_set_tiles(dvm_batch* b, int32_t lanes, int32_t pick_k, int32_t item_samples);
+int dvm_batch_get_tiles(dvm_batch* b, int32_t* tiles /* [4] */);
+/* Autotuner: times `steps` replays of the resident set for each candidate
+   tile, restoring the resident state before each candidate and at the end,
+   keeps the fastest (tiles_out[3]) and its ms per step. */
+int dvm_batch_tune(dvm_batch* b, int32_t steps, int32_t* tiles_out, double* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

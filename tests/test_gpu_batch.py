@@ -1,0 +1,152 @@
+"""Batched solves (dvm_batch_*): every resident instance follows the trajectory
+that a standalone context with the same config produces ("alone and in the
+mix"), through admissions, retirements and autotuning.
+
+Bars: FP64, per cell max|a - b| <= 1e-12 max|b| (the two runs differ only in
+the order of floating-point atomic additions). FP32 collision arithmetic: the
+same bar after one step (identical FP32 inputs); afterwards 1e-6, because a
+last-bit FP64 difference can move an FP32 rounding of the next step's inputs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def cellwise(a, b):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    err = np.max(np.abs(a - b), axis=1)
+    scale = np.maximum(np.max(np.abs(b), axis=1), 1e-300)
+    return float(np.max(err / scale))
+
+
+def instances(pkg, precision=0):
+    """Uneven mix sharing one species: two mixtures in LDS, a 29^3 grid that
+    takes global atomics over several work items, a 1-D shock."""
+    rng = np.random.default_rng(5)
+    out = [
+        pkg.mixture_instance(rng, 11, samples_per_node=4, seed=101, precision=precision),
+        pkg.mixture_instance(rng, 15, samples_per_node=3, seed=102, precision=precision),
+    ]
+    c, f = pkg.homogeneous_relaxation(n_v=29, samples=40000, seed=103)
+    out.append((c.with_(precision=precision), f))
+    c, f = pkg.normal_shock_1d(nx=8, n_v=9, vmax=6.0, samples=600, omega=0.5, seed=104)
+    out.append((c.with_(precision=precision), f))
+    return out
+
+
+def standalone(pkg, cfg, f, steps):
+    with pkg.Solver(cfg) as s:
+        s.initialize(f)
+        for n in range(1, steps + 1):
+            s.step(n)
+        return s.download_f()
+
+
+def make_batch(pkg, insts, max_slots=8, item=16384):
+    b = pkg.Batch(insts[0][0], max_slots=max_slots, node_capacity=200_000, cell_capacity=64)
+    b.set_tiles(1024, 1, item)
+    return b
+
+
+def test_instances_match_standalone_in_the_mix(pkg):
+    insts = instances(pkg)
+    print("[batch] 11^3, 15^3, 29^3 (global atomics, 16384-sample items), 1-D shock 8x9^3; 3 steps", flush=True)
+    with make_batch(pkg, insts) as b:
+        slots = [b.admit(c, f) for c, f in insts]
+        infos = [b.slot_info(s) for s in slots]
+        assert [i["in_lds"] for i in infos] == [True, True, False, True]
+        b.step(3)
+        b.synchronize()
+        for (c, f), s in zip(insts, slots):
+            assert b.slot_info(s)["steps"] == 3
+            got = b.download_f(s)
+            want = standalone(pkg, c, f, 3)
+            assert cellwise(got, want) <= RTOL, (c.name, cellwise(got, want))
+            with pkg.Solver(c) as ref:
+                ref.initialize(want)
+                pm = ref.calculate_physical_properties()
+            assert cellwise(b.moments(s), pm) <= RTOL
+
+
+def test_admit_and_retire_between_steps(pkg):
+    insts = instances(pkg)
+    print("[batch] admit / retire between steps", flush=True)
+    with make_batch(pkg, insts) as b:
+        a = b.admit(*insts[0])
+        g = b.admit(*insts[2])
+        sh = b.admit(*insts[3])
+        b.step(2)
+        b.retire(g)
+        late = b.admit(*insts[1])
+        b.step(2)
+        assert b.slot_info(a)["steps"] == 4 and b.slot_info(late)["steps"] == 2
+        assert cellwise(b.download_f(a), standalone(pkg, *insts[0], 4)) <= RTOL
+        assert cellwise(b.download_f(sh), standalone(pkg, *insts[3], 4)) <= RTOL
+        assert cellwise(b.download_f(late), standalone(pkg, *insts[1], 2)) <= RTOL
+        # the retired range is reusable
+        again = b.admit(*insts[2])
+        b.step(1)
+        assert cellwise(b.download_f(again), standalone(pkg, *insts[2], 1)) <= RTOL
+
+
+def test_tuner_keeps_the_state(pkg):
+    insts = instances(pkg)
+    print("[batch] autotune on the resident set", flush=True)
+    with make_batch(pkg, insts) as b:
+        slots = [b.admit(c, f) for c, f in insts]
+        b.step(1)
+        before = [b.download_f(s) for s in slots]
+        tiles, ms = b.tune(1)
+        assert tiles["lanes"] in (256, 512, 1024) and tiles["pick_k"] in (1, 2) and ms > 0.0
+        for s, x in zip(slots, before):
+            assert np.array_equal(b.download_f(s), x)
+        b.step(1)
+        for (c, f), s in zip(insts, slots):
+            assert cellwise(b.download_f(s), standalone(pkg, c, f, 2)) <= RTOL
+
+
+def test_admission_reports_a_full_pool(pkg):
+    c, f = pkg.homogeneous_relaxation(n_v=11, samples=1000)
+    print("[batch] admission beyond the node pool", flush=True)
+    with pkg.Batch(c, max_slots=2, node_capacity=2000, cell_capacity=4) as b:
+        s0 = b.admit(c, f)
+        assert b.try_admit(c, f) is None  # 2 x 1331 nodes > 2000
+        with pytest.raises(pkg.DvmError) as e:
+            b.admit(c, f)
+        assert e.value.code == -3  # DVM_E_NOMEM
+        b.retire(s0)
+        assert b.try_admit(c, f) is not None
+        assert b.capacity()["free_slots"] == 1
+
+
+def test_l1_distance_from_equilibrium(pkg):
+    rng = np.random.default_rng(9)
+    c, f = pkg.mixture_instance(rng, 13, samples_per_node=20, seed=7)
+    print("[batch] L1 to the discrete Maxwellian", flush=True)
+    with pkg.Batch(c, max_slots=3, node_capacity=10_000, cell_capacity=4) as b:
+        s = b.admit(c, f)
+        l1 = b.l1()
+        assert np.all(l1[[q for q in range(3) if q != s]] == -1.0)
+        p = b.moments(s)[0]
+        m = pkg.maxwellian(c, p[0], p[1], p[2], p[3], p[4], renormalise=True)
+        want = np.abs(f[0] - m).sum() / p[0]
+        assert abs(l1[s] - want) <= 1e-12 * want
+        b.step(20)
+        assert b.l1()[s] < l1[s]
+
+
+def test_fp32_batch_matches_fp32_standalone(pkg):
+    insts = instances(pkg, precision=1)
+    print("[batch] FP32 collision arithmetic, 3 steps", flush=True)
+    with make_batch(pkg, insts) as b:
+        slots = [b.admit(c, f) for c, f in insts]
+        b.step(1)
+        for (c, f), s in zip(insts, slots):
+            assert cellwise(b.download_f(s), standalone(pkg, c, f, 1)) <= RTOL, c.name
+        b.step(2)
+        for (c, f), s in zip(insts, slots):
+            assert cellwise(b.download_f(s), standalone(pkg, c, f, 3)) <= 1e-6, c.name
