@@ -341,7 +341,8 @@ struct dvm_batch {
   bool any_global = false;
   size_t coll_lds = 0;
   int persist = 0;
-  int bs = 1024, k = 1, item_samples = 1 << 20;
+  int bs = 1024, k = 1, item_samples = 0;  // 0: sized from the resident load
+  long long item_used = 0;
   hipGraphExec_t exec = nullptr;
   bool dirty = true;
   char msg[512] = {0};
@@ -620,6 +621,19 @@ static int batch_lists(dvm_batch* b) {
   b->max_n1d = 0;
   b->any_global = false;
   int lds_n = 0;
+  // Samples per work item: the tile when set; else about four items per CU
+  // over the resident samples (never below 2^15: an item that splits a cell
+  // flushes its window with one atomic per node), so that a few large solves
+  // still spread over the whole chip.
+  long long step = b->item_samples;
+  if (step <= 0) {
+    long long total = 0;
+    for (int s = 0; s < b->max_slots; ++s)
+      if (b->hs[s].active) total += (explicit_mode ? b->hs[s].samples : 0) * b->hs[s].ncell;
+    step = total / (4LL * (b->cus > 0 ? b->cus : 256));
+    if (step < (1LL << 15)) step = 1LL << 15;
+  }
+  b->item_used = step;
   for (int s = 0; s < b->max_slots; ++s) {
     const HostSlot& h = b->hs[s];
     if (!h.active) continue;
@@ -632,7 +646,6 @@ static int batch_lists(dvm_batch* b) {
     // sample ranges of item_samples per work item (the count is known up
     // front only in the explicit sample mode); the last range is open-ended
     const long long est = explicit_mode ? h.samples : (long long)h.N;
-    const long long step = b->item_samples;
     d.split = (explicit_mode && est > step) ? (int)((est + step - 1) / step) : 1;
     if (!d.full || d.split > 1) b->any_global = true;
     for (int c = 0; c < h.ncell; ++c) {
@@ -851,7 +864,8 @@ extern "C" int dvm_batch_set_tiles(dvm_batch* b, int32_t lanes, int32_t pick_k, 
   if (!b) return bfail(b, DVM_E_ARG, "null batch");
   if (lanes != 256 && lanes != 512 && lanes != 1024) return bfail(b, DVM_E_ARG, "lanes must be 256, 512 or 1024");
   if (pick_k != 1 && pick_k != 2) return bfail(b, DVM_E_ARG, "pick_k must be 1 or 2");
-  if (item_samples < 1024) return bfail(b, DVM_E_ARG, "item_samples must be >= 1024");
+  if (item_samples != 0 && item_samples < 1024)
+    return bfail(b, DVM_E_ARG, "item_samples must be 0 (sized from the load) or >= 1024");
   BHIP(b, hipStreamSynchronize(b->stream));
   b->bs = lanes;
   b->k = pick_k;
@@ -864,7 +878,7 @@ extern "C" int dvm_batch_get_tiles(dvm_batch* b, int32_t* tiles) {
   if (!b || !tiles) return bfail(b, DVM_E_ARG, "null argument");
   tiles[0] = b->bs;
   tiles[1] = b->k;
-  tiles[2] = b->item_samples;
+  tiles[2] = (int32_t)(b->item_used < 0x7fffffff ? b->item_used : 0x7fffffff);
   tiles[3] = b->persist;
   return DVM_OK;
 }
@@ -945,7 +959,7 @@ extern "C" int dvm_batch_tune(dvm_batch* b, int32_t steps, int32_t* tiles_out, d
       if (rc == DVM_OK) rc = trial(bs, k, old_item);
   if (rc == DVM_OK && b->species.sample_mode == DVM_SAMPLES_EXPLICIT) {
     const int bb = best_bs, bk = best_k;
-    for (int item : {1 << 18, 1 << 20, 1 << 30})
+    for (int item : {0, 1 << 18, 1 << 20, 1 << 30})
       if (rc == DVM_OK && item != old_item) rc = trial(bb, bk, item);
   }
   b->bs = (rc == DVM_OK) ? best_bs : old_bs;

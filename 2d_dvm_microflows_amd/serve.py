@@ -1,0 +1,246 @@
+"""Continuous-batching request loop over a resident `Batch`.
+
+Requests (independent space-homogeneous relaxations and 1-D shocks, each with
+its own velocity grid and cell count) arrive over time. The loop admits them
+into the batch's device pools while their footprint fits (first come first
+served, later requests backfill around one that does not fit yet), advances
+every resident solve together `check_every` steps at a time (one replayed HIP
+graph per step), checks each solve's stop rule, retires finished solves and
+records each request's latency (completion - arrival).
+
+Stop rules (a convergence test of this library; the reference runs a fixed
+number of steps, solver/dvm_solver.f90:299):
+  homogeneous  L1 = sum |f - M| / n <= tol (M the discrete Maxwellian at the
+               solve's own moments), or an L1 plateau: no 5 % improvement on
+               the best L1 for `plateau_steps` steps (the discrete collision
+               operator's fixed point sits slightly off the discrete
+               Maxwellian on coarse grids), or max_steps
+  shock        steady state: max |n - n_prev| / max n over one check interval
+               <= steady_tol, or max_steps
+An instance's trajectory depends only on its own config (seeded counter-based
+sampling), and its checks fall every check_every steps from its own
+admission, so a request ends in the same state and step whether it is served
+alone or in a mix.
+"""
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .batch import Batch
+from .config import SolverConfig, maxwellian, mixture_instance, rankine_hugoniot, uniform_axis
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float               # seconds after the loop starts
+    cfg: SolverConfig
+    f0: np.ndarray
+    kind: str = "homogeneous"    # or "shock"
+    tol: float = 1e-4            # L1 (homogeneous) or steady-state (shock) tolerance
+    max_steps: int = 400
+
+
+@dataclass
+class Outcome:
+    rid: int
+    kind: str
+    arrival: float
+    admitted: float = math.nan
+    done: float = math.nan
+    steps: int = 0
+    reason: str = ""             # tol | plateau | steady | max_steps | too_large
+    metric: float = math.nan     # final L1 or steady-state change
+    nodes: int = 0
+    f: np.ndarray = None         # final distribution when kept
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+@dataclass
+class ServeStats:
+    outcomes: list
+    wall_s: float
+    steps_launched: int
+    max_resident: int
+    samples: float               # collision samples drawn over the run
+    node_updates: float          # cell-node updates over the run
+    latency_p50: float = math.nan
+    latency_p99: float = math.nan
+    extra: dict = field(default_factory=dict)
+
+
+class Server:
+    """species       SolverConfig carrying the shared species / precision
+    node_capacity   pool nodes (the HBM budget: dvm_batch_footprint gives an
+                    instance's bytes)
+    batch           an existing Batch (or an object with its admit / step /
+                    l1 / moments / retire interface) instead of a new one
+    """
+
+    def __init__(self, species, node_capacity, cell_capacity, max_slots, check_every=10, plateau_steps=50,
+                 keep_f=False, clock=time.perf_counter, sleep=time.sleep, batch=None):
+        self.batch = batch if batch is not None else Batch(species, max_slots=max_slots,
+                                                           node_capacity=node_capacity,
+                                                           cell_capacity=cell_capacity)
+        self.node_capacity = int(node_capacity)
+        self.cell_capacity = int(cell_capacity)
+        self.check_every = int(check_every)
+        self.plateau_steps = int(plateau_steps)
+        self.keep_f = keep_f
+        self.clock = clock
+        self.sleep = sleep
+
+    def close(self):
+        self.batch.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _fits_at_all(self, req):
+        return req.cfg.nx * req.cfg.num_points <= self.node_capacity and req.cfg.nx <= self.cell_capacity
+
+    def run(self, requests):
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        out = {r.rid: Outcome(r.rid, r.kind, r.arrival, nodes=r.cfg.nx * r.cfg.num_points) for r in reqs}
+        waiting = deque()
+        resident = {}                # slot -> dict(req, steps, best, since, prev)
+        b = self.batch
+        i, launched, max_res = 0, 0, 0
+        samples = updates = 0.0
+        t0 = self.clock()
+        while i < len(reqs) or waiting or resident:
+            now = self.clock() - t0
+            while i < len(reqs) and reqs[i].arrival <= now:
+                waiting.append(reqs[i])
+                i += 1
+            for req in list(waiting):
+                if not self._fits_at_all(req):
+                    waiting.remove(req)
+                    o = out[req.rid]
+                    o.reason, o.done = "too_large", now
+                    continue
+                slot = b.try_admit(req.cfg, req.f0)
+                if slot is None:
+                    continue
+                waiting.remove(req)
+                out[req.rid].admitted = self.clock() - t0
+                resident[slot] = dict(req=req, steps=0, best=math.inf, since=0, prev=None)
+            max_res = max(max_res, len(resident))
+            if not resident:
+                if i < len(reqs):
+                    self.sleep(max(0.0, reqs[i].arrival - (self.clock() - t0)))
+                    continue
+                for req in waiting:       # cannot be admitted even into empty pools
+                    out[req.rid].reason, out[req.rid].done = "too_large", self.clock() - t0
+                break
+            n = self.check_every
+            b.step(n)
+            launched += n
+            for st in resident.values():
+                c = st["req"].cfg
+                samples += n * c.samples_per_cell * c.nx
+                updates += n * c.num_points * c.nx
+            l1 = b.l1()                   # waits for the steps
+            now = self.clock() - t0
+            finished = []
+            for slot, st in resident.items():
+                req = st["req"]
+                st["steps"] += n
+                reason, metric = "", math.nan
+                if req.kind == "homogeneous":
+                    metric = float(l1[slot])
+                    if metric < 0.95 * st["best"]:
+                        st["best"], st["since"] = metric, 0
+                    else:
+                        st["since"] += n
+                    if metric <= req.tol:
+                        reason = "tol"
+                    elif st["since"] >= self.plateau_steps:
+                        reason = "plateau"
+                else:
+                    dens = b.moments(slot)[:, capi.P_DENS]
+                    if st["prev"] is not None:
+                        metric = float(np.max(np.abs(dens - st["prev"])) / max(np.max(np.abs(dens)), 1e-300))
+                        if metric <= req.tol:
+                            reason = "steady"
+                    st["prev"] = dens
+                if not reason and st["steps"] >= req.max_steps:
+                    reason = "max_steps"
+                if reason:
+                    o = out[req.rid]
+                    o.done, o.steps, o.reason, o.metric = now, st["steps"], reason, metric
+                    if self.keep_f:
+                        o.f = b.download_f(slot)
+                    finished.append(slot)
+            for slot in finished:
+                b.retire(slot)
+                del resident[slot]
+        wall = self.clock() - t0
+        outs = [out[r.rid] for r in reqs]
+        lat = np.array([o.latency for o in outs if o.reason != "too_large"])
+        stats = ServeStats(outs, wall, launched, max_res, samples, updates)
+        if lat.size:
+            stats.latency_p50 = float(np.percentile(lat, 50))
+            stats.latency_p99 = float(np.percentile(lat, 99))
+        return stats
+
+
+# ------------------------------------------------------------------- workload
+def shock_instance(rng, nx, edge, mach, samples_per_cell, seed, precision, dx=0.1, cfl=0.5, kn=1.0):
+    """A 1-D normal shock request: Mach `mach` upstream (T = 1), the
+    Rankine-Hugoniot state downstream, a tanh profile between them, in/out
+    flow at both ends; hard spheres so that it shares the homogeneous
+    requests' species."""
+    gamma = 5.0 / 3.0
+    u1 = mach * math.sqrt(gamma / 2.0)
+    rd, ru, rt, _ = rankine_hugoniot(mach, gamma)
+    vmax = max(6.0, u1 + 4.0 * math.sqrt(max(1.0, rt)))
+    ax = uniform_axis(edge, -vmax, vmax)
+    wall = np.array([[1.0, u1, 0.0, 0.0, 1.0], [rd, u1 * ru, 0.0, 0.0, rt],
+                     [1.0, 0.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 0.0, 1.0]])
+    cfg = SolverConfig(dim=capi.DVM_DIM_1D, nx=nx, ny=1, dx=dx, vx=ax, vy=ax.copy(), vz=ax.copy(),
+                       dt=cfl * dx / vmax, kn=kn, omega=0.5, alpha=1.0, sample_mode=capi.DVM_SAMPLES_EXPLICIT,
+                       samples_per_cell=int(samples_per_cell), seed=seed, precision=precision, enable_convection=1,
+                       bc_left=capi.DVM_BC_INOUT, bc_right=capi.DVM_BC_INOUT, wall=wall,
+                       name=f"shock M{mach:.2f} {nx} cells {edge}^3")
+    xc = (np.arange(nx) + 0.5) - nx / 2.0
+    s = 0.5 * (1.0 + np.tanh(xc / 5.0))
+    f = np.empty((nx, cfg.num_points))
+    for c in range(nx):
+        f[c] = maxwellian(cfg, 1.0 + (rd - 1.0) * s[c], u1 + (u1 * ru - u1) * s[c], 0.0, 0.0,
+                          1.0 + (rt - 1.0) * s[c])
+    return cfg, f
+
+
+def synthetic_requests(count=120, rate=20.0, seed=5, homogeneous_share=0.7, precision=capi.DVM_PREC_FP32,
+                       edges=tuple(range(15, 42, 2)), samples_per_node=50, shock_cells=(50, 400),
+                       shock_edges=tuple(range(21, 32, 2)), shock_samples=20000, l1_tol=1e-4, steady_tol=2e-3,
+                       max_steps=300):
+    """The serving workload: Poisson arrivals at `rate` per second; 70 %
+    space-homogeneous mixtures (edge 15..41 odd, 50 samples per node), 30 %
+    1-D shocks (Mach U[1.2, 3], 50-400 cells, edge 21..31 odd)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < homogeneous_share:
+            cfg, f = mixture_instance(rng, int(rng.choice(edges)), samples_per_node=samples_per_node,
+                                      seed=seed * 7919 + rid, precision=precision)
+            out.append(Request(rid, t, cfg, f, "homogeneous", l1_tol, max_steps))
+        else:
+            nx = int(rng.integers(shock_cells[0], shock_cells[1] + 1))
+            cfg, f = shock_instance(rng, nx, int(rng.choice(shock_edges)), float(rng.uniform(1.2, 3.0)),
+                                    shock_samples, seed * 7919 + rid, precision)
+            out.append(Request(rid, t, cfg, f, "shock", steady_tol, max_steps))
+    return out

@@ -253,13 +253,18 @@ int dvm_batch_capacity(dvm_batch* b, int64_t* free_nodes, int64_t* largest_free,
                        int32_t* free_slots);
 
 /* Collision tiles: lanes per workgroup (256, 512, 1024), samples per lane
-   picked in lockstep (1, 2) and samples per global-atomic work item.
-   dvm_batch_get_tiles returns those three and the persistent grid size. */
+   picked in lockstep (1, 2) and samples per work item (a cell's samples are
+   split into items of that many, each accumulating in its own LDS window and
+   adding it to the cell atomically; 0, the default, sizes items from the
+   resident load: about four per CU, at least 2^15 samples).
+   dvm_batch_get_tiles returns lanes, pick_k, the item size in use and the
+   persistent grid size. */
 int dvm_batch_set_tiles(dvm_batch* b, int32_t lanes, int32_t pick_k, int32_t item_samples);
 int dvm_batch_get_tiles(dvm_batch* b, int32_t* tiles /* [4] */);
 /* Autotuner: times `steps` replays of the resident set for each candidate
    tile, restoring the resident state before each candidate and at the end,
-   keeps the fastest (tiles_out[3]) and its ms per step. */
+   keeps the fastest (tiles_out[3], item size 0 = sized from the load) and
+   its ms per step. */
 int dvm_batch_tune(dvm_batch* b, int32_t steps, int32_t* tiles_out, double* ms_out);
 
 #ifdef __cplusplus

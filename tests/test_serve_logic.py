@@ -1,0 +1,145 @@
+"""Host logic of the serving loop (admission by footprint, FIFO with backfill,
+stop rules, latency percentiles), on the CPU: a stand-in batch with scripted
+convergence replaces the device pools, and a simulated clock advances with
+each step."""
+import math
+
+import numpy as np
+import pytest
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(0.0, dt)
+
+
+class ScriptedBatch:
+    """Admission against a node budget; L1 decays as 0.3 exp(-steps/40) + floor
+    (1e-5 for even seeds, 2e-3 for odd),
+    shock density profiles change by c0 exp(-steps/30) per check."""
+
+    def __init__(self, capacity, clock, step_cost=1e-3):
+        self.capacity, self.clock, self.step_cost = capacity, clock, step_cost
+        self.res = {}
+        self.log = []
+
+    def try_admit(self, cfg, f0):
+        n = cfg.nx * cfg.num_points
+        if sum(r["n"] for r in self.res.values()) + n > self.capacity:
+            return None
+        slot = next(s for s in range(10_000) if s not in self.res)
+        self.res[slot] = dict(n=n, steps=0, cfg=cfg)
+        self.log.append(("admit", cfg.seed))
+        return slot
+
+    def step(self, n):
+        for r in self.res.values():
+            r["steps"] += n
+        self.clock.t += self.step_cost * n
+
+    def l1(self):
+        out = np.full(64, -1.0)
+        for s, r in self.res.items():
+            floor = 1e-5 if r["cfg"].seed % 2 == 0 else 2e-3
+            out[s] = 0.3 * math.exp(-r["steps"] / 40.0) + floor
+        return out
+
+    def moments(self, slot):
+        r = self.res[slot]
+        p = np.zeros((r["cfg"].nx, 8))
+        p[:, 0] = 1.0 + 0.5 * math.exp(-r["steps"] / 30.0) * np.linspace(0, 1, r["cfg"].nx)
+        return p
+
+    def download_f(self, slot):
+        return np.zeros(1)
+
+    def retire(self, slot):
+        self.log.append(("retire", self.res[slot]["cfg"].seed))
+        del self.res[slot]
+
+    def close(self):
+        pass
+
+
+def homogeneous(pkg, rid, arrival, edge, seed):
+    rng = np.random.default_rng(rid)
+    cfg, f = pkg.mixture_instance(rng, edge, samples_per_node=1, seed=seed)
+    return pkg.Request(rid, arrival, cfg, f, "homogeneous", 1e-4, 400)
+
+
+def shock(pkg, rid, arrival, nx, seed, tol=2e-3):
+    rng = np.random.default_rng(rid)
+    cfg, f = pkg.shock_instance(rng, nx, 9, 2.0, 100, seed, 0)
+    return pkg.Request(rid, arrival, cfg, f, "shock", tol, 300)
+
+
+def server(pkg, capacity, clock):
+    sb = ScriptedBatch(capacity, clock)
+    return pkg.Server(None, capacity, 1000, 64, check_every=10, plateau_steps=50, clock=clock, sleep=clock.sleep,
+                      batch=sb), sb
+
+
+def test_stop_rules(pkg):
+    clock = Clock()
+    srv, sb = server(pkg, 10**7, clock)
+    reqs = [homogeneous(pkg, 0, 0.0, 9, seed=2), homogeneous(pkg, 1, 0.0, 9, seed=3),
+            shock(pkg, 2, 0.0, 6, seed=4), shock(pkg, 3, 0.0, 6, seed=5, tol=0.0)]
+    st = srv.run(reqs)
+    o = {x.rid: x for x in st.outcomes}
+    # 0.3 exp(-s/40) + 1e-5 <= 1e-4 first at s = 40 ln(0.3/9e-5) = 324.4 -> the check at 330
+    assert o[0].reason == "tol" and o[0].steps == 330
+    # floor 2e-3 above the tolerance: the best stops improving by 5 % and the plateau rule ends it
+    assert o[1].reason == "plateau" and o[1].metric > 1e-4
+    assert o[2].reason == "steady" and o[2].metric <= 2e-3
+    assert o[3].reason == "max_steps" and o[3].steps == 300
+    assert not sb.res
+    assert st.latency_p50 <= st.latency_p99
+
+
+def test_admission_by_footprint_with_backfill(pkg):
+    clock = Clock()
+    big = homogeneous(pkg, 0, 0.0, 15, seed=2)            # 3375 nodes
+    too_big = homogeneous(pkg, 1, 0.001, 21, seed=4)      # 9261 nodes > capacity
+    second_big = homogeneous(pkg, 2, 0.002, 15, seed=6)
+    small = homogeneous(pkg, 3, 0.003, 7, seed=8)          # 343 nodes
+    srv, sb = server(pkg, 4000, clock)
+    st = srv.run([big, too_big, second_big, small])
+    o = {x.rid: x for x in st.outcomes}
+    assert o[1].reason == "too_large"
+    # the small request backfills next to the first big one; the second big waits for the first to retire
+    assert o[3].admitted < o[0].done <= o[2].admitted
+    assert sb.log.index(("admit", 6)) > sb.log.index(("retire", 2))
+    assert st.max_resident == 2
+    assert all(x.reason in ("tol", "plateau", "too_large") for x in st.outcomes)
+
+
+def test_latency_counts_from_arrival(pkg):
+    clock = Clock()
+    srv, _ = server(pkg, 10**7, clock)
+    reqs = [homogeneous(pkg, q, 0.5 * q, 9, seed=2) for q in range(4)]
+    st = srv.run(reqs)
+    lat = sorted(x.latency for x in st.outcomes)
+    # each runs alone after an idle gap: 330 steps x 1 ms
+    assert lat == pytest.approx([0.33] * 4)
+    assert st.latency_p50 == pytest.approx(0.33) and st.latency_p99 == pytest.approx(0.33)
+
+
+def test_synthetic_requests_mix(pkg):
+    reqs = pkg.synthetic_requests(count=400, rate=50.0, seed=3)
+    kinds = [r.kind for r in reqs]
+    share = kinds.count("homogeneous") / len(reqs)
+    assert 0.63 < share < 0.77
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0) and abs(len(reqs) / arr[-1] - 50.0) < 10.0
+    for r in reqs:
+        if r.kind == "shock":
+            assert 50 <= r.cfg.nx <= 400 and r.cfg.nvx in range(21, 32, 2)
+            assert r.f0.shape == (r.cfg.nx, r.cfg.num_points)
+        else:
+            assert r.cfg.nvx in range(15, 42, 2) and abs(r.f0.sum() - 1.0) < 1e-12
