@@ -18,12 +18,13 @@ struct dvm_ctx {
   dvm_config cfg;
   int ncell = 0, N = 0;
   hipStream_t stream = nullptr;
-  // phase events: per slot, boundaries of (cdf, collide, walls, sweep, moments)
+  // phase events: per slot, boundaries of the DVM_NPHASE phases
+  // (cdf, prep, collide, conserve, walls, sweep, moments)
   static constexpr int kSlots = 16;
-  hipEvent_t ev[kSlots][6] = {};
+  hipEvent_t ev[kSlots][DVM_NPHASE + 1] = {};
   bool slot_used[kSlots] = {};
   int slot = 0;
-  double phase_ms[5] = {0, 0, 0, 0, 0};
+  double phase_ms[DVM_NPHASE] = {};
   long long timed_steps = 0;
   // host copy of the grid
   std::vector<double> ax[3], beta[3];
@@ -476,8 +477,8 @@ extern "C" int dvm_download_f(dvm_ctx* c, int32_t species, double* f) {
 // ---- phase timing: a ring of event slots, one slot per entry-point call
 static int timing_drain_slot(dvm_ctx* c, int k) {
   if (!c->slot_used[k]) return DVM_OK;
-  HIP_TRY(c, hipEventSynchronize(c->ev[k][5]));
-  for (int q = 0; q < 5; ++q) {
+  HIP_TRY(c, hipEventSynchronize(c->ev[k][DVM_NPHASE]));
+  for (int q = 0; q < DVM_NPHASE; ++q) {
     float ms = 0.0f;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[k][q], c->ev[k][q + 1]));
     c->phase_ms[q] += ms;
@@ -515,23 +516,36 @@ static int cdf_async(dvm_ctx* c) {
   return DVM_OK;
 }
 
-static int sample_async(dvm_ctx* c, int ntime) {
+// fd = f (FP64) or dacc = 0 (FP32) before a collision that spills from its
+// LDS window to global atomics.
+static int prep_async(dvm_ctx* c) {
+  if (c->cfg.coll_solver != DVM_COLL_VR || c->lds_collide) return DVM_OK;
+  if (c->cfg.precision == DVM_PREC_FP32)
+    hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->dacc, (long long)c->ncell * c->N, 0.0);
+  else
+    HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+  HIP_TRY(c, hipGetLastError());
+  return DVM_OK;
+}
+
+static int collide_async(dvm_ctx* c, int ntime) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
-  const bool f32 = c->cfg.precision == DVM_PREC_FP32;
-  if (!c->lds_collide) {
-    if (f32)
-      hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->dacc, (long long)c->ncell * c->N, 0.0);
-    else
-      HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
-                                c->stream));
-  }
   void* args[] = {&c->g,   &c->collp, &ntime,  &c->wd,     &c->props,   &c->f,   &c->fd,
                   &c->dacc, &c->eqv, &c->cdf, &c->tabs, &c->counts, &c->factors, &c->err};
   HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3(c->ncell), dim3(c->coll_bs), args, c->coll_lds, c->stream));
-  if (f32)
-    hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
+  if (c->cfg.precision != DVM_PREC_FP32) {
+    double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
+  }
+  return DVM_OK;
+}
+
+// FP32 path: fd = f + dacc with the FP64 conservation correction, then phi := delta_phi.
+static int conserve_async(dvm_ctx* c) {
+  if (c->cfg.coll_solver != DVM_COLL_VR || c->cfg.precision != DVM_PREC_FP32) return DVM_OK;
+  hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
   HIP_TRY(c, hipGetLastError());
-  double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
+  double* t = c->f; c->f = c->fd; c->fd = t;
   return DVM_OK;
 }
 
@@ -569,23 +583,25 @@ static int sweep_async(dvm_ctx* c) {
   return DVM_OK;
 }
 
-// Runs the phases [first, last] of the step (0 cdf, 1 collide, 2 walls,
-// 3 sweep, 4 moments) with timing marks around each.
+// Runs the phases [first, last] of the step (0 cdf, 1 prep, 2 collide,
+// 3 conserve, 4 walls, 5 sweep, 6 moments) with timing marks around each.
 static int run_phases(dvm_ctx* c, int ntime, int first, int last) {
   if (int rc = begin_slot(c)) return rc;
   int rc = DVM_OK;
-  for (int q = 0; q < 5 && rc == DVM_OK; ++q) {
+  for (int q = 0; q < DVM_NPHASE && rc == DVM_OK; ++q) {
     if ((rc = mark(c, q))) break;
     if (q < first || q > last) continue;
     switch (q) {
       case 0: rc = cdf_async(c); break;
-      case 1: rc = sample_async(c, ntime); break;
-      case 2: rc = walls_async(c); break;
-      case 3: rc = sweep_async(c); break;
-      case 4: rc = moments_async(c); break;
+      case 1: rc = prep_async(c); break;
+      case 2: rc = collide_async(c, ntime); break;
+      case 3: rc = conserve_async(c); break;
+      case 4: rc = walls_async(c); break;
+      case 5: rc = sweep_async(c); break;
+      case 6: rc = moments_async(c); break;
     }
   }
-  if (rc == DVM_OK) rc = mark(c, 5);
+  if (rc == DVM_OK) rc = mark(c, DVM_NPHASE);
   if (rc == DVM_OK) end_slot(c);
   return rc;
 }
@@ -593,7 +609,7 @@ static int run_phases(dvm_ctx* c, int ntime, int first, int last) {
 extern "C" int dvm_moments(dvm_ctx* c, double* props_out) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid) return fail(c, DVM_E_STATE, "moments before upload");
-  if (int rc = run_phases(c, 0, 4, 4)) return rc;
+  if (int rc = run_phases(c, 0, 6, 6)) return rc;
   if (props_out) {
     HIP_TRY(c, hipMemcpyAsync(props_out, c->props, (size_t)c->ncell * DVM_NPROP * sizeof(double),
                               hipMemcpyDeviceToHost, c->stream));
@@ -606,7 +622,7 @@ extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "collide needs f and its moments");
   if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
-  if (int rc = run_phases(c, ntime, 0, 1)) return rc;
+  if (int rc = run_phases(c, ntime, 0, 3)) return rc;
   if (samples_out) {
     std::vector<int> h((size_t)c->ncell * 4, 0);
     if (c->cfg.coll_solver == DVM_COLL_VR)
@@ -621,7 +637,7 @@ extern "C" int dvm_convect(dvm_ctx* c, int32_t species, int32_t ntime) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (species != 0) return fail(c, DVM_E_ARG, "only species 0 is supported");
   if (!c->f_valid) return fail(c, DVM_E_STATE, "convect before upload");
-  if (int rc = run_phases(c, ntime, 2, 3)) return rc;
+  if (int rc = run_phases(c, ntime, 4, 5)) return rc;
   return check_err(c);
 }
 
@@ -629,7 +645,7 @@ extern "C" int dvm_step(dvm_ctx* c, int32_t ntime) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "step needs f and its moments");
   if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
-  return run_phases(c, ntime, 0, 4);
+  return run_phases(c, ntime, 0, 6);
 }
 
 extern "C" int dvm_timing_reset(dvm_ctx* c) {
@@ -646,7 +662,7 @@ extern "C" int dvm_timing_read(dvm_ctx* c, double* phase_ms, int64_t* calls) {
   for (int k = 0; k < dvm_ctx::kSlots; ++k)
     if (int rc = timing_drain_slot(c, k)) return rc;
   if (phase_ms)
-    for (int q = 0; q < 5; ++q) phase_ms[q] = c->phase_ms[q];
+    for (int q = 0; q < DVM_NPHASE; ++q) phase_ms[q] = c->phase_ms[q];
   if (calls) *calls = c->timed_steps;
   return check_err(c);
 }

@@ -172,7 +172,7 @@ class Solver:
         k = min(n.value, max_records)
         return irec[:k], drec[:k], n.value
 
-    PHASES = ("cdf", "collide", "walls", "sweep", "moments")
+    PHASES = ("cdf", "prep", "collide", "conserve", "walls", "sweep", "moments")
 
     def timing_reset(self):
         """Wait for queued work and zero the per-phase device-time totals."""
@@ -181,7 +181,7 @@ class Solver:
     def timing_read(self):
         """Per-phase device ms since the last reset (HIP events on the
         context's stream) and the number of calls they cover."""
-        ms = (C.c_double * 5)()
+        ms = (C.c_double * len(self.PHASES))()
         n = C.c_int64(0)
         self._check(self.lib.dvm_timing_read(self.h, ms, C.byref(n)))
         return dict(zip(self.PHASES, list(ms))), n.value

@@ -57,6 +57,7 @@ class Outcome:
     metric: float = math.nan     # final L1 or steady-state change
     nodes: int = 0
     f: np.ndarray = None         # final distribution when kept
+    props: np.ndarray = None     # final moments [nx, DVM_NPROP] when kept
 
     @property
     def latency(self):
@@ -85,7 +86,7 @@ class Server:
     """
 
     def __init__(self, species, node_capacity, cell_capacity, max_slots, check_every=10, plateau_steps=50,
-                 keep_f=False, clock=time.perf_counter, sleep=time.sleep, batch=None):
+                 keep_f=False, keep_moments=False, clock=time.perf_counter, sleep=time.sleep, batch=None):
         self.batch = batch if batch is not None else Batch(species, max_slots=max_slots,
                                                            node_capacity=node_capacity,
                                                            cell_capacity=cell_capacity)
@@ -94,6 +95,7 @@ class Server:
         self.check_every = int(check_every)
         self.plateau_steps = int(plateau_steps)
         self.keep_f = keep_f
+        self.keep_moments = keep_moments
         self.clock = clock
         self.sleep = sleep
 
@@ -181,6 +183,8 @@ class Server:
                     o.done, o.steps, o.reason, o.metric = now, st["steps"], reason, metric
                     if self.keep_f:
                         o.f = b.download_f(slot)
+                    if self.keep_moments:
+                        o.props = b.moments(slot)
                     finished.append(slot)
             for slot in finished:
                 b.retire(slot)
@@ -244,3 +248,55 @@ def synthetic_requests(count=120, rate=20.0, seed=5, homogeneous_share=0.7, prec
                                     shock_samples, seed * 7919 + rid, precision)
             out.append(Request(rid, t, cfg, f, "shock", steady_tol, max_steps))
     return out
+
+
+def moment_errors(p, ref):
+    """Max over cells of the relative error of density, of velocity (in units
+    of the thermal speed sqrt(T)) and of temperature, p against ref."""
+    th = np.sqrt(np.abs(ref[:, capi.P_TRTEMP]))
+    return {"density": float(np.max(np.abs(p[:, 0] - ref[:, 0]) / np.abs(ref[:, 0]))),
+            "velocity": float(np.max(np.max(np.abs(p[:, 1:4] - ref[:, 1:4]), axis=1) / th)),
+            "temperature": float(np.max(np.abs(p[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])))}
+
+
+def replay_fp64(requests, outcomes, node_capacity, cell_capacity, max_slots, check_every=10, seed_offset=0):
+    """Runs every served request again with plain FP64 collision arithmetic for
+    exactly the steps it was served (one batch, admitted in arrival order as
+    pool space allows) and returns {rid: errors}: moment_errors of the served
+    moments against the FP64 ones and, when the served f was kept, "f_l1" =
+    sum |f - f64| / sum |f64|. seed_offset != 0 replays with another seed: the
+    spread between two FP64 realisations is the Monte Carlo noise that the
+    FP32 error is to be read against."""
+    done = {o.rid: o for o in outcomes if o.props is not None and o.steps > 0}
+    todo = deque(r for r in sorted(requests, key=lambda r: r.arrival) if r.rid in done)
+    if not todo:
+        return {}
+    species = todo[0].cfg.with_(precision=capi.DVM_PREC_FP64)
+    errs = {}
+    with Batch(species, max_slots=max_slots, node_capacity=node_capacity, cell_capacity=cell_capacity) as b:
+        resident = {}
+        while todo or resident:
+            while todo:
+                r = todo[0]
+                cfg = r.cfg.with_(precision=capi.DVM_PREC_FP64, seed=r.cfg.seed + seed_offset)
+                slot = b.try_admit(cfg, r.f0)
+                if slot is None:
+                    break
+                todo.popleft()
+                resident[slot] = [r, 0]
+            if not resident:
+                raise RuntimeError("a request does not fit the FP64 replay pools")
+            b.step(check_every)
+            for slot in list(resident):
+                r, n = resident[slot]
+                resident[slot][1] = n = n + check_every
+                if n >= done[r.rid].steps:
+                    o = done[r.rid]
+                    e = moment_errors(o.props, b.moments(slot))
+                    if o.f is not None:
+                        f64 = b.download_f(slot)
+                        e["f_l1"] = float(np.abs(o.f - f64).sum() / np.abs(f64).sum())
+                    errs[r.rid] = e
+                    b.retire(slot)
+                    del resident[slot]
+    return errs

@@ -1,15 +1,24 @@
 """Throughput of the DVM time step on one MI355X.
 
-Workload (the 2-D lid-driven cavity microflow): 64x64 cells, 25^3 velocity
-nodes on +-5, hard spheres, Kn = 0.1, four diffuse walls at T_w = 1, lid at
-0.3 along x, uniform equilibrium start, 20 000 collision samples per cell per
-step. One step = CDF build + VR collision sampling + wall ghosts + upwind sweep
-+ moments, with the distributions resident in HBM before timing starts.
+Headline workload (the 2-D lid-driven cavity microflow): 64x64 cells, 25^3
+velocity nodes on +-5, hard spheres, Kn = 0.1, four diffuse walls at T_w = 1,
+lid at 0.3 along x, uniform equilibrium start, 20 000 collision samples per
+cell per step. One step = CDF build + VR collision sampling + wall ghosts +
+upwind sweep + moments, with the distributions resident in HBM before timing
+starts. It runs with FP32 collision arithmetic (the value) and in plain FP64.
+
+Also measured (unless --no-batched / --no-serving):
+  batched   4096 space-homogeneous instances, grid edge 15..41 odd, mixtures of
+            1-3 Maxwellians, 50 samples per node, stepped by one replayed HIP
+            graph per step with autotuned collision tiles
+  serving   Poisson arrivals of solve requests (70 % homogeneous, 30 % 1-D
+            shocks) served by continuous batching; p50 / p99 latency
 
 Prints one JSON line: cell-node updates/s (cells x velocity nodes x steps /
-wall time of the timed steps), the per-phase device times from HIP events on
-the library's stream, the sweep's HBM roofline and the CPU oracle's rate on a
-bounded sample of the same per-cell work.
+wall time of the timed steps), per-phase device times from HIP events on the
+library's stream, the roofline of the dominant kernel (the collision kernel)
+and of the sweep, and the CPU oracle's rate on a bounded sample of the same
+per-cell work.
 """
 import argparse
 import ctypes as C
@@ -25,6 +34,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell-node of one launch:
+#   sweep     read f once, write the convected f once (FP64)
+#   collide   read f, the cached equilibrium and both CDFs once, write the
+#             result (fd or the FP32 path's FP64 change) once
+SWEEP_BYTES_PER_NODE = 2 * 8
+COLLIDE_BYTES_PER_NODE = 5 * 8
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
 METRIC = "cell-node updates/s, FP32-collision vs plain FP64 at 1e-5; p50/p99 solve latency"
 
 
@@ -69,7 +85,7 @@ def run_gpu(pkg, cfg, f, steps, warmup):
     return (t1 - t0), {k: v / max(calls, 1) for k, v in phase.items()}, props
 
 
-def cpu_baseline(pkg, cfg_full, steps=4):
+def cpu_baseline(pkg, cfg_full, steps=12):
     """The oracle (serial C, one core) on a 16x16-cell cavity with the same
     velocity grid and samples per cell; rate in cell-node updates/s."""
     from tests._oracle import oracle_solver
@@ -86,6 +102,96 @@ def cpu_baseline(pkg, cfg_full, steps=4):
             "kind": "port",
             "sample": f"CPU oracle (serial C, -O2), 16x16-cell cavity, {cfg.nvx}^3 nodes, "
                       f"{cfg.samples_per_cell} samples/cell, {steps} full steps, {dt:.1f} s"}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of a kernel from the table of rocprofv3 PMC
+    measurements (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration),
+    or None when it has not been measured."""
+    if not os.path.exists(TRAFFIC_TABLE):
+        return None
+    for line in open(TRAFFIC_TABLE):
+        f = line.split()
+        if len(f) >= 3 and not line.startswith("#") and f[0] == kernel and f[1] == workload:
+            return float(f[2])
+    return None
+
+
+def roofline(kernel, workload, alg_bytes, ms):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": measured_traffic(kernel, workload), "algorithmic_bytes_per_launch": alg_bytes,
+            "launch_ms": ms}
+
+
+def run_batched(pkg, count, steps, precision, tune=True):
+    """The batched workload: admit `count` instances, autotune the collision
+    tiles on them (the state is restored), time `steps` graph replays."""
+    inst = pkg.batched_homogeneous(count, precision=precision)
+    nodes = sum(c.num_points for c, _ in inst)
+    samples = sum(c.samples_per_cell for c, _ in inst)
+    b = pkg.Batch(inst[0][0], max_slots=count, node_capacity=nodes, cell_capacity=count)
+    for c, f in inst:
+        b.admit(c, f)
+    b.step(1)
+    b.synchronize()
+    tuned = b.tune(1)[0] if tune else None
+    b.step(1)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    l1 = b.l1()
+    tiles = b.tiles()
+    b.close()
+    return {"instances": count, "nodes": nodes, "samples_per_step": samples, "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "cell_node_updates_per_s": nodes * steps / wall,
+            "collision_samples_per_s": samples * steps / wall, "tiles": tiles, "tuned": tuned,
+            "l1_median_after": float(np.median(l1[l1 >= 0]))}
+
+
+def run_serving(pkg, count, rate, precision):
+    reqs = pkg.synthetic_requests(count=count, rate=rate, seed=5, precision=precision)
+    pools = dict(node_capacity=160_000_000, cell_capacity=65536, max_slots=512)
+    with pkg.Server(reqs[0].cfg, check_every=10, keep_moments=True, keep_f=True, **pools) as srv:
+        st = srv.run(reqs)
+    # error of every solve against the same solve in plain FP64 (same seed),
+    # beside the spread between two FP64 runs with different seeds (the Monte
+    # Carlo noise of the method)
+    from importlib import import_module
+    serve = import_module(pkg.__name__ + ".serve")
+    errs = serve.replay_fp64(reqs, st.outcomes, check_every=10, **pools)
+    noise = serve.replay_fp64(reqs, [o for o in st.outcomes if o.kind == "shock"], check_every=10,
+                              seed_offset=1_000_000, **pools)
+    for o in st.outcomes:
+        o.f = None
+
+    def summary(table, kind):
+        e = [table[o.rid] for o in st.outcomes if o.kind == kind and o.rid in table]
+        if not e:
+            return None
+        return {q: {"median": float(np.median([x[q] for x in e])), "max": float(np.max([x[q] for x in e]))}
+                for q in e[0]}
+
+    err_summary = {k: summary(errs, k) for k in ("homogeneous", "shock")}
+    err_summary["shock_fp64_seed_spread"] = summary(noise, "shock")
+    lat = {k: [o.latency for o in st.outcomes if o.kind == k] for k in ("homogeneous", "shock")}
+    reasons = {}
+    for o in st.outcomes:
+        reasons[f"{o.kind}:{o.reason}"] = reasons.get(f"{o.kind}:{o.reason}", 0) + 1
+    return {"requests": count, "rate_per_s": rate, "wall_s": st.wall_s,
+            "latency_p50_s": st.latency_p50, "latency_p99_s": st.latency_p99,
+            "latency_p50_s_by_kind": {k: float(np.percentile(v, 50)) for k, v in lat.items() if v},
+            "latency_p99_s_by_kind": {k: float(np.percentile(v, 99)) for k, v in lat.items() if v},
+            "collision_samples_per_s": st.samples / st.wall_s, "cell_node_updates_per_s": st.node_updates / st.wall_s,
+            "max_resident": st.max_resident, "stop_reasons": reasons,
+            "fp32_error_vs_fp64": err_summary,
+            "workload": f"{count} requests, Poisson {rate}/s; 70% space-homogeneous mixtures (edge 15..41 odd, "
+                        f"50 samples/node, stop at L1 <= 1e-4 or L1 plateau or 300 steps), 30% 1-D shocks "
+                        f"(Mach U[1.2,3], 50-400 cells, edge 21..31 odd, 20000 samples/cell, stop at density "
+                        f"change <= 2e-3 per 10 steps or 300 steps); FP32 collision arithmetic"}
 
 
 def moment_error(p32, p64):
@@ -109,6 +215,13 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true", help="also run a 512 MiB device copy (profiling)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="batched: keep the default tiles (profiling runs)")
+    ap.add_argument("--batched-instances", type=int, default=4096)
+    ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--serving-requests", type=int, default=120)
+    ap.add_argument("--serving-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -127,10 +240,8 @@ def main():
     head = "fp32" if "fp32" in runs else "fp64"
     wall, phase, props = runs[head]
     value = updates / wall
-    sweep_ms = phase["sweep"]
-    alg_bytes = 2 * 8 * ncell * N
-    achieved = alg_bytes / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else None
     samples_per_step = cfg.samples_per_cell * ncell
+    workload = f"cavity{cfg.nx}_{head}"
     out = {
         "metric": METRIC,
         "value": value,
@@ -152,10 +263,10 @@ def main():
                    "precision": head},
         "collision_samples_per_s": samples_per_step * args.steps / wall,
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_convect2d (upwind sweep)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "algorithmic_bytes_per_launch": alg_bytes},
+        # the collision kernel is the dominant one; it is bound by VALU work and
+        # LDS atomics per sample rather than by HBM, so its HBM fraction is low
+        "roofline": roofline("k_collide", workload, COLLIDE_BYTES_PER_NODE * ncell * N, phase["collide"]),
+        "roofline_sweep": roofline("k_convect2d", workload, SWEEP_BYTES_PER_NODE * ncell * N, phase["sweep"]),
     }
     if "fp64" in runs and "fp32" in runs:
         w64 = runs["fp64"][0]
@@ -164,6 +275,21 @@ def main():
         out["speedup_fp32_vs_fp64"] = w64 / wall
         out["fp32_moment_rel_error_vs_fp64"] = moment_error(runs["fp32"][2], runs["fp64"][2])
         out["fp32_moment_error_after_steps"] = args.warmup + args.steps
+    if not args.no_batched:
+        out["batched"] = {}
+        for prec in ("fp32", "fp64"):
+            code = pkg.capi.DVM_PREC_FP32 if prec == "fp32" else pkg.capi.DVM_PREC_FP64
+            out["batched"][prec] = run_batched(pkg, args.batched_instances, args.batched_steps, code,
+                                               tune=not args.no_tune)
+            print(f"[bench] batched {prec}: {out['batched'][prec]['ms_per_step']:.1f} ms/step",
+                  file=sys.stderr, flush=True)
+        out["batched"]["workload"] = (f"{args.batched_instances} space-homogeneous instances, edge U{{15..41 odd}}, "
+                                      "1-3 Maxwellian mixtures, 50 samples/node, one replayed HIP graph per step, "
+                                      "autotuned collision tiles")
+    if not args.no_serving:
+        out["serving"] = run_serving(pkg, args.serving_requests, args.serving_rate, pkg.capi.DVM_PREC_FP32)
+        print(f"[bench] serving: p50 {out['serving']['latency_p50_s']:.3f} s, p99 {out['serving']['latency_p99_s']:.3f} s",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pkg, cfg)
     print(json.dumps(out))

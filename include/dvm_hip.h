@@ -184,13 +184,15 @@ int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_rec
 
 /* Device time per phase, measured with HIP events on the context's stream
    around every phase that dvm_step / dvm_collide / dvm_convect / dvm_moments
-   enqueue. Phases: 0 CDF build, 1 collision sampling, 2 wall ghosts,
-   3 upwind sweep, 4 moments. dvm_timing_reset waits for pending work and
-   zeroes the totals; dvm_timing_read waits for pending work and returns the
-   totals in ms since the last reset and the number of calls they cover.
+   enqueue. Phases: 0 CDF build, 1 collision prefill (only when a cell's slice
+   exceeds LDS), 2 collision sampling (the collide kernel alone), 3 FP32
+   conservation correction, 4 wall ghosts, 5 upwind sweep, 6 moments.
+   dvm_timing_reset waits for pending work and zeroes the totals;
+   dvm_timing_read waits for pending work and returns the totals in ms since
+   the last reset and the number of calls they cover.
    (No counterpart in the reference, which times phases with cpu_time at
    solver/dvm_solver.f90:294-366.) */
-#define DVM_NPHASE 5
+#define DVM_NPHASE 7
 int dvm_timing_reset(dvm_ctx* ctx);
 int dvm_timing_read(dvm_ctx* ctx, double* phase_ms /* [DVM_NPHASE] */, int64_t* calls);
 
