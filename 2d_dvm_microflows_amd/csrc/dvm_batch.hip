@@ -130,7 +130,9 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(P.queue, 1);
     __syncthreads();
-    const int it = s_item;
+    // uniform across the workgroup: the item, its slot descriptor, pointers
+    // and window bounds then live in scalar registers
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= n_items) return;
     const WorkItem w = items[it];
     const SlotDesc& d = slots[w.slot];

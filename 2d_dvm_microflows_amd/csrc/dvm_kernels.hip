@@ -693,7 +693,7 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
     int o = (int)x - wd[d] / 2;
     o = o < 0 ? 0 : o;
     o = o > n - wd[d] ? n - wd[d] : o;
-    win.o[d] = o;
+    win.o[d] = __builtin_amdgcn_readfirstlane(o);  // the same in every lane
   }
   win.wxy = win.w[0] * win.w[1];
   win.wn = win.wxy * win.w[2];
