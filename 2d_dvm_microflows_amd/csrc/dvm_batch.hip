@@ -20,7 +20,8 @@
 //                      an LDS window (the whole grid when it fits, else a box
 //                      around the cell's bulk velocity, spilling the rest to
 //                      global FP64 atomics)
-//   k_conserve_cells   FP32 path: FP64 conservation correction
+//   k_conserve_cells   FP32 path: FP64 conservation correction of the cells
+//                      the collision kernel could not correct itself
 //   k_convect1d_cells  1-D slots: the zone-grid sweep with in/out flow
 //   k_tick_slots       step counter += 1; ping-pong index flips unless the
 //                      sweep brought f back into the original pool
@@ -126,6 +127,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
                                                       const WorkItem* __restrict__ items, int n_items,
                                                       CollParams base, BatchPools P) {
   extern __shared__ double dyn[];
+  __shared__ ConserveShared<BS> csh;
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(P.queue, 1);
@@ -151,7 +153,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
     const int bad = collide_item<kF32, BS, K>(g, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
-                                              P.f[st.cur ^ 1] + n0, P.dacc + n0);
+                                              P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh);
     if (bad) atomicOr(P.err, 1);
     __syncthreads();  // LDS axes, accumulator and s_item are reused by the next item
   }
@@ -164,6 +166,7 @@ __global__ void __launch_bounds__(BS) k_conserve_cells(const SlotDesc* __restric
   __shared__ ConserveShared<BS> sh;
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
+  if (d.full && d.split == 1) return;  // corrected inside the collision kernel
   const int cur = state[r.slot].cur;
   const size_t n0 = cell_node0(d, r.cell);
   conserve_cell<BS>(d.g, P.f[cur] + n0, P.dacc + n0, P.f[cur ^ 1] + n0, sh);
@@ -383,6 +386,7 @@ static const void* batch_collide_kernel(bool f32, int bs, int k) {
   switch (bs) {
     case 256: return f32 ? batch_kernel_k<true, 256>(k) : batch_kernel_k<false, 256>(k);
     case 512: return f32 ? batch_kernel_k<true, 512>(k) : batch_kernel_k<false, 512>(k);
+    case 768: return f32 ? batch_kernel_k<true, 768>(k) : batch_kernel_k<false, 768>(k);
     default: return f32 ? batch_kernel_k<true, 1024>(k) : batch_kernel_k<false, 1024>(k);
   }
 }
@@ -447,8 +451,8 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
   b->nodes.init(node_cap);
   b->cells.init(cell_cap);
   b->cdfs.init(2 * (node_cap + cell_cap));
-  const int lds_max = (int)(kLdsBytes - 64);
-  for (int bs : {256, 512, 1024})
+  const int lds_max = (int)(kLdsBytes - kStaticLds);
+  for (int bs : {256, 512, 768, 1024})
     for (int k : {1, 2}) BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k),
                                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   return DVM_OK;
@@ -864,7 +868,8 @@ extern "C" int dvm_batch_capacity(dvm_batch* b, int64_t* free_nodes, int64_t* la
 
 extern "C" int dvm_batch_set_tiles(dvm_batch* b, int32_t lanes, int32_t pick_k, int32_t item_samples) {
   if (!b) return bfail(b, DVM_E_ARG, "null batch");
-  if (lanes != 256 && lanes != 512 && lanes != 1024) return bfail(b, DVM_E_ARG, "lanes must be 256, 512 or 1024");
+  if (lanes != 256 && lanes != 512 && lanes != 768 && lanes != 1024)
+    return bfail(b, DVM_E_ARG, "lanes must be 256, 512, 768 or 1024");
   if (pick_k != 1 && pick_k != 2) return bfail(b, DVM_E_ARG, "pick_k must be 1 or 2");
   if (item_samples != 0 && item_samples < 1024)
     return bfail(b, DVM_E_ARG, "item_samples must be 0 (sized from the load) or >= 1024");
@@ -956,7 +961,7 @@ extern "C" int dvm_batch_tune(dvm_batch* b, int32_t steps, int32_t* tiles_out, d
     return DVM_OK;
   };
   rc = (e == hipSuccess) ? DVM_OK : bfail(b, DVM_E_HIP, "tune setup: %s", hipGetErrorString(e));
-  for (int bs : {256, 512, 1024})
+  for (int bs : {256, 512, 768, 1024})
     for (int k : {1, 2})
       if (rc == DVM_OK) rc = trial(bs, k, old_item);
   if (rc == DVM_OK && b->species.sample_mode == DVM_SAMPLES_EXPLICIT) {

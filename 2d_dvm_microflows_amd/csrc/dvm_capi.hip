@@ -181,14 +181,17 @@ static const void* collide_kernel(const dvm_ctx* c) {
   const bool f32 = c->cfg.precision == DVM_PREC_FP32;
   switch (c->coll_bs) {
     case 256: return collide_kernel_bs<256>(c->pick_k, c->lds_collide, f32);
+    case 768: return collide_kernel_bs<768>(c->pick_k, c->lds_collide, f32);
     case 1024: return collide_kernel_bs<1024>(c->pick_k, c->lds_collide, f32);
     default: return collide_kernel_bs<512>(c->pick_k, c->lds_collide, f32);
   }
 }
 
 // LDS window nodes available to a collision workgroup next to the staged axes
-// (64 bytes kept for the kernels' static LDS).
-constexpr int kWindowCap = (int)((kLdsBytes - 64) / sizeof(double)) - kAxisSlots;
+// and the kernels' static LDS (the conservation reduction of the FP32 path).
+constexpr size_t kStaticLds = 3072;
+static_assert(sizeof(ConserveShared<1024>) + 64 <= kStaticLds, "static LDS of the collide kernels");
+constexpr int kWindowCap = (int)((kLdsBytes - kStaticLds) / sizeof(double)) - kAxisSlots;
 
 // Extents of the collision LDS window for a grid of n[0] x n[1] x n[2] nodes:
 // the grid itself when it fits (returns true), else the largest box, near
@@ -346,7 +349,7 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   const bool f32 = cfg->precision == DVM_PREC_FP32;
   if (const char* e = getenv("DVM_COLL_BS")) {
     const int v = atoi(e);
-    if (v == 256 || v == 512 || v == 1024) c->coll_bs = v;
+    if (v == 256 || v == 512 || v == 768 || v == 1024) c->coll_bs = v;
   }
   if (const char* e = getenv("DVM_PICK_K")) {
     const int v = atoi(e);
@@ -529,20 +532,27 @@ static int prep_async(dvm_ctx* c) {
   return DVM_OK;
 }
 
+// The FP32 path corrects conservation inside the collide kernel when the
+// cell's slice fits LDS, else in k_conserve.
+static bool separate_conserve(const dvm_ctx* c) {
+  return c->cfg.precision == DVM_PREC_FP32 && !c->lds_collide;
+}
+
 static int collide_async(dvm_ctx* c, int ntime) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
   void* args[] = {&c->g,   &c->collp, &ntime,  &c->wd,     &c->props,   &c->f,   &c->fd,
                   &c->dacc, &c->eqv, &c->cdf, &c->tabs, &c->counts, &c->factors, &c->err};
   HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3(c->ncell), dim3(c->coll_bs), args, c->coll_lds, c->stream));
-  if (c->cfg.precision != DVM_PREC_FP32) {
+  if (!separate_conserve(c)) {
     double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
   }
   return DVM_OK;
 }
 
-// FP32 path: fd = f + dacc with the FP64 conservation correction, then phi := delta_phi.
+// FP32 path beyond LDS: fd = f + dacc with the FP64 conservation correction,
+// then phi := delta_phi.
 static int conserve_async(dvm_ctx* c) {
-  if (c->cfg.coll_solver != DVM_COLL_VR || c->cfg.precision != DVM_PREC_FP32) return DVM_OK;
+  if (c->cfg.coll_solver != DVM_COLL_VR || !separate_conserve(c)) return DVM_OK;
   hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;

@@ -673,6 +673,95 @@ __device__ __forceinline__ CellView cell_view(const double* f, const double* eqv
   return cv;
 }
 
+// FP64 conservation correction of the collision change d of each cell computed
+// with FP32 arithmetic: d is replaced by d - |f| (lambda . phi), phi = (1, v, |v|^2), with
+// lambda solving sum |f| phi phi^T lambda = sum d phi. In exact arithmetic
+// the collision step conserves mass, momentum and energy (each sample's
+// depletions equal its replenishments and the remap keeps moments 0-2); the
+// correction removes what FP32 rounding leaves, so the cell's moments change by
+// FP64 round-off only. One workgroup per cell.
+template <int BS>
+struct ConserveShared {
+  double red[BS / 64][20];
+  double tot[20];
+  double M[25];
+  double lam[5];
+};
+
+template <int BS>
+__device__ __forceinline__ void conserve_cell(const Grid& g, const double* __restrict__ fc,
+                                              const double* __restrict__ da, double* __restrict__ out,
+                                              ConserveShared<BS>& sh) {
+  const int N = g.N;
+  auto& red = sh.red;
+  double* lam = sh.lam;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // The 20 sums (sum d phi_a, then sum |f| phi_a phi_b for a <= b) in two
+  // rounds of 10 accumulators, so that the reduction fits next to a
+  // register-heavy caller.
+#pragma unroll 1
+  for (int round = 0; round < 2; ++round) {
+    double sums[10];
+#pragma unroll
+    for (int z = 0; z < 10; ++z) sums[z] = 0.0;
+    for (int l = threadIdx.x; l < N; l += BS) {
+      int i, j, k;
+      node_ijk(g, l, i, j, k);
+      const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+      const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
+      if (round == 0) {
+        const double d = da[l];
+        const double w = fabs(fc[l]);
+#pragma unroll
+        for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
+#pragma unroll
+        for (int b = 0; b < 5; ++b) sums[5 + b] += w * phi[0] * phi[b];
+      } else {
+        const double w = fabs(fc[l]);
+        int o = 0;
+#pragma unroll
+        for (int a = 1; a < 5; ++a)
+#pragma unroll
+          for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
+      }
+    }
+#pragma unroll
+    for (int z = 0; z < 10; ++z) {
+      const double v = wave_sum(sums[z]);
+      if (lane == 0) red[wid][round * 10 + z] = v;
+    }
+  }
+  __syncthreads();
+  // totals, the 5x5 system and its solve in LDS (one lane solves; register
+  // arrays here would set the register budget of a heavy caller)
+  double* tot = sh.tot;
+  double* M = sh.M;
+  if (threadIdx.x < 20) {
+    double v = 0.0;
+    for (int w = 0; w < BS / 64; ++w) v += red[w][threadIdx.x];
+    tot[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int o = 5;
+    for (int a = 0; a < 5; ++a)
+      for (int c = a; c < 5; ++c) { M[a * 5 + c] = tot[o]; M[c * 5 + a] = tot[o]; ++o; }
+    for (int a = 0; a < 5; ++a) lam[a] = tot[a];
+    if (!solve5(M, lam))
+      for (int a = 0; a < 5; ++a) lam[a] = 0.0;
+  }
+  __syncthreads();
+  const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
+  for (int l = threadIdx.x; l < N; l += BS) {
+    int i, j, k;
+    node_ijk(g, l, i, j, k);
+    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+    const double fl = fc[l];
+    const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
+    out[l] = fl + (da[l] - corr);
+  }
+}
+
 // LDS accumulation window of a cell: a box of w[0] x w[1] x w[2] nodes, the
 // whole grid when the cell's FP64 slice fits LDS, otherwise centred on the
 // node nearest to the cell's bulk velocity (where a relaxing distribution and
@@ -710,11 +799,15 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
 //   sole: the item is the cell's only one and writes its window directly
 //         (FP64: fd = f + change; FP32: dacc = change, which the conservation
 //         kernel adds to f); otherwise it adds its window to tgt atomically.
+//   FP32, full and sole: the whole change is in LDS, so the FP64
+//         conservation correction runs here (csh its reduction space) and
+//         fd = f + corrected change is written directly.
 template <bool kF32, int BS, int K>
 __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp, const CellView& cv, int rng_cell,
                                             int ntime, int q0, int q1, const int* __restrict__ cnt,
                                             const double* __restrict__ fac, const Window& win, bool full, bool sole,
-                                            double* acc, double* __restrict__ fd, double* __restrict__ da) {
+                                            double* acc, double* __restrict__ fd, double* __restrict__ da,
+                                            ConserveShared<BS>* csh) {
   const int N = g.N;
   const int total = cnt[0] + cnt[1] + cnt[2];
   q1 = q1 < total ? q1 : total;
@@ -739,6 +832,10 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
                                        });
   }
   __syncthreads();
+  if (kF32 && full && sole) {
+    conserve_cell<BS>(g, cv.f, acc, fd, *csh);
+    return bad;
+  }
   const int nx = g.n[0], nxy = g.nxy;
   for (int q = threadIdx.x; q < wn; q += BS) {
     int l = q;
@@ -762,7 +859,8 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
 //   kLds: the cell's FP64 slice fits LDS (window = grid, no prefill);
 //         otherwise the window of extents wd around the cell's bulk velocity,
 //         with fd prefilled with f (FP64) or dacc zeroed (FP32).
-//   kF32: FP32 collision arithmetic (dacc, then k_conserve), else FP64 (fd).
+//   kF32: FP32 collision arithmetic; with kLds the conservation correction is
+//         applied here and fd written, else dacc (then k_conserve). FP64: fd.
 struct WinDims {
   int w[3];
 };
@@ -776,6 +874,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                                 const int2* __restrict__ tabs, const int* __restrict__ counts,
                                                 const double* __restrict__ factors, int* __restrict__ err) {
   extern __shared__ double dyn[];
+  __shared__ ConserveShared<BS> csh;
   const int cell = blockIdx.x;
   const int N = g.N;
   Window win{};
@@ -785,80 +884,8 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                 tabs + (size_t)cell * 2 * kNBucket, N);
   const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, 0, 0x7fffffff, counts + cell * 4,
                                             factors + cell * 3, win, kLds, true, dyn + kAxisSlots,
-                                            fd + (size_t)cell * N, dacc + (size_t)cell * N);
+                                            fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh);
   if (bad) atomicOr(err, 1);
-}
-
-// FP64 conservation correction of the collision change d of each cell computed
-// with FP32 arithmetic: d is replaced by d - |f| (lambda . phi), phi = (1, v, |v|^2), with
-// lambda solving sum |f| phi phi^T lambda = sum d phi. In exact arithmetic
-// the collision step conserves mass, momentum and energy (each sample's
-// depletions equal its replenishments and the remap keeps moments 0-2); the
-// correction removes what FP32 rounding leaves, so the cell's moments change by
-// FP64 round-off only. One workgroup per cell.
-template <int BS>
-struct ConserveShared {
-  double red[BS / 64][20];
-  double lam[5];
-};
-
-template <int BS>
-__device__ __forceinline__ void conserve_cell(const Grid& g, const double* __restrict__ fc,
-                                              const double* __restrict__ da, double* __restrict__ out,
-                                              ConserveShared<BS>& sh) {
-  const int N = g.N;
-  auto& red = sh.red;
-  double* lam = sh.lam;
-  double sums[20];
-#pragma unroll
-  for (int z = 0; z < 20; ++z) sums[z] = 0.0;
-  for (int l = threadIdx.x; l < N; l += BS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-    const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
-    const double d = da[l];
-    const double w = fabs(fc[l]);
-#pragma unroll
-    for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
-    int o = 5;
-#pragma unroll
-    for (int a = 0; a < 5; ++a)
-#pragma unroll
-      for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int z = 0; z < 20; ++z) {
-    const double v = wave_sum(sums[z]);
-    if (lane == 0) red[wid][z] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double tot[20];
-    for (int z = 0; z < 20; ++z) {
-      double v = 0.0;
-      for (int w = 0; w < BS / 64; ++w) v += red[w][z];
-      tot[z] = v;
-    }
-    double M[25], b[5];
-    int o = 5;
-    for (int a = 0; a < 5; ++a)
-      for (int c = a; c < 5; ++c) { M[a * 5 + c] = tot[o]; M[c * 5 + a] = tot[o]; ++o; }
-    for (int a = 0; a < 5; ++a) b[a] = tot[a];
-    const bool ok = solve5(M, b);
-    for (int a = 0; a < 5; ++a) lam[a] = ok ? b[a] : 0.0;
-  }
-  __syncthreads();
-  const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
-  for (int l = threadIdx.x; l < N; l += BS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-    const double fl = fc[l];
-    const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
-    out[l] = fl + (da[l] - corr);
-  }
 }
 
 template <int BS>

@@ -254,7 +254,7 @@ int dvm_batch_slot_info(dvm_batch* b, int32_t slot, int32_t* steps, int64_t* nod
 int dvm_batch_capacity(dvm_batch* b, int64_t* free_nodes, int64_t* largest_free, int32_t* free_cells,
                        int32_t* free_slots);
 
-/* Collision tiles: lanes per workgroup (256, 512, 1024), samples per lane
+/* Collision tiles: lanes per workgroup (256, 512, 768, 1024), samples per lane
    picked in lockstep (1, 2) and samples per work item (a cell's samples are
    split into items of that many, each accumulating in its own LDS window and
    adding it to the cell atomically; 0, the default, sizes items from the
