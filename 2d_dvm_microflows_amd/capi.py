@@ -17,6 +17,7 @@ DVM_CS_CLARKE, DVM_CS_BIRD, DVM_CS_VKI = 1, 2, 3
 DVM_DIM_0D, DVM_DIM_1D, DVM_DIM_2D = 0, 1, 2
 DVM_PREC_FP64, DVM_PREC_FP32 = 0, 1
 DVM_SAMPLES_REFERENCE, DVM_SAMPLES_EXPLICIT = 0, 1
+DVM_CONV_UPWIND1, DVM_CONV_MINMOD2 = 1, 2
 DVM_NPROP = 8
 P_DENS, P_U, P_V, P_W, P_TRTEMP, P_TEMP, P_PRESSURE, P_AVGSPEED = range(8)
 W_DENS, W_U, W_V, W_W, W_TEMP = range(5)
@@ -46,6 +47,7 @@ class DvmConfig(C.Structure):
         ("bc_left", C.c_int32), ("bc_right", C.c_int32),
         ("acc_coeff", C.c_double),
         ("wall", (C.c_double * 5) * 4),
+        ("conv_order", C.c_int32),
     ]
 
 

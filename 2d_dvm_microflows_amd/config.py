@@ -66,6 +66,7 @@ class SolverConfig:
     bc_right: int = capi.DVM_BC_WALL
     acc_coeff: float = 1.0
     wall: np.ndarray = field(default_factory=lambda: np.tile(np.array([1.0, 0.0, 0.0, 0.0, 1.0]), (4, 1)))
+    conv_order: int = capi.DVM_CONV_UPWIND1
     name: str = "custom"
 
     @property
@@ -106,6 +107,7 @@ class SolverConfig:
         for s in range(4):
             for q in range(5):
                 c.wall[s][q] = w[s, q]
+        c.conv_order = self.conv_order
         return c, keep
 
 

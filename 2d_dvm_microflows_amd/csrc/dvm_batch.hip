@@ -49,7 +49,8 @@ struct SlotDesc {
   int wd[3];               // LDS window extents
   int split;               // work items per cell (sample ranges)
   int convect;             // 1-D sweep after the collision
-  int bc[2];               // 1-D boundary types (left, right)
+  int bc[2];               // 1-D boundary types (left, right): in/out flow or zero gradient
+  int order;               // 1-D sweep order
 };
 
 struct SlotState {
@@ -187,9 +188,15 @@ __global__ void __launch_bounds__(kConvBS) k_convect1d_cells(const SlotDesc* __r
     const int cur = state[r.slot].cur;
     int i, j, k;
     node_ijk(d.g, l, i, j, k);
-    const double xv = d.g.ax[0][i];
-    const double cfl = d.fx * xv;
-    convect1d_point(N, d.ncell, d.bc, cfl, xv, P.f[cur ^ 1] + d.off, P.f[cur] + d.off, P.inflow + d.off, r.cell, l);
+    ConvParams p{};
+    p.nx = d.ncell;
+    p.fx = d.fx;
+    p.acc = 1.0;
+    p.bc[0] = d.bc[0];
+    p.bc[1] = d.bc[1];
+    p.order = d.order;
+    convect1d_point(d.g, p, i, d.g.ax[0][i], P.f[cur ^ 1] + d.off, P.f[cur] + d.off, P.inflow + d.off, nullptr,
+                    nullptr, nullptr, r.cell, l);
   }
 }
 
@@ -508,6 +515,9 @@ static int check_instance(dvm_batch* b, const dvm_config* inst) {
       inst->cdf_tolerance != s.cdf_tolerance || inst->num_repl != s.num_repl || inst->coln_rms != s.coln_rms ||
       inst->sample_mode != s.sample_mode || inst->precision != s.precision)
     return bfail(b, DVM_E_ARG, "instance species / collision settings differ from the batch's");
+  if (inst->dim == DVM_DIM_1D && inst->enable_convection &&
+      (inst->bc_left == DVM_BC_WALL || inst->bc_right == DVM_BC_WALL))
+    return bfail(b, DVM_E_ARG, "batched 1-D solves take in/out flow or zero-gradient ends (1-D walls: dvm_ctx)");
   if (inst->sample_mode == DVM_SAMPLES_EXPLICIT && inst->samples_per_cell > 2147483647LL)
     return bfail(b, DVM_E_ARG, "samples_per_cell must fit in 31 bits");
   return DVM_OK;
@@ -562,6 +572,7 @@ extern "C" int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const doubl
   d.fx = inst->dt / (inst->dx > 0.0 ? inst->dx : 1.0);
   d.bc[0] = inst->bc_left;
   d.bc[1] = inst->bc_right;
+  d.order = inst->conv_order == 2 ? 2 : 1;
   const int N = hg.g.N;
   if (d.convect) {
     std::vector<double> inf;

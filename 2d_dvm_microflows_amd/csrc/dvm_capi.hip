@@ -144,6 +144,9 @@ static int validate(const dvm_config* cfg) {
   if (cfg->enable_convection && cfg->dim != DVM_DIM_0D && (!(cfg->dx > 0.0) || (cfg->dim == DVM_DIM_2D && !(cfg->dy > 0.0))))
     return fail(nullptr, DVM_E_ARG, "dx (and dy in 2-D) must be > 0");
   if (!(cfg->acc_coeff >= 0.0 && cfg->acc_coeff <= 1.0)) return fail(nullptr, DVM_E_ARG, "acc_coeff must be in [0,1]");
+  if (cfg->conv_order < 0 || cfg->conv_order > 2) return fail(nullptr, DVM_E_ARG, "conv_order must be 1 or 2");
+  if (cfg->conv_order == 2 && cfg->dim != DVM_DIM_1D)
+    return fail(nullptr, DVM_E_ARG, "the second-order sweep is available for 1-D runs");
   return DVM_OK;
 }
 
@@ -374,6 +377,7 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   p.acc = cfg->acc_coeff;
   p.bc[0] = cfg->bc_left;
   p.bc[1] = cfg->bc_right;
+  p.order = cfg->conv_order == 2 ? 2 : 1;
   for (int s = 0; s < 4; ++s) { p.wall_side[s] = 0; p.couette_side[s] = 0; p.flux_out[s] = 1.0; }
   // boundary_conditions.f90:662-673 (flag kept per grid; the reference's SAVE makes it sticky across grids)
   const int izero = c->g.zero[0], jzero = c->g.zero[1];
@@ -422,6 +426,37 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     inflow_maxwellians(cfg, hg, inf);
     if ((rc = dalloc(c, &c->inflow, (size_t)2 * N))) return rc;
     HIP_TRY(c, hipMemcpy(c->inflow, inf.data(), inf.size() * sizeof(double), hipMemcpyHostToDevice));
+    // 1-D walls (apply_diffuse_reflection, grid_ref_code
+    // boundary_conditions.f90:566-781): M_wall beta^3 on the outgoing half
+    // and its sum, in the reference's loop order
+    std::vector<double> wm((size_t)2 * N, 0.0);
+    bool any_wall = false;
+    for (int s = 0; s < 2; ++s) {
+      if (p.bc[s] != DVM_BC_WALL) continue;
+      any_wall = true;
+      const double* wl = cfg->wall[s];
+      const double kt = wl[DVM_W_TEMP], u = wl[DVM_W_U], v = wl[DVM_W_V], w = wl[DVM_W_W];
+      const double mc = sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * kt * kt * kt));
+      const int o0 = s == 0 ? c->g.pos[0] : 0, o1 = s == 0 ? n[0] - 1 : c->g.neg[0];
+      double computed = 0.0;
+      for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+          for (int i = o0; i <= o1; ++i) {
+            const double beta3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
+            const double mv = maxwell_host(c->ax[0][i], c->ax[1][j], c->ax[2][k], mc, m, u, v, w, kt);
+            wm[(size_t)s * N + i + n[0] * (j + n[1] * k)] = mv * beta3;
+            computed = computed + mv * beta3;
+          }
+      p.flux_out[s] = computed;
+    }
+    if (any_wall) {
+      if ((rc = dalloc(c, &c->wallM, (size_t)2 * N)) || (rc = dalloc(c, &c->ratio, 2)) ||
+          (rc = dalloc(c, &c->spec, (size_t)2 * N)))
+        return rc;
+      HIP_TRY(c, hipMemcpy(c->wallM, wm.data(), wm.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIP_TRY(c, hipMemset(c->ratio, 0, 2 * sizeof(double)));
+      HIP_TRY(c, hipMemset(c->spec, 0, (size_t)2 * N * sizeof(double)));
+    }
   }
   return DVM_OK;
 }
@@ -562,9 +597,20 @@ static int conserve_async(dvm_ctx* c) {
 static bool convects(const dvm_ctx* c) { return c->cfg.enable_convection && c->cfg.dim != DVM_DIM_0D; }
 
 static int walls_async(dvm_ctx* c) {
-  if (!convects(c) || c->cfg.dim != DVM_DIM_2D) return DVM_OK;
+  if (!convects(c)) return DVM_OK;
   ConvParams& p = c->convp;
   const int N = c->N;
+  if (c->cfg.dim == DVM_DIM_1D) {
+    if (!c->wallM) return DVM_OK;  // no 1-D wall
+    hipLaunchKernelGGL(k_wall1d_ratio, dim3(2), dim3(kMomBS), 0, c->stream, c->g, p, c->f, c->ratio);
+    if (p.acc < 1.0) {
+      hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, c->stream, c->spec, (long long)2 * N, 0.0);
+      hipLaunchKernelGGL(k_spec1d, dim3((N + 255) / 256, 2), dim3(256), 0, c->stream, c->g, p, c->f, c->spec,
+                         c->err);
+    }
+    HIP_TRY(c, hipGetLastError());
+    return DVM_OK;
+  }
   if (p.acc > 0.0)
     hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, c->stream, c->g, p, c->f, c->ratio);
   if (p.acc < 1.0) {
@@ -587,7 +633,7 @@ static int sweep_async(dvm_ctx* c) {
                        c->fd, c->wallM, c->ratio, c->spec);
   else
     hipLaunchKernelGGL(k_convect1d, dim3(chunks, ys), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd,
-                       c->inflow);
+                       c->inflow, c->wallM, c->ratio, c->spec);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;  // phi := phi_conv
   return DVM_OK;

@@ -50,7 +50,9 @@ struct ConvParams {
   int couette_side[4];
   int bc[2];                     // 1-D boundary types (left, right)
   int izflag, jzflag;            // zero-plane flags of boundary_conditions.f90:662-673
-  double flux_out[4];            // sum |v_n| M_wall beta^3 over the wall's outgoing half-space
+  double flux_out[4];            // 2-D: sum |v_n| M_wall beta^3 over the wall's outgoing half-space;
+                                 // 1-D walls (sides 0, 1): sum M_wall beta^3 over the outgoing half
+  int order;                     // 1-D sweep order (1 or 2)
 };
 
 __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass);
@@ -73,6 +75,10 @@ __global__ void k_convect2d(Grid g, ConvParams p, const double* __restrict__ f, 
                             const double* __restrict__ wallM, const double* __restrict__ ratio,
                             const double* __restrict__ spec);
 __global__ void k_convect1d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ fo,
-                            const double* __restrict__ inflow);
+                            const double* __restrict__ inflow, const double* __restrict__ wallM,
+                            const double* __restrict__ ratio, const double* __restrict__ spec);
+__global__ void k_wall1d_ratio(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ ratio);
+__global__ void k_spec1d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ spec,
+                         int* __restrict__ err);
 
 }  // namespace dvm

@@ -1104,37 +1104,149 @@ __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, con
 }
 
 // 1-D sweep of the zone-grid solver (grid_ref_code convection.f90:111-256)
-// with its in/out-flow (boundary_conditions.f90:783-913) and zero-gradient
-// (:915-973) boundary cells.
-__device__ __forceinline__ double upwind1d(const double* __restrict__ f, size_t idx, int N, double cfl, double xv) {
-  const double p0 = f[idx];
-  if (xv < 0.0) return p0 - cfl * (f[idx + N] - p0);
-  return p0 - cfl * (p0 - f[idx - N]);
+// with its in/out-flow (boundary_conditions.f90:783-913), zero-gradient
+// (:915-973) and diffuse / specular wall (:566-781) boundary cells. Order 2
+// adds the limited Lax-Wendroff correction with minmod slopes (zero slope in
+// the boundary cells), which the reference does not have:
+//   x >= 0: - nu (1 - nu) / 2 (s_i - s_(i-1)),   nu = dt/dx x
+//   x <  0: + mu (1 - mu) / 2 (s_i - s_(i+1)),   mu = -nu.
+__device__ __forceinline__ double minmod(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  return fabs(a) < fabs(b) ? a : b;
 }
 
-// One node l of 1-D cell ix (of nx) from the post-collision f into fo.
-__device__ __forceinline__ void convect1d_point(int N, int nx, const int* bc, double cfl, double xv,
+__device__ __forceinline__ double slope1d(const double* __restrict__ f, int ix, int nx, size_t N, int l) {
+  if (ix <= 0 || ix >= nx - 1) return 0.0;
+  const double p0 = f[(size_t)ix * N + l];
+  return minmod(f[(size_t)(ix + 1) * N + l] - p0, p0 - f[(size_t)(ix - 1) * N + l]);
+}
+
+__device__ __forceinline__ double upwind1d(const double* __restrict__ f, int ix, int nx, int N, int l, double cfl,
+                                           double xv, int order) {
+  const double p0 = f[(size_t)ix * N + l];
+  if (xv < 0.0) {
+    double out = p0 - cfl * (f[(size_t)(ix + 1) * N + l] - p0);
+    if (order == 2) {
+      const double mu = -cfl;
+      out = out + (mu * (1.0 - mu) * 0.5) * (slope1d(f, ix, nx, N, l) - slope1d(f, ix + 1, nx, N, l));
+    }
+    return out;
+  }
+  double out = p0 - cfl * (p0 - f[(size_t)(ix - 1) * N + l]);
+  if (order == 2) out = out - (cfl * (1.0 - cfl) * 0.5) * (slope1d(f, ix, nx, N, l) - slope1d(f, ix - 1, nx, N, l));
+  return out;
+}
+
+// x-index range of the incoming and outgoing halves of a 1-D wall (side 0
+// left, 1 right), from the trunk's zero points (the zone-grid solver reads
+// i_neg / i_pos without setting them).
+__device__ __forceinline__ void wall1d_halves(const Grid& g, int side, int& in0, int& in1, int& out0, int& out1) {
+  if (side == 0) { in0 = 0; in1 = g.neg[0]; out0 = g.pos[0]; out1 = g.n[0] - 1; }
+  else { in0 = g.pos[0]; in1 = g.n[0] - 1; out0 = 0; out1 = g.neg[0]; }
+}
+
+// One node l of 1-D cell ix from the post-collision f into fo.
+__device__ __forceinline__ void convect1d_point(const Grid& g, const ConvParams& p, int i, double xv,
                                                 const double* __restrict__ f, double* __restrict__ fo,
-                                                const double* __restrict__ inflow, int ix, int l) {
+                                                const double* __restrict__ inflow, const double* __restrict__ wallM,
+                                                const double* __restrict__ ratio, const double* __restrict__ spec,
+                                                int ix, int l) {
+  const int N = g.N, nx = p.nx;
+  const double cfl = p.fx * xv;
   const size_t idx = (size_t)ix * N + l;
-  if (ix > 0 && ix < nx - 1) { fo[idx] = upwind1d(f, idx, N, cfl, xv); return; }
+  if (ix > 0 && ix < nx - 1) { fo[idx] = upwind1d(f, ix, nx, N, l, cfl, xv, p.order); return; }
   const int side = (ix == 0) ? 0 : 1;
-  const int b = bc[side];
-  if (b == DVM_BC_INOUT) fo[idx] = inflow[(size_t)side * N + l];
-  else if (b == DVM_BC_ZEROGRAD) fo[idx] = upwind1d(f, (side == 0) ? idx + N : idx - N, N, cfl, xv);
-  else fo[idx] = f[idx];
+  const int b = p.bc[side];
+  if (b == DVM_BC_INOUT) {
+    fo[idx] = inflow[(size_t)side * N + l];
+  } else if (b == DVM_BC_ZEROGRAD) {
+    fo[idx] = upwind1d(f, side == 0 ? 1 : nx - 2, nx, N, l, cfl, xv, p.order);
+  } else if (b == DVM_BC_WALL) {
+    int in0, in1, out0, out1;
+    wall1d_halves(g, side, in0, in1, out0, out1);
+    double v = 0.0;
+    if (i >= out0 && i <= out1) {
+      v = ratio[side] * wallM[(size_t)side * N + l];
+    } else if (i >= in0 && i <= in1) {
+      const double f0 = f[idx], fn = f[side == 0 ? idx + N : idx - N];
+      v = f0 - fabs(p.fx * xv) * (f0 - fn);
+    }
+    if (p.acc < 1.0) v = v + spec[(size_t)side * N + l];
+    fo[idx] = v;
+  } else {
+    fo[idx] = f[idx];
+  }
 }
 
 __global__ void __launch_bounds__(kConvBS) k_convect1d(Grid g, ConvParams p, const double* __restrict__ f,
-                                                       double* __restrict__ fo, const double* __restrict__ inflow) {
+                                                       double* __restrict__ fo, const double* __restrict__ inflow,
+                                                       const double* __restrict__ wallM,
+                                                       const double* __restrict__ ratio,
+                                                       const double* __restrict__ spec) {
   const int N = g.N;
   const int l = blockIdx.x * kConvBS + threadIdx.x;
   if (l >= N) return;
   int i, j, k;
   node_ijk(g, l, i, j, k);
   const double xv = g.ax[0][i];
-  const double cfl = p.fx * xv;
-  for (int ix = blockIdx.y; ix < p.nx; ix += gridDim.y) convect1d_point(N, p.nx, p.bc, cfl, xv, f, fo, inflow, ix, l);
+  for (int ix = blockIdx.y; ix < p.nx; ix += gridDim.y)
+    convect1d_point(g, p, i, xv, f, fo, inflow, wallM, ratio, spec, ix, l);
+}
+
+// 1-D wall scale factor per side (grid.x = side): acc * (sum of the wall
+// cell's f over the incoming half) / (sum of M_wall beta^3 over the outgoing
+// half), apply_diffuse_reflection (boundary_conditions.f90:566-781).
+__global__ void __launch_bounds__(kMomBS) k_wall1d_ratio(Grid g, ConvParams p, const double* __restrict__ f,
+                                                         double* __restrict__ ratio) {
+  __shared__ double red[kMomBS / 64];
+  const int side = blockIdx.x;
+  if (p.bc[side] != DVM_BC_WALL) return;
+  int in0, in1, out0, out1;
+  wall1d_halves(g, side, in0, in1, out0, out1);
+  const double* fc = f + (size_t)(side == 0 ? 0 : p.nx - 1) * g.N;
+  double s = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(g, l, i, j, k);
+    if (i >= in0 && i <= in1) s += fc[l];
+  }
+  s = block_sum<kMomBS>(s, red);
+  if (threadIdx.x == 0) ratio[side] = (p.acc * s) / p.flux_out[side];
+}
+
+// Specular part of a 1-D wall: (1 - acc) f of every incoming node of the wall
+// cell, remapped from (-x, y, z), added into spec[side] (zeroed before). The
+// reference visits the incoming nodes in index order and assigns each node
+// before adding its own remap, so what a node adds to a later incoming node
+// is overwritten: those contributions are dropped here.
+__global__ void k_spec1d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ spec,
+                         int* __restrict__ err) {
+  const int side = blockIdx.y;
+  if (p.bc[side] != DVM_BC_WALL) return;
+  const int N = g.N;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= N) return;
+  int i, j, k;
+  node_ijk(g, l, i, j, k);
+  int in0, in1, out0, out1;
+  wall1d_halves(g, side, in0, in1, out0, out1);
+  if (i < in0 || i > in1) return;
+  const double f0 = f[(size_t)(side == 0 ? 0 : p.nx - 1) * N + l];
+  const double ds = (1.0 - p.acc) * f0;
+  Stencil st;
+  remap(g, -g.ax[0][i], g.ax[1][j], g.ax[2][k], 1.0, st);
+  if (st.flag == 0) { atomicOr(err, 2); return; }
+  double* sp = spec + (size_t)side * N;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    if (q >= 4 && !(st.flag & (1 << (q - 4)))) continue;
+    int ti, tj, tk;
+    stencil_ijk(st, q, ti, tj, tk);
+    const int t = ti + g.n[0] * tj + g.nxy * tk;
+    if ((unsigned)t >= (unsigned)N) continue;
+    if (ti >= in0 && ti <= in1 && t > l) continue;  // overwritten by the later node's assignment
+    atomicAdd(&sp[t], ds * st.m[q]);
+  }
 }
 
 }  // namespace dvm

@@ -16,6 +16,7 @@ module DVMHip
   integer(c_int32_t), parameter, public :: DVM_DIM_0D = 0, DVM_DIM_1D = 1, DVM_DIM_2D = 2
   integer(c_int32_t), parameter, public :: DVM_PREC_FP64 = 0, DVM_PREC_FP32 = 1
   integer(c_int32_t), parameter, public :: DVM_SAMPLES_REFERENCE = 0, DVM_SAMPLES_EXPLICIT = 1
+  integer(c_int32_t), parameter, public :: DVM_CONV_UPWIND1 = 1, DVM_CONV_MINMOD2 = 2
   integer, parameter, public :: DVM_NPROP = 8
   integer, parameter, public :: DVM_P_DENS = 1, DVM_P_U = 2, DVM_P_V = 3, DVM_P_W = 4, DVM_P_TRTEMP = 5, &
        DVM_P_TEMP = 6, DVM_P_PRESSURE = 7, DVM_P_AVGSPEED = 8   ! 1-based positions in a props row
@@ -42,6 +43,7 @@ module DVMHip
      integer(c_int32_t) :: bc_left, bc_right
      real(c_double)     :: acc_coeff
      real(c_double)     :: wall(5,4)   ! wall(q, side): C wall[side][q]
+     integer(c_int32_t) :: conv_order = 1
   end type dvm_config
 
   public :: dvm_ctx_create, dvm_ctx_destroy, dvm_last_error, dvm_upload_f, dvm_download_f, &

@@ -60,6 +60,9 @@ extern "C" {
 #define DVM_SAMPLES_REFERENCE 0 /* nint(dt*T^(2/3)*nA*nB/(2 Kn rms^2 beta3)) per integral */
 #define DVM_SAMPLES_EXPLICIT  1 /* samples_per_cell split over the 3 integrals
                                    in the reference's nA*nB proportions       */
+/* 1-D sweep order (dvm_config.conv_order) */
+#define DVM_CONV_UPWIND1  1
+#define DVM_CONV_MINMOD2  2
 
 /* ---- per-cell property record returned by dvm_moments ------------------ */
 #define DVM_P_DENS        0   /* number density                 */
@@ -109,6 +112,10 @@ typedef struct dvm_config {
   int32_t bc_left, bc_right;/* DVM_BC_* (bottom/top are walls in 2-D, as trunk) */
   double  acc_coeff;        /* accommodation: 1 diffuse, 0 specular             */
   double  wall[4][5];       /* [side][DVM_W_*]                                  */
+  /* 1-D sweep order: 1 the reference's donor-cell upwind (0 is taken as 1);
+     2 limited second-order upwind (minmod slopes), which the reference does
+     not have */
+  int32_t conv_order;
 } dvm_config;
 
 typedef struct dvm_ctx dvm_ctx;

@@ -1068,25 +1068,113 @@ static void interior_2d(orc_ctx* c, int ix, int iy, double* fc) {
 
 /* 1-D upwind and boundaries of the zone-grid solver
    (grid_ref_code convection.f90:111-256, boundary_conditions.f90:319-444,783-973). */
-static void convect_1d(orc_ctx* c, double* fc) {
+/* Limited slope of 1-D cell ix for one node (stride N): minmod of the two
+   one-sided differences; zero in the boundary cells. Second-order upwind is
+   not in the reference (its 1-D solver has first-order upwind only,
+   grid_ref_code convection.f90:111-256); this is the library's definition. */
+static double minmod(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  return fabs(a) < fabs(b) ? a : b;
+}
+
+static double slope1d(const double* f, int ix, int nxs, size_t N, int l) {
+  if (ix <= 0 || ix >= nxs - 1) return 0.0;
+  const double p0 = f[(size_t)ix * N + l];
+  return minmod(f[(size_t)(ix + 1) * N + l] - p0, p0 - f[(size_t)(ix - 1) * N + l]);
+}
+
+/* Interior 1-D update of node l of cell ix: the reference's donor-cell
+   upwind (convection.f90:111-256 of the zone-grid solver), plus with order 2
+   the limited Lax-Wendroff correction
+     x >= 0: - nu (1 - nu) / 2 (s_i - s_(i-1)),   nu = dt/dx x
+     x <  0: + mu (1 - mu) / 2 (s_i - s_(i+1)),   mu = -nu. */
+static double upwind1d(const double* f, int ix, int nxs, size_t N, int l, double cfl, double xv, int order) {
+  const double p0 = f[(size_t)ix * N + l];
+  if (xv < 0.0) {
+    double out = p0 - cfl * (f[(size_t)(ix + 1) * N + l] - p0);
+    if (order == 2) {
+      const double mu = -cfl;
+      out = out + (mu * (1.0 - mu) * 0.5) * (slope1d(f, ix, nxs, N, l) - slope1d(f, ix + 1, nxs, N, l));
+    }
+    return out;
+  }
+  double out = p0 - cfl * (p0 - f[(size_t)(ix - 1) * N + l]);
+  if (order == 2) out = out - (cfl * (1.0 - cfl) * 0.5) * (slope1d(f, ix, nxs, N, l) - slope1d(f, ix - 1, nxs, N, l));
+  return out;
+}
+
+/* 1-D wall cell, apply_diffuse_reflection of the zone-grid solver
+   (grid_ref_code boundary_conditions.f90:566-781): the cell is zeroed; its
+   outgoing half becomes the wall Maxwellian times beta^3, scaled to
+   acc * (sum of f over the incoming half, not flux-weighted); then, node by
+   node over the incoming half in (k, j, i) order, the node takes the upwind
+   value from the neighbour cell (dt/dx |x| (f - f_nb)) and (1 - acc) f of the
+   node is remapped from (-x, y, z) and added (apply_bc_remapping :975-1053),
+   so a later node's assignment overwrites what earlier nodes added to it.
+   The zone-grid solver never sets the i_neg / i_pos it reads here; the
+   trunk's zero points (velocity_grid.f90:653-689) are used. */
+static int wall_1d(orc_ctx* c, int side, double* out) {
+  const dvm_config* cfg = &c->cfg;
+  const ogrid* g = &c->g;
+  const int N = g->num_points, nxs = cfg->nx, n0 = g->n[0];
+  const int ix = side == 0 ? 0 : nxs - 1, nb = side == 0 ? 1 : nxs - 2;
+  const double* f0 = c->f + (size_t)ix * N;
+  const double* fn = c->f + (size_t)nb * N;
+  const double acc = cfg->acc_coeff, mass = cfg->mass;
+  const double* wl = cfg->wall[side];
+  const double kt = wl[DVM_W_TEMP], u = wl[DVM_W_U], v = wl[DVM_W_V], w = wl[DVM_W_W];
+  int in0, in1, out0, out1;
+  if (side == 0) { in0 = 0; in1 = g->neg[0]; out0 = g->pos[0]; out1 = n0 - 1; }
+  else { in0 = g->pos[0]; in1 = n0 - 1; out0 = 0; out1 = g->neg[0]; }
+  memset(out, 0, sizeof(double) * (size_t)N);
+  double dens = 0.0;
+  for (int k = 0; k < g->n[2]; ++k)
+    for (int j = 0; j < g->n[1]; ++j)
+      for (int i = in0; i <= in1; ++i) dens = dens + f0[i + n0 * (j + g->n[1] * k)];
+  dens = acc * dens;
+  const double mc = sqrt(mass * mass * mass / (PI_REF * PI_REF * PI_REF * kt * kt * kt));
+  double computed = 0.0;
+  for (int k = 0; k < g->n[2]; ++k)
+    for (int j = 0; j < g->n[1]; ++j)
+      for (int i = out0; i <= out1; ++i) {
+        const double beta3 = g->beta[0][i] * g->beta[1][j] * g->beta[2][k];
+        const double mv = maxwell(g->ax[0][i], g->ax[1][j], g->ax[2][k], mc, mass, u, v, w, kt);
+        out[i + n0 * (j + g->n[1] * k)] = mv * beta3;
+        computed = computed + mv * beta3;
+      }
+  const double ratio = dens / computed;
+  for (int l = 0; l < N; ++l) out[l] = ratio * out[l];
+  const double factor = cfg->dt / cfg->dx;
+  for (int k = 0; k < g->n[2]; ++k)
+    for (int j = 0; j < g->n[1]; ++j)
+      for (int i = in0; i <= in1; ++i) {
+        const int l = i + n0 * (j + g->n[1] * k);
+        const double x = g->ax[0][i];
+        out[l] = f0[l] - fabs(factor * x) * (f0[l] - fn[l]);
+        const double ds = (1.0 - acc) * f0[l];
+        int st[7];
+        double m[7];
+        remap1(g, -x, g->ax[1][j], g->ax[2][k], 1.0, st, m);
+        if (st[6] == 0) return set_err(c, DVM_E_REMAP, "1-D wall: remap found no exterior point");
+        scatter_stencil(out, g, st, m, ds);
+      }
+  return DVM_OK;
+}
+
+static int convect_1d(orc_ctx* c, double* fc) {
   const dvm_config* cfg = &c->cfg;
   const ogrid* g = &c->g;
   int N = g->num_points, nxs = cfg->nx;
+  const int order = cfg->conv_order == 2 ? 2 : 1;
   double factor = cfg->dt / cfg->dx;
   for (int ix = 1; ix < nxs - 1; ++ix) {
-    const double* f0 = c->f + (size_t)ix * N;
-    const double* fp = c->f + (size_t)(ix + 1) * N;
-    const double* fm = c->f + (size_t)(ix - 1) * N;
     double* out = fc + (size_t)ix * N;
     int l = 0;
     for (int k = 0; k < g->n[2]; ++k)
       for (int j = 0; j < g->n[1]; ++j)
         for (int i = 0; i < g->n[0]; ++i, ++l) {
           double xv = g->ax[0][i];
-          double cfl = factor * xv;
-          double p0 = f0[l];
-          if (xv < 0.0) out[l] = p0 - cfl * (fp[l] - p0);
-          else          out[l] = p0 - cfl * (p0 - fm[l]);
+          out[l] = upwind1d(c->f, ix, nxs, (size_t)N, l, factor * xv, xv, order);
         }
   }
   for (int side = 0; side < 2; ++side) {
@@ -1109,11 +1197,15 @@ static void convect_1d(orc_ctx* c, double* fc) {
       /* apply_zero_gradient_flow: boundary cell takes its neighbour's new values */
       int nb = side == 0 ? 1 : nxs - 2;
       memcpy(out, fc + (size_t)nb * N, sizeof(double) * N);
+    } else if (bc == DVM_BC_WALL) {
+      int rc = wall_1d(c, side, out);
+      if (rc) return rc;
     } else {
       /* other 1-D boundary types keep the cell unchanged */
       memcpy(out, c->f + (size_t)ix * N, sizeof(double) * N);
     }
   }
+  return DVM_OK;
 }
 
 int orc_convect(orc_ctx* c, int32_t species, int32_t ntime) {
@@ -1124,7 +1216,8 @@ int orc_convect(orc_ctx* c, int32_t species, int32_t ntime) {
   double* fc = c->fd;
   if (c->cfg.dim == DVM_DIM_1D) {
     if (c->cfg.nx < 3) return set_err(c, DVM_E_ARG, "1-D convection needs nx >= 3");
-    convect_1d(c, fc);
+    int rc = convect_1d(c, fc);
+    if (rc) return rc;
   } else {
     int nxs = c->cfg.nx, nys = c->cfg.ny;
     double* ghost = (double*)malloc(sizeof(double) * 4 * (size_t)N);

@@ -25,7 +25,8 @@ def cellwise(a, b):
 
 def instances(pkg, precision=0):
     """Uneven mix sharing one species: two mixtures in LDS, a 29^3 grid that
-    takes global atomics over several work items, a 1-D shock."""
+    takes global atomics over several work items, a 1-D shock with the
+    second-order sweep."""
     rng = np.random.default_rng(5)
     out = [
         pkg.mixture_instance(rng, 11, samples_per_node=4, seed=101, precision=precision),
@@ -34,7 +35,7 @@ def instances(pkg, precision=0):
     c, f = pkg.homogeneous_relaxation(n_v=29, samples=40000, seed=103)
     out.append((c.with_(precision=precision), f))
     c, f = pkg.normal_shock_1d(nx=8, n_v=9, vmax=6.0, samples=600, omega=0.5, seed=104)
-    out.append((c.with_(precision=precision), f))
+    out.append((c.with_(precision=precision, conv_order=2), f))
     return out
 
 

@@ -89,6 +89,9 @@ def test_collision_trace_matches_oracle(pkg, case):
     assert ok, (err, scale)
 
 
+WALLS_1D = np.array([[1.0, 0.0, 0.1, 0.0, 1.0], [1.0, 0.0, -0.1, 0.0, 1.2],
+                     [1.0, 0.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 0.0, 1.0]])
+
 STEP_CASES = {
     "homogeneous": lambda pkg: pkg.homogeneous_relaxation(n_v=15, samples=20000),
     # 29^3 nodes: the FP64 slice exceeds LDS, increments go to global atomics
@@ -103,6 +106,17 @@ STEP_CASES = {
     "shock_1d": lambda pkg: pkg.normal_shock_1d(nx=12, n_v=11, vmax=6.0, samples=1500),
     "shock_1d_zerograd": lambda pkg: (lambda c, f: (c.with_(bc_right=3), f))(
         *pkg.normal_shock_1d(nx=10, n_v=9, vmax=6.0, samples=800)),
+    "shock_1d_minmod": lambda pkg: (lambda c, f: (c.with_(conv_order=2), f))(
+        *pkg.normal_shock_1d(nx=12, n_v=11, vmax=6.0, samples=1500)),
+    "shock_1d_minmod_zerograd": lambda pkg: (lambda c, f: (c.with_(conv_order=2, bc_right=3), f))(
+        *pkg.normal_shock_1d(nx=9, n_v=9, vmax=6.0, samples=600)),
+    # 1-D walls: diffuse (acc 1) on an even grid, partially specular (acc 0.4)
+    # on an odd grid with a node at x = 0 (left out of both halves)
+    "channel_1d_walls_diffuse": lambda pkg: (lambda c, f: (c.with_(bc_left=1, bc_right=1, acc_coeff=1.0, wall=WALLS_1D), f))(
+        *pkg.normal_shock_1d(nx=8, n_v=10, vmax=6.0, samples=800, mach=1.5)),
+    "channel_1d_walls_partial": lambda pkg: (lambda c, f: (c.with_(bc_left=1, bc_right=1, acc_coeff=0.4, wall=WALLS_1D,
+                                                                    conv_order=2), f))(
+        *pkg.normal_shock_1d(nx=8, n_v=9, vmax=6.0, samples=800, mach=1.5)),
 }
 
 
