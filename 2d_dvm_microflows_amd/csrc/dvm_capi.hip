@@ -45,6 +45,7 @@ struct dvm_ctx {
   size_t coll_lds = 0;
   int coll_bs = 1024;            // lanes per collision workgroup (256, 512 or 1024)
   int pick_k = 1;                // samples per lane picked in lockstep (1, 2 or 4)
+  int split = 1;                 // workgroups per cell in the collision kernel
   double* dacc = nullptr;        // [C][N] collision change of the FP32 path
   bool f_valid = false, props_valid = false;
   int trace_ready = -1;
@@ -359,6 +360,22 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     if (v == 1 || v == 2 || v == 4) c->pick_k = v;
   }
   c->lds_collide = window_dims(n, c->wd.w);
+  // Few cells: split each cell's samples over several workgroups (about two
+  // per CU, at least 4096 samples each when the count is known up front).
+  {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    long long want = (2LL * cus + c->ncell - 1) / c->ncell;
+    if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT) {
+      const long long by_samples = cfg->samples_per_cell / 4096;
+      want = want < by_samples ? want : by_samples;
+    }
+    c->split = (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
+    if (const char* e = getenv("DVM_COLL_SPLIT")) {
+      const int v = atoi(e);
+      if (v >= 1 && v <= 1024) c->split = v;
+    }
+  }
   c->coll_lds = (size_t)(kAxisSlots + c->wd.w[0] * c->wd.w[1] * c->wd.w[2]) * sizeof(double);
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
@@ -557,7 +574,7 @@ static int cdf_async(dvm_ctx* c) {
 // fd = f (FP64) or dacc = 0 (FP32) before a collision that spills from its
 // LDS window to global atomics.
 static int prep_async(dvm_ctx* c) {
-  if (c->cfg.coll_solver != DVM_COLL_VR || c->lds_collide) return DVM_OK;
+  if (c->cfg.coll_solver != DVM_COLL_VR || (c->lds_collide && c->split == 1)) return DVM_OK;
   if (c->cfg.precision == DVM_PREC_FP32)
     hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->dacc, (long long)c->ncell * c->N, 0.0);
   else
@@ -568,16 +585,17 @@ static int prep_async(dvm_ctx* c) {
 }
 
 // The FP32 path corrects conservation inside the collide kernel when the
-// cell's slice fits LDS, else in k_conserve.
+// cell's slice fits LDS and one workgroup takes the cell, else in k_conserve.
 static bool separate_conserve(const dvm_ctx* c) {
-  return c->cfg.precision == DVM_PREC_FP32 && !c->lds_collide;
+  return c->cfg.precision == DVM_PREC_FP32 && (!c->lds_collide || c->split > 1);
 }
 
 static int collide_async(dvm_ctx* c, int ntime) {
   if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
-  void* args[] = {&c->g,   &c->collp, &ntime,  &c->wd,     &c->props,   &c->f,   &c->fd,
-                  &c->dacc, &c->eqv, &c->cdf, &c->tabs, &c->counts, &c->factors, &c->err};
-  HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3(c->ncell), dim3(c->coll_bs), args, c->coll_lds, c->stream));
+  void* args[] = {&c->g,    &c->collp, &ntime,   &c->split, &c->wd,     &c->props,   &c->f,  &c->fd,
+                  &c->dacc, &c->eqv,   &c->cdf,  &c->tabs,  &c->counts, &c->factors, &c->err};
+  HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(c->coll_bs), args,
+                             c->coll_lds, c->stream));
   if (!separate_conserve(c)) {
     double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
   }

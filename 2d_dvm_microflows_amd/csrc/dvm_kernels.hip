@@ -855,18 +855,20 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
   return bad;
 }
 
-// Collision phase of one cell per workgroup of BS lanes.
-//   kLds: the cell's FP64 slice fits LDS (window = grid, no prefill);
-//         otherwise the window of extents wd around the cell's bulk velocity,
-//         with fd prefilled with f (FP64) or dacc zeroed (FP32).
-//   kF32: FP32 collision arithmetic; with kLds the conservation correction is
-//         applied here and fd written, else dacc (then k_conserve). FP64: fd.
+// Collision phase: `split` workgroups of BS lanes per cell (workgroup b takes
+// part b % split of cell b / split, a contiguous range of its samples).
+//   kLds: the cell's FP64 slice fits LDS (window = grid); otherwise the
+//         window of extents wd around the cell's bulk velocity.
+//   split == 1 and kLds: fd (FP64) is written directly, or (FP32) the
+//         conservation correction is applied here and fd written. Otherwise
+//         fd is prefilled with f (FP64) or dacc zeroed (FP32, then
+//         k_conserve) and each part adds its window atomically.
 struct WinDims {
   int w[3];
 };
 
 template <bool kLds, bool kF32, int BS, int K>
-__global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, WinDims wd,
+__global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, int split, WinDims wd,
                                                 const double* __restrict__ props,
                                                 const double* __restrict__ f, double* __restrict__ fd,
                                                 double* __restrict__ dacc,
@@ -875,15 +877,18 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                                 const double* __restrict__ factors, int* __restrict__ err) {
   extern __shared__ double dyn[];
   __shared__ ConserveShared<BS> csh;
-  const int cell = blockIdx.x;
+  const int cell = blockIdx.x / split, part = blockIdx.x - cell * split;
   const int N = g.N;
   Window win{};
   if (!kLds) win = make_window(g, wd.w, props + (size_t)cell * DVM_NPROP);
   stage_axes(g, dyn);
   const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
                                 tabs + (size_t)cell * 2 * kNBucket, N);
-  const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, 0, 0x7fffffff, counts + cell * 4,
-                                            factors + cell * 3, win, kLds, true, dyn + kAxisSlots,
+  const int* cnt = counts + cell * 4;
+  const int chunk = (cnt[0] + cnt[1] + cnt[2] + split - 1) / split;
+  const int q0 = part * chunk;
+  const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
+                                            factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh);
   if (bad) atomicOr(err, 1);
 }

@@ -152,6 +152,22 @@ def run_batched(pkg, count, steps, precision, tune=True):
             "l1_median_after": float(np.median(l1[l1 >= 0]))}
 
 
+def run_shock(pkg, steps, warmup):
+    """The 1-D normal shock: 200 cells x 31^3 nodes on +-6, VHS omega 0.81,
+    Mach 2 with the Rankine-Hugoniot downstream state and a tanh profile,
+    in/out flow at both ends, 50 000 samples per cell per step, the limited
+    second-order sweep, plain FP64."""
+    cfg, f = pkg.normal_shock_1d(nx=200, n_v=31, vmax=6.0, mach=2.0, samples=50_000)
+    cfg = cfg.with_(conv_order=pkg.capi.DVM_CONV_MINMOD2, precision=pkg.capi.DVM_PREC_FP64)
+    wall, phase, _ = run_gpu(pkg, cfg, f, steps, warmup)
+    nodes = cfg.nx * cfg.num_points
+    return {"workload": f"1-D normal shock Mach 2, {cfg.nx} cells x {cfg.nvx}^3 nodes on +-6, VHS omega {cfg.omega}, "
+                        f"{cfg.samples_per_cell} samples/cell/step, minmod second-order sweep, in/out flow, FP64",
+            "steps": steps, "ms_per_step": wall / steps * 1e3, "cell_node_updates_per_s": nodes * steps / wall,
+            "collision_samples_per_s": cfg.samples_per_cell * cfg.nx * steps / wall,
+            "phase_ms": {k: round(v, 4) for k, v in phase.items()}}
+
+
 def run_serving(pkg, count, rate, precision):
     reqs = pkg.synthetic_requests(count=count, rate=rate, seed=5, precision=precision)
     pools = dict(node_capacity=160_000_000, cell_capacity=65536, max_slots=512)
@@ -217,6 +233,7 @@ def main():
     ap.add_argument("--copy-calib", action="store_true", help="also run a 512 MiB device copy (profiling)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-shock", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default tiles (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=3)
@@ -275,6 +292,9 @@ def main():
         out["speedup_fp32_vs_fp64"] = w64 / wall
         out["fp32_moment_rel_error_vs_fp64"] = moment_error(runs["fp32"][2], runs["fp64"][2])
         out["fp32_moment_error_after_steps"] = args.warmup + args.steps
+    if not args.no_shock:
+        out["shock_1d"] = run_shock(pkg, args.steps, args.warmup)
+        print(f"[bench] shock 1-D: {out['shock_1d']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_batched:
         out["batched"] = {}
         for prec in ("fp32", "fp64"):
