@@ -17,7 +17,7 @@ module DVMHip
   integer(c_int32_t), parameter, public :: DVM_PREC_FP64 = 0, DVM_PREC_FP32 = 1
   integer(c_int32_t), parameter, public :: DVM_SAMPLES_REFERENCE = 0, DVM_SAMPLES_EXPLICIT = 1
   integer(c_int32_t), parameter, public :: DVM_CONV_UPWIND1 = 1, DVM_CONV_MINMOD2 = 2
-  integer, parameter, public :: DVM_NPROP = 8
+  integer, parameter, public :: DVM_NPROP = 8, DVM_NPHASE = 7
   integer, parameter, public :: DVM_P_DENS = 1, DVM_P_U = 2, DVM_P_V = 3, DVM_P_W = 4, DVM_P_TRTEMP = 5, &
        DVM_P_TEMP = 6, DVM_P_PRESSURE = 7, DVM_P_AVGSPEED = 8   ! 1-based positions in a props row
 
@@ -48,6 +48,9 @@ module DVMHip
 
   public :: dvm_ctx_create, dvm_ctx_destroy, dvm_last_error, dvm_upload_f, dvm_download_f, &
        dvm_moments, dvm_collide, dvm_convect, dvm_step, dvm_timing_reset, dvm_timing_read
+  public :: dvm_batch_create, dvm_batch_destroy, dvm_batch_last_error, dvm_batch_admit, dvm_batch_retire, &
+       dvm_batch_step, dvm_batch_synchronize, dvm_batch_download_f, dvm_batch_moments, dvm_batch_l1, &
+       dvm_batch_tune
   public :: dvm_error_message
 
   interface
@@ -104,9 +107,72 @@ module DVMHip
      integer(c_int) function dvm_timing_read(ctx, phase_ms, calls) bind(c, name="dvm_timing_read")
        import :: c_int, c_ptr, c_double, c_int64_t
        type(c_ptr), value :: ctx
-       real(c_double), intent(out) :: phase_ms(5)
+       real(c_double), intent(out) :: phase_ms(DVM_NPHASE)
        integer(c_int64_t), intent(out) :: calls
      end function dvm_timing_read
+
+     ! batched solves (dvm_batch_* of include/dvm_hip.h)
+     integer(c_int) function dvm_batch_create(species, max_slots, node_capacity, cell_capacity, b) &
+          bind(c, name="dvm_batch_create")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr, dvm_config
+       type(dvm_config), intent(in) :: species
+       integer(c_int32_t), value :: max_slots, cell_capacity
+       integer(c_int64_t), value :: node_capacity
+       type(c_ptr), intent(out) :: b
+     end function dvm_batch_create
+     subroutine dvm_batch_destroy(b) bind(c, name="dvm_batch_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: b
+     end subroutine dvm_batch_destroy
+     type(c_ptr) function dvm_batch_last_error(b) bind(c, name="dvm_batch_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: b
+     end function dvm_batch_last_error
+     integer(c_int) function dvm_batch_admit(b, inst, f0, slot) bind(c, name="dvm_batch_admit")
+       import :: c_int, c_int32_t, c_ptr, c_double, dvm_config
+       type(c_ptr), value :: b
+       type(dvm_config), intent(in) :: inst
+       real(c_double), intent(in) :: f0(*)
+       integer(c_int32_t), intent(out) :: slot
+     end function dvm_batch_admit
+     integer(c_int) function dvm_batch_retire(b, slot) bind(c, name="dvm_batch_retire")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: slot
+     end function dvm_batch_retire
+     integer(c_int) function dvm_batch_step(b, nsteps) bind(c, name="dvm_batch_step")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: nsteps
+     end function dvm_batch_step
+     integer(c_int) function dvm_batch_synchronize(b) bind(c, name="dvm_batch_synchronize")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: b
+     end function dvm_batch_synchronize
+     integer(c_int) function dvm_batch_download_f(b, slot, f) bind(c, name="dvm_batch_download_f")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: slot
+       real(c_double), intent(out) :: f(*)
+     end function dvm_batch_download_f
+     integer(c_int) function dvm_batch_moments(b, slot, props) bind(c, name="dvm_batch_moments")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: slot
+       real(c_double), intent(out) :: props(*)
+     end function dvm_batch_moments
+     integer(c_int) function dvm_batch_l1(b, l1) bind(c, name="dvm_batch_l1")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: b
+       real(c_double), intent(out) :: l1(*)
+     end function dvm_batch_l1
+     integer(c_int) function dvm_batch_tune(b, steps, tiles, ms) bind(c, name="dvm_batch_tune")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: steps
+       integer(c_int32_t), intent(out) :: tiles(3)
+       real(c_double), intent(out) :: ms
+     end function dvm_batch_tune
   end interface
 
 contains
