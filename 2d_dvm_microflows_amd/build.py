@@ -1,4 +1,6 @@
-"""Builds libdvmhip.so for gfx950 in-tree (hipcc, no JIT cache)."""
+"""Builds libdvmhip.so for gfx950 in-tree (hipcc, no JIT cache) and the
+Fortran deck driver bin/dvm_gpu (amdflang) that runs a trunk input deck
+through the library's ISO_C_BINDING module."""
 import os
 import subprocess
 
@@ -33,5 +35,31 @@ def build_hip(verbose=False):
     return out
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose=False):
+    """bin/dvm_gpu from fortran/dvm_hip.f90, dvm_deck.f90 and dvm_gpu.f90,
+    linked against lib/libdvmhip.so with an $ORIGIN-relative rpath. Returns
+    None (and builds nothing) when amdflang is not installed."""
+    if not os.path.exists(FLANG):
+        return None
+    src = [os.path.join(HERE, "fortran", n) for n in ("dvm_hip.f90", "dvm_deck.f90", "dvm_gpu.f90")]
+    bin_dir = os.path.join(HERE, "bin")
+    mod_dir = os.path.join(bin_dir, "mod")
+    os.makedirs(mod_dir, exist_ok=True)
+    out = os.path.join(bin_dir, "dvm_gpu")
+    cmd = [FLANG, "-O2", "-J", mod_dir] + src + ["-o", out + ".tmp", "-L" + os.path.join(HERE, "lib"), "-ldvmhip",
+                                                "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build_hip(verbose=True)
+    build_fortran(verbose=True)

@@ -1,0 +1,449 @@
+! dvm_deck.f90 -- reads a trunk input deck (software/trunk/input_files/*.inp,
+! the keys of input_output/read_input_2D.f90:75-777) and turns it into the
+! dvm_config, velocity axes and initial distribution of the GPU library.
+!
+! Deck format: one "key = value" per line; '#' starts a comment; section
+! headers ("[phy_grid]") and lines without '=' are skipped, as by the
+! reference's file_read (input_output/input.f90). The reference keeps the
+! quotes of quoted values ('1.0d0') and its numeric reads then fail (Q16);
+! here quotes are stripped. Vector values (one entry per species) take their
+! first entry: the library runs one monatomic species.
+!
+! Supported: num_species = 1 of He, Ne, Ar, Xe (species_data.f90:245-524),
+! csection_method 1-4, csection_coeff 1-3, uniform velocity grids
+! (x/y/z_vel_grid_type = 1), iColl 0 or 3, iConv 0 or 1, InitC 1, 2, 3, 5
+! with InitDF = 1 (Maxwellian), LWallBC / RWallBC 1-4. Anything else stops
+! with a message, as the reference stops on an invalid deck value.
+!
+! Extensions (keys the reference does not read):
+!   dvm_precision    64 (default) or 32: FP32 collision arithmetic
+!   dvm_dim          1: zone-grid 1-D semantics for nx_space > 1, ny_space = 1
+!   dvm_conv_order   2: limited second-order 1-D sweep
+!   dvm_samples      > 0: that many samples per cell per step instead of the
+!                    ColnRMS count rule
+module DVMDeck
+  use iso_c_binding
+  use DVMHip
+  implicit none
+  private
+
+  integer, parameter :: max_keys = 1000
+  real(c_double), parameter :: pi_ref = 3.1459265358979323d0   ! constants.f90:30
+  real(c_double), parameter :: kb = 1.3805d-23                 ! constants.f90:31
+
+  type, public :: deck_t
+     integer :: n = 0
+     character(len=64) :: key(max_keys)
+     character(len=256) :: val(max_keys)
+  end type deck_t
+
+  ! Everything a run needs, derived from the deck.
+  type, public :: run_t
+     type(dvm_config) :: cfg
+     real(c_double), allocatable :: vx(:), vy(:), vz(:)
+     real(c_double), allocatable :: f0(:,:)            ! (nnode, ncell)
+     integer :: ntstep = 0, display_freq = 1, verbosity = 2
+     integer :: out_x = 1, out_y = 1                   ! screen_output_loc
+     real(c_double) :: c_ref, sigma_ref, l_scale, t_scale, mass_ref, diam_ref, dens_ref, temp_ref
+     character(len=16) :: species = ''
+  end type run_t
+
+  public :: deck_read, deck_has, deck_real, deck_int, deck_str, run_from_deck, write_reference_values
+
+contains
+
+  subroutine deck_read(filename, d, ok, msg)
+    character(len=*), intent(in) :: filename
+    type(deck_t), intent(out) :: d
+    logical, intent(out) :: ok
+    character(len=*), intent(out) :: msg
+    character(len=512) :: line
+    integer :: u, ios, p, q
+    ok = .false.
+    msg = ''
+    open(newunit=u, file=filename, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       msg = 'cannot open deck ' // trim(filename)
+       return
+    end if
+    do
+       read(u, '(a)', iostat=ios) line
+       if (ios /= 0) exit
+       p = index(line, '#')
+       if (p > 0) line = line(1:p-1)
+       p = index(line, '=')
+       if (p <= 1) cycle
+       if (d%n >= max_keys) exit
+       d%n = d%n + 1
+       d%key(d%n) = adjustl(line(1:p-1))
+       d%val(d%n) = adjustl(line(p+1:))
+       ! strip quotes
+       do q = 1, len_trim(d%val(d%n))
+          if (d%val(d%n)(q:q) == "'" .or. d%val(d%n)(q:q) == '"') d%val(d%n)(q:q) = ' '
+       end do
+       d%val(d%n) = adjustl(d%val(d%n))
+    end do
+    close(u)
+    ok = .true.
+  end subroutine deck_read
+
+  integer function find_key(d, key)
+    type(deck_t), intent(in) :: d
+    character(len=*), intent(in) :: key
+    integer :: i
+    find_key = 0
+    do i = 1, d%n
+       if (trim(d%key(i)) == key) then
+          find_key = i
+          return
+       end if
+    end do
+  end function find_key
+
+  logical function deck_has(d, key)
+    type(deck_t), intent(in) :: d
+    character(len=*), intent(in) :: key
+    deck_has = find_key(d, key) > 0
+  end function deck_has
+
+  ! First whitespace- or comma-separated token of a value (one species).
+  function first_token(v) result(t)
+    character(len=*), intent(in) :: v
+    character(len=256) :: t
+    integer :: i
+    t = adjustl(v)
+    do i = 1, len_trim(t)
+       if (t(i:i) == ' ' .or. t(i:i) == ',') then
+          t = t(1:i-1)
+          return
+       end if
+    end do
+  end function first_token
+
+  real(c_double) function deck_real(d, key, default, token)
+    type(deck_t), intent(in) :: d
+    character(len=*), intent(in) :: key
+    real(c_double), intent(in) :: default
+    integer, intent(in), optional :: token
+    integer :: i, ios, k, n
+    character(len=256) :: rest
+    deck_real = default
+    i = find_key(d, key)
+    if (i == 0) return
+    rest = d%val(i)
+    n = 1
+    if (present(token)) n = token
+    do k = 1, n - 1
+       rest = adjustl(rest(len_trim(first_token(rest)) + 2:))
+    end do
+    read(first_token(rest), *, iostat=ios) deck_real
+    if (ios /= 0) deck_real = default
+  end function deck_real
+
+  integer function deck_int(d, key, default, token)
+    type(deck_t), intent(in) :: d
+    character(len=*), intent(in) :: key
+    integer, intent(in) :: default
+    integer, intent(in), optional :: token
+    real(c_double) :: r
+    if (present(token)) then
+       r = deck_real(d, key, real(default, c_double), token)
+    else
+       r = deck_real(d, key, real(default, c_double))
+    end if
+    deck_int = nint(r)
+  end function deck_int
+
+  function deck_str(d, key, default) result(s)
+    type(deck_t), intent(in) :: d
+    character(len=*), intent(in) :: key, default
+    character(len=256) :: s
+    integer :: i
+    s = default
+    i = find_key(d, key)
+    if (i > 0) s = first_token(d%val(i))
+  end function deck_str
+
+  ! Monatomic species constants of species_data.f90:245-524, kept as the
+  ! reference writes them (default-real literals for diameter and mass).
+  subroutine species_constants(code, diam, mass, omega, alpha, visc, tvisc, ok)
+    character(len=*), intent(in) :: code
+    real(c_double), intent(out) :: diam, mass, omega, alpha, visc, tvisc
+    logical, intent(out) :: ok
+    ok = .true.
+    visc = 0.0d0
+    tvisc = 0.0d0
+    select case (trim(code))
+    case ('He')
+       diam = 2.33e-10; mass = 6.65e-27; omega = 0.66d0; alpha = 1.31d0
+    case ('Ne')
+       diam = 2.77e-10; mass = 33.5e-27; omega = 0.66d0; alpha = 1.31d0
+    case ('Ar')
+       diam = 4.17e-10; mass = 66.3e-27; omega = 0.81d0; alpha = 1.40d0
+       visc = 2.75e-5; tvisc = 380.0d0
+    case ('Xe')
+       diam = 5.74e-10; mass = 218.e-27; omega = 0.85d0; alpha = 1.44d0
+    case default
+       ok = .false.
+    end select
+  end subroutine species_constants
+
+  ! The deck's run: scaling (scaling.f90:96-130), species constants scaled by
+  ! the reference species (species_data.f90:155-229), grids, boundary states
+  ! and the initial distribution (initial_conditions.f90:212-650).
+  subroutine run_from_deck(d, r, ok, msg)
+    type(deck_t), intent(in) :: d
+    type(run_t), intent(inout) :: r
+    logical, intent(out) :: ok
+    character(len=*), intent(out) :: msg
+    character(len=16) :: ref_species
+    real(c_double) :: dref, mref, d_s, m_s, om, al, vis, tvis, dd, mm, oo, aa, vv, tt
+    real(c_double) :: vmin(3), vmax(3), dens, u, v, w, temp, opt, coeff, s, x, y, z, b
+    integer :: npt(3), method, icoll, iconv, initc, mid, nx, ny, ncell, n, i, j, k, c, ix, iy, gt
+    logical :: sok
+    type(dvm_config) :: cfg
+    character(len=2), parameter :: side(4) = ['LW', 'RW', 'BW', 'TW']
+    ok = .false.
+    msg = ''
+    if (deck_int(d, 'num_species', 1) /= 1) then
+       msg = 'num_species must be 1 (one monatomic species)'
+       return
+    end if
+    r%species = deck_str(d, 'species_codes', 'Ar')
+    ref_species = deck_str(d, 'species_ref', r%species)
+    call species_constants(r%species, d_s, m_s, om, al, vis, tvis, sok)
+    if (.not. sok) then
+       msg = 'species ' // trim(r%species) // ' is not a supported monatomic species (He, Ne, Ar, Xe)'
+       return
+    end if
+    call species_constants(ref_species, dref, mref, oo, aa, vv, tt, sok)
+    if (.not. sok) then
+       msg = 'species_ref ' // trim(ref_species) // ' is not a supported monatomic species'
+       return
+    end if
+    ! scaling.f90:96-108 set_scaling_parameters
+    r%dens_ref = deck_real(d, 'n_ref', 1.0d20)
+    r%temp_ref = deck_real(d, 'T_ref', 273.0d0)
+    r%mass_ref = mref
+    r%diam_ref = dref
+    cfg%kn = deck_real(d, 'Kn', 1.0d0)
+    r%c_ref = sqrt(2.0d0 * kb * r%temp_ref / mref)
+    r%sigma_ref = pi_ref * dref * dref
+    r%l_scale = 1.0d0 / (cfg%kn * r%dens_ref * r%sigma_ref)
+    r%t_scale = r%l_scale / r%c_ref
+    ! species_data.f90:155-229 and the collision-model overrides :503-520
+    method = deck_int(d, 'csection_method', 1)
+    select case (method)
+    case (1); om = 0.5d0; al = 1.0d0
+    case (2); om = 1.0d0; al = 1.0d0
+    case (3); al = 1.0d0
+    case (4)
+    case default
+       msg = 'csection_method must be 1-4'
+       return
+    end select
+    cfg%mass = m_s / mref
+    cfg%diam = d_s / dref
+    cfg%omega = om
+    cfg%alpha = al
+    cfg%viscosity = vis * r%sigma_ref / (mref * r%c_ref)
+    cfg%t_viscosity = tvis / r%temp_ref
+    cfg%csection_coeff = deck_int(d, 'csection_coeff', 1)
+    if (cfg%csection_coeff == DVM_CS_VKI .and. vis == 0.0d0) then
+       msg = 'csection_coeff 3 (vki) needs the species viscosity (Ar only)'
+       return
+    end if
+    ! physical grid
+    nx = deck_int(d, 'nx_space', 1)
+    ny = deck_int(d, 'ny_space', 1)
+    cfg%nx = nx
+    cfg%ny = ny
+    cfg%dx = deck_real(d, 'delta_x', 1.0d0)
+    cfg%dy = deck_real(d, 'delta_y', 1.0d0)
+    if (nx == 1 .and. ny == 1) then
+       cfg%dim = DVM_DIM_0D
+    else if (deck_int(d, 'dvm_dim', 2) == 1) then
+       cfg%dim = DVM_DIM_1D
+    else
+       cfg%dim = DVM_DIM_2D
+    end if
+    ! time stepping
+    cfg%dt = deck_real(d, 'deltat', 0.01d0)
+    r%ntstep = deck_int(d, 'ntstep', 0)
+    ! velocity grid (velocity_grid.f90:636-648 generate_uniform_grid)
+    npt = [deck_int(d, 'num_points_x', 11), deck_int(d, 'num_points_y', 11), deck_int(d, 'num_points_z', 11)]
+    vmin = [deck_real(d, 'xv_min', -3.0d0), deck_real(d, 'yv_min', -3.0d0), deck_real(d, 'zv_min', -3.0d0)]
+    vmax = [deck_real(d, 'xv_max', 3.0d0), deck_real(d, 'yv_max', 3.0d0), deck_real(d, 'zv_max', 3.0d0)]
+    do k = 1, 3
+       gt = 1
+       if (k == 1) gt = deck_int(d, 'x_vel_grid_type', 1)
+       if (k == 2) gt = deck_int(d, 'y_vel_grid_type', 1)
+       if (k == 3) gt = deck_int(d, 'z_vel_grid_type', 1)
+       if (gt /= 1) then
+          msg = 'only uniform velocity grids (vel_grid_type = 1) are supported'
+          return
+       end if
+    end do
+    allocate(r%vx(npt(1)), r%vy(npt(2)), r%vz(npt(3)))
+    do i = 0, npt(1) - 1
+       r%vx(i + 1) = vmin(1) + dble(i) * (vmax(1) - vmin(1)) / dble(npt(1) - 1)
+    end do
+    do i = 0, npt(2) - 1
+       r%vy(i + 1) = vmin(2) + dble(i) * (vmax(2) - vmin(2)) / dble(npt(2) - 1)
+    end do
+    do i = 0, npt(3) - 1
+       r%vz(i + 1) = vmin(3) + dble(i) * (vmax(3) - vmin(3)) / dble(npt(3) - 1)
+    end do
+    cfg%nvx = npt(1)
+    cfg%nvy = npt(2)
+    cfg%nvz = npt(3)
+    ! collisions
+    icoll = deck_int(d, 'iColl', 3)
+    select case (icoll)
+    case (0); cfg%coll_solver = DVM_COLL_NONE
+    case (3); cfg%coll_solver = DVM_COLL_VR
+    case default
+       msg = 'iColl must be 0 (none) or 3 (variance reduction)'
+       return
+    end select
+    cfg%cdf_tolerance = deck_real(d, 'cdf_tolerance', 0.01d0)
+    cfg%num_repl = deck_int(d, 'NRepl', 1)
+    cfg%coln_rms = deck_real(d, 'ColnRMS', 0.001d0)
+    cfg%sample_mode = DVM_SAMPLES_REFERENCE
+    cfg%samples_per_cell = 0
+    if (deck_int(d, 'dvm_samples', 0) > 0) then
+       cfg%sample_mode = DVM_SAMPLES_EXPLICIT
+       cfg%samples_per_cell = deck_int(d, 'dvm_samples', 0)
+    end if
+    cfg%seed = int(deck_int(d, 'seed', 0), c_int64_t)
+    cfg%precision = DVM_PREC_FP64
+    if (deck_int(d, 'dvm_precision', 64) == 32) cfg%precision = DVM_PREC_FP32
+    ! convection and boundaries
+    iconv = deck_int(d, 'iConv', 1)
+    if (iconv /= 0 .and. iconv /= 1) then
+       msg = 'iConv must be 0 or 1 (first-order upwind)'
+       return
+    end if
+    cfg%enable_convection = iconv
+    cfg%conv_order = deck_int(d, 'dvm_conv_order', 1)
+    cfg%bc_left = deck_int(d, 'LWallBC', DVM_BC_WALL)
+    cfg%bc_right = deck_int(d, 'RWallBC', DVM_BC_WALL)
+    cfg%acc_coeff = deck_real(d, 'acc_coeff', 1.0d0)
+    do k = 1, 4
+       cfg%wall(1, k) = deck_real(d, 'dens_' // side(k), 1.0d0)
+       cfg%wall(2, k) = deck_real(d, 'x_vel_' // side(k), 0.0d0)
+       cfg%wall(3, k) = deck_real(d, 'y_vel_' // side(k), 0.0d0)
+       cfg%wall(4, k) = deck_real(d, 'z_vel_' // side(k), 0.0d0)
+       cfg%wall(5, k) = deck_real(d, 'tr_temp_' // side(k), 1.0d0)
+    end do
+    ! screen output
+    r%verbosity = deck_int(d, 'verbosity', 2)
+    r%display_freq = max(1, deck_int(d, 'display_freq', 1))
+    r%out_x = min(max(deck_int(d, 'screen_output_loc', 1, 1), 1), nx)
+    r%out_y = min(max(deck_int(d, 'screen_output_loc', 1, 2), 1), ny)
+    ! initial distribution: Maxwellian x beta^3 per cell at the InitC state,
+    ! renormalised to the cell's density (initial_conditions.f90:338-650)
+    if (deck_int(d, 'InitDF', 1) /= 1) then
+       msg = 'only InitDF = 1 (Maxwellian) is supported'
+       return
+    end if
+    initc = deck_int(d, 'InitC', 5)
+    mid = deck_int(d, 'spatial_middle', nx / 2)
+    if (initc < 1 .or. initc > 5 .or. initc == 4) then
+       msg = 'InitC must be 1, 2, 3 or 5 (the Rankine-Hugoniot profile, 4, is not supported)'
+       return
+    end if
+    ncell = nx * ny
+    n = npt(1) * npt(2) * npt(3)
+    allocate(r%f0(n, ncell))
+    do iy = 1, ny
+       do ix = 1, nx
+          c = ix + nx * (iy - 1)
+          select case (initc)
+          case (1)
+             dens = cfg%wall(1, 1); u = cfg%wall(2, 1); v = cfg%wall(3, 1); w = cfg%wall(4, 1); temp = cfg%wall(5, 1)
+          case (2)
+             dens = cfg%wall(1, 2); u = cfg%wall(2, 2); v = cfg%wall(3, 2); w = cfg%wall(4, 2); temp = cfg%wall(5, 2)
+          case (3)
+             if (ix > mid) then
+                dens = cfg%wall(1, 2); u = cfg%wall(2, 2); v = cfg%wall(3, 2); w = cfg%wall(4, 2); temp = cfg%wall(5, 2)
+             else
+                dens = cfg%wall(1, 1); u = cfg%wall(2, 1); v = cfg%wall(3, 1); w = cfg%wall(4, 1); temp = cfg%wall(5, 1)
+             end if
+          case default
+             dens = deck_real(d, 'dens_init', 1.0d0)
+             u = deck_real(d, 'x_vel_init', 0.0d0)
+             v = deck_real(d, 'y_vel_init', 0.0d0)
+             w = deck_real(d, 'z_vel_init', 0.0d0)
+             temp = deck_real(d, 'tr_temp_init', 1.0d0)
+          end select
+          if (dens == 0.0d0) then
+             r%f0(:, c) = 0.0d0
+             cycle
+          end if
+          opt = cfg%mass / (pi_ref * temp)
+          coeff = dens * sqrt(opt * opt * opt)
+          s = 0.0d0
+          i = 0
+          do k = 1, npt(3)
+             z = r%vz(k)
+             do j = 1, npt(2)
+                y = r%vy(j)
+                do gt = 1, npt(1)
+                   x = r%vx(gt)
+                   i = i + 1
+                   b = beta(r%vx, gt) * beta(r%vy, j) * beta(r%vz, k)
+                   r%f0(i, c) = coeff * exp(-((x - u)**2 + (y - v)**2 + (z - w)**2) * cfg%mass / temp) * b
+                   s = s + r%f0(i, c)
+                end do
+             end do
+          end do
+          r%f0(:, c) = (dens / s) * r%f0(:, c)
+       end do
+    end do
+    r%cfg = cfg
+    ok = .true.
+  end subroutine run_from_deck
+
+  ! velocity_grid.f90:486-540 compute_beta
+  real(c_double) function beta(a, i)
+    real(c_double), intent(in) :: a(:)
+    integer, intent(in) :: i
+    integer :: m
+    m = size(a)
+    if (i == 1) then
+       beta = a(2) - a(1)
+    else if (i == m) then
+       beta = a(m) - a(m - 1)
+    else
+       beta = (a(i + 1) - a(i)) / 2.0d0 + (a(i) - a(i - 1)) / 2.0d0
+    end if
+  end function beta
+
+  ! reference_values.txt in the format of scaling.f90:203-237.
+  subroutine write_reference_values(r, filename)
+    type(run_t), intent(in) :: r
+    character(len=*), intent(in) :: filename
+    integer :: u
+    real(c_double) :: press, hf, e
+    press = 2.0d0 * kb * r%dens_ref * r%temp_ref
+    hf = r%mass_ref * r%c_ref * r%c_ref * r%c_ref
+    e = r%mass_ref * r%c_ref * r%c_ref
+    open(newunit=u, file=filename, status='replace', action='write')
+    write(u, '(a)') 'Reference values:'
+    write(u, '(a7,es16.8)') 'Mass = ', r%mass_ref
+    write(u, '(a11,es16.8)') 'Diameter = ', r%diam_ref
+    write(u, '(a16,es16.8)') 'Cross-section = ', r%sigma_ref
+    write(u, '(a10,es16.8)') 'Density = ', r%dens_ref
+    write(u, '(a14,es16.8)') 'Temperature = ', r%temp_ref
+    write(u, '(a8,es16.8)') 'Speed = ', r%c_ref
+    write(u, '(a11,es16.8)') 'Pressure = ', press
+    write(u, '(a12,es16.8)') 'Heat Flux = ', hf
+    write(u, '(a9,es16.8)') 'Energy = ', e
+    write(u, '(a22,es16.8)') 'Flow Knudsen number = ', r%cfg%kn
+    write(u, '(a30,es16.8)') 'Characteristic length scale = ', r%l_scale
+    write(u, '(a13,es16.8)') 'Time scale = ', r%t_scale
+    close(u)
+  end subroutine write_reference_values
+
+end module DVMDeck

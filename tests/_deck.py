@@ -1,0 +1,70 @@
+"""Runs the Fortran deck driver (2d_dvm_microflows_amd/bin/dvm_gpu) for the
+tests and turns its --check output into a SolverConfig."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "2d_dvm_microflows_amd", "bin", "dvm_gpu")
+DECKS = os.path.join(ROOT, "tests", "golden", "decks")
+
+
+def deck_text(name="cavity10_ar.inp", **changes):
+    """The fixture deck with `key = value` lines replaced or appended."""
+    lines = open(os.path.join(DECKS, name)).read().splitlines()
+    done = set()
+    for i, ln in enumerate(lines):
+        key = ln.split("=", 1)[0].strip() if "=" in ln and not ln.lstrip().startswith("#") else None
+        if key in changes:
+            lines[i] = f"{key} = {changes[key]}"
+            done.add(key)
+    lines += [f"{k} = {v}" for k, v in changes.items() if k not in done]
+    return "\n".join(lines) + "\n"
+
+
+def run_driver(workdir, deck, *args, check=True):
+    """Writes the deck into workdir and runs the driver there."""
+    path = os.path.join(workdir, "run.inp")
+    with open(path, "w") as fh:
+        fh.write(deck)
+    return subprocess.run([DRIVER, path, *args], cwd=workdir, capture_output=True, text=True, check=check)
+
+
+def parse_check(stdout):
+    out = {}
+    for ln in stdout.splitlines():
+        f = ln.split()
+        if not f:
+            continue
+        if f[0].startswith("wall") and len(f) == 6:
+            out[f[0]] = [float(x) for x in f[1:]]
+        elif len(f) == 2:
+            try:
+                out[f[0]] = int(f[1])
+            except ValueError:
+                try:
+                    out[f[0]] = float(f[1])
+                except ValueError:
+                    out[f[0]] = f[1]
+    return out
+
+
+def solver_config(pkg, c):
+    """SolverConfig with the driver's derived fields (uniform axes)."""
+    ax = [pkg.uniform_axis(c[f"nv{a}"], c[f"v{a}0"], c[f"v{a}1"]) for a in "xyz"]
+    wall = np.array([c[f"wall{k}"] for k in range(4)])
+    return pkg.SolverConfig(dim=c["dim"], nx=c["nx"], ny=c["ny"], dx=c["dx"], dy=c["dy"], vx=ax[0], vy=ax[1],
+                            vz=ax[2], dt=c["dt"], kn=c["kn"], mass=c["mass"], diam=c["diam"], omega=c["omega"],
+                            alpha=c["alpha"], viscosity=c["viscosity"], t_viscosity=c["t_viscosity"],
+                            csection_coeff=c["csection_coeff"], coll_solver=c["coll_solver"],
+                            cdf_tolerance=c["cdf_tolerance"], num_repl=c["num_repl"], coln_rms=c["coln_rms"],
+                            sample_mode=c["sample_mode"], samples_per_cell=c["samples_per_cell"], seed=c["seed"],
+                            precision=c["precision"], enable_convection=c["enable_convection"],
+                            bc_left=c["bc_left"], bc_right=c["bc_right"], acc_coeff=c["acc_coeff"], wall=wall,
+                            conv_order=c["conv_order"])
+
+
+def have_driver():
+    return os.path.exists(DRIVER) and shutil.which("true") is not None
