@@ -19,6 +19,7 @@ DVM_PREC_FP64, DVM_PREC_FP32 = 0, 1
 DVM_SAMPLES_REFERENCE, DVM_SAMPLES_EXPLICIT = 0, 1
 DVM_CONV_UPWIND1, DVM_CONV_MINMOD2 = 1, 2
 DVM_NPROP = 8
+DVM_NEXT = 10  # qx qy qz txx tyy tzz txy txz tyz entropy
 P_DENS, P_U, P_V, P_W, P_TRTEMP, P_TEMP, P_PRESSURE, P_AVGSPEED = range(8)
 W_DENS, W_U, W_V, W_W, W_TEMP = range(5)
 TRACE_INTS, TRACE_DOUBLES = 32, 16
@@ -62,6 +63,7 @@ def declare(lib, prefix):
         "upload_f": (C.c_int, [P, I32, pd]),
         "download_f": (C.c_int, [P, I32, pd]),
         "moments": (C.c_int, [P, pd]),
+        "moments_ext": (C.c_int, [P, pd]),
         "collide": (C.c_int, [P, I32, pi64]),
         "convect": (C.c_int, [P, I32, I32]),
         "step": (C.c_int, [P, I32]),

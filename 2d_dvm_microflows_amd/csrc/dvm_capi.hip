@@ -692,6 +692,23 @@ extern "C" int dvm_moments(dvm_ctx* c, double* props_out) {
   return DVM_OK;
 }
 
+extern "C" int dvm_moments_ext(dvm_ctx* c, double* ext) {
+  if (!c || !ext) return fail(c, DVM_E_ARG, "null argument");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "moments before upload");
+  if (int rc = run_phases(c, 0, 6, 6)) return rc;
+  double* d_ext = nullptr;
+  if (int rc = dalloc(c, &d_ext, (size_t)c->ncell * DVM_NEXT)) return rc;
+  hipLaunchKernelGGL(k_moments_ext, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass,
+                     d_ext);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ext, d_ext, (size_t)c->ncell * DVM_NEXT * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_ext);
+  if (e != hipSuccess) return fail(c, DVM_E_HIP, "dvm_moments_ext: %s", hipGetErrorString(e));
+  return DVM_OK;
+}
+
 extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "collide needs f and its moments");

@@ -56,6 +56,8 @@ struct ConvParams {
 };
 
 __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass);
+__global__ void k_moments_ext(Grid g, const double* __restrict__ f, const double* __restrict__ props, double mass,
+                              double* __restrict__ ext);
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
                       double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
                       double* __restrict__ eqpar,

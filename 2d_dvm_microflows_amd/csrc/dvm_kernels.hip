@@ -125,6 +125,52 @@ __device__ __forceinline__ void moments_cell(const Grid& g, const double* __rest
   }
 }
 
+// Output moments of one cell, from f and its moments p (density, velocity,
+// pressure): heat flux q_i = m/2 sum (c_i - u_i)|c - u|^2 f
+// (physical_props.f90:1914-1986), viscous stress
+// tau_ij = -m sum (c_i - u_i)(c_j - u_j) f, plus the pressure on the diagonal
+// (:1988-2065), and entropy -sum_{f != 0} |f| log |f| (:1871-1912, C = 1).
+// ext = {qx, qy, qz, txx, tyy, tzz, txy, txz, tyz, entropy}.
+__global__ void __launch_bounds__(kMomBS) k_moments_ext(Grid g, const double* __restrict__ f,
+                                                        const double* __restrict__ props, double mass,
+                                                        double* __restrict__ ext) {
+  __shared__ double red[kMomBS / 64];
+  const int cell = blockIdx.x;
+  const double* fc = f + (size_t)cell * g.N;
+  const double* p = props + (size_t)cell * DVM_NPROP;
+  const double dens = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W];
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(g, l, i, j, k);
+    const double cx = g.ax[0][i] - u, cy = g.ax[1][j] - v, cz = g.ax[2][k] - w;
+    const double fl = fc[l];
+    const double csq = cx * cx + cy * cy + cz * cz;
+    acc[0] += cx * csq * fl;
+    acc[1] += cy * csq * fl;
+    acc[2] += cz * csq * fl;
+    acc[3] -= cx * cx * fl;
+    acc[4] -= cy * cy * fl;
+    acc[5] -= cz * cz * fl;
+    acc[6] -= cx * cy * fl;
+    acc[7] -= cx * cz * fl;
+    acc[8] -= cy * cz * fl;
+    if (fl != 0.0) acc[9] -= fabs(fl) * log(fabs(fl));
+  }
+  double* out = ext + (size_t)cell * DVM_NEXT;
+#pragma unroll 1
+  for (int q = 0; q < 10; ++q) {
+    const double t = block_sum<kMomBS>(acc[q], red);
+    if (threadIdx.x == 0) {
+      double o = t;
+      if (q < 3) o = (dens == 0.0) ? 0.0 : t * (0.5 * mass);
+      else if (q < 6) o = p[DVM_P_PRESSURE] + mass * t;
+      else if (q < 9) o = mass * t;
+      out[q] = o;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
                                                     double* __restrict__ props, double mass) {
   __shared__ double red[6][kMomBS / 64];

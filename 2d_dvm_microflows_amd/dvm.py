@@ -118,6 +118,14 @@ class Solver:
         self._check(self._fn("moments")(self.h, _dptr(out)))
         return out
 
+    def output_moments(self):
+        """Heat flux, viscous stress and entropy of every cell, [ncell, DVM_NEXT]
+        (compute_heat_flux / compute_viscous_stress / compute_entropy,
+        physical_props.f90:1871-2065)."""
+        out = np.empty((self.ncell, capi.DVM_NEXT), dtype=np.float64)
+        self._check(self._fn("moments_ext")(self.h, _dptr(out)))
+        return out
+
     def compute_collision_integral(self, ntime):
         """Collision phase of every cell (collision_integral.f90:163); returns
         the samples drawn per cell."""

@@ -46,9 +46,34 @@ module DVMDeck
      integer :: out_x = 1, out_y = 1                   ! screen_output_loc
      real(c_double) :: c_ref, sigma_ref, l_scale, t_scale, mass_ref, diam_ref, dens_ref, temp_ref
      character(len=16) :: species = ''
+     ! macroscopic-properties output (visualization.f90:270-340, 615-660, 1591-1716)
+     logical :: props_vis = .false.
+     character(len=128) :: props_file = 'macro_properties'
+     integer :: props_freq = 1
+     logical :: to_output(27) = .false.
+     integer :: n_stress = 0, n_moment = 0
+     integer :: stress_dirs(6) = 0, moment_nums(16) = 0
+     real(c_double) :: gamma = 5.0d0 / 3.0d0, mol_mass = 1.0d0
   end type run_t
 
+  ! deck switches of the 27 output properties, in the order of
+  ! visualization.f90:276-311
+  character(len=20), parameter :: prop_keys(27) = [character(len=20) :: &
+       'write_dens', 'write_x_vel', 'write_y_vel', 'write_z_vel', 'write_speed', 'write_mach', &
+       'write_tr_temp', 'write_rot_temp', 'write_vib_temp', 'write_tot_temp', 'write_temp_x', 'write_temp_y', &
+       'write_temp_z', 'write_tr_energy', 'write_rot_energy', 'write_vib_energy', 'write_tot_energy', &
+       'write_entropy', 'write_heat_flux_x', 'write_heat_flux_y', 'write_heat_flux_z', 'write_rot_dof', &
+       'write_vib_dof', 'write_rot_temp_relax', 'write_vib_temp_relax', 'write_kin_temp_relax', 'write_pressure']
+  ! property names of visualization.f90:314-340
+  character(len=24), parameter, public :: prop_names(27) = [character(len=24) :: &
+       '"Density"', '"X-Velocity"', '"Y-Velocity"', '"Z-Velocity"', '"Speed"', '"Mach Number"', &
+       '"Trans. Temp."', '"Rot. Temp."', '"Vib. Temp."', '"Total Temp."', '"Temp. X"', '"Temp. Y"', &
+       '"Temp. Z"', '"Trans. Energy"', '"Rot. Energy"', '"Vib. Energy"', '"Total Energy"', '"Entropy"', &
+       '"Heat Flux X"', '"Heat Flux Y"', '"Heat Flux Z"', '"Rot DOF"', '"Vib DOF"', '"Rot. Temp Analytic"', &
+       '"Vib. Temp Analytic"', '"Trans. Temp Analytic"', '"Pressure"']
+
   public :: deck_read, deck_has, deck_real, deck_int, deck_str, run_from_deck, write_reference_values
+  public :: props_header, props_record
 
 contains
 
@@ -336,6 +361,28 @@ contains
        cfg%wall(4, k) = deck_real(d, 'z_vel_' // side(k), 0.0d0)
        cfg%wall(5, k) = deck_real(d, 'tr_temp_' // side(k), 1.0d0)
     end do
+    ! properties output
+    r%props_vis = deck_int(d, 'enable_properties_vis', 0) /= 0
+    r%props_file = deck_str(d, 'properties_vis_filename', 'macro_properties')
+    r%props_freq = max(1, deck_int(d, 'properties_vis_dump_freq', 1))
+    do k = 1, 27
+       r%to_output(k) = deck_int(d, prop_keys(k), 0) /= 0
+    end do
+    r%n_stress = min(6, max(0, deck_int(d, 'write_stress', 0)))
+    do k = 1, r%n_stress
+       r%stress_dirs(k) = deck_int(d, 'stress_directions', 11, k)
+    end do
+    r%n_moment = min(16, max(0, deck_int(d, 'write_moment', 0)))
+    do k = 1, r%n_moment
+       r%moment_nums(k) = deck_int(d, 'moment_numbers', 4, k)
+    end do
+    ! calculate_specific_heat_ratio / calculate_molecular_mass
+    ! (shock_conditions.f90:222-314) for one monatomic species (3 dof),
+    ! weighted by the right-wall density
+    opt = cfg%mass * deck_real(d, 'dens_RW', 1.0d0)
+    coeff = opt * (3.0d0 * 0.5d0 + 1.0d0) / opt
+    r%gamma = coeff / (coeff - 1.0d0)
+    r%mol_mass = opt / deck_real(d, 'dens_RW', 1.0d0)
     ! screen output
     r%verbosity = deck_int(d, 'verbosity', 2)
     r%display_freq = max(1, deck_int(d, 'display_freq', 1))
@@ -445,5 +492,102 @@ contains
     write(u, '(a13,es16.8)') 'Time scale = ', r%t_scale
     close(u)
   end subroutine write_reference_values
+
+  ! Header of <properties_vis_filename>.dat (visualization.f90:615-645).
+  subroutine props_header(r, end_time)
+    type(run_t), intent(in) :: r
+    integer, intent(in) :: end_time
+    integer :: u, i, print_times
+    character(len=24) :: name
+    open(newunit=u, file=trim(adjustl(r%props_file)) // '.dat', status='replace', action='write')
+    write(u, advance='yes', fmt='(a)') 'title = "Macroscopic Properties"'
+    write(u, advance='no', fmt='(a)') 'variables = "Time", "x-Location", "y-Location"'
+    do i = 1, 27
+       name = prop_names(i)
+       ! the names are blank-padded variables, as in the reference: a14 keeps
+       ! their leftmost 14 characters
+       if (r%to_output(i)) write(u, advance='no', fmt='(a3,a14)') ', ', name
+    end do
+    do i = 1, r%n_stress
+       write(u, advance='no', fmt='(a7,i2,a1)') ', "Tau ', r%stress_dirs(i), '"'
+    end do
+    do i = 1, r%n_moment
+       write(u, advance='no', fmt='(a10,i2,a1)') ', "Moment ', r%moment_nums(i), '"'
+    end do
+    write(u, advance='yes', fmt='(1x)')
+    if (r%cfg%nx == 1) then
+       if (mod(end_time, r%props_freq) == 0) then
+          print_times = end_time / r%props_freq + 1
+       else
+          print_times = end_time / r%props_freq + 2
+       end if
+       write(u, advance='yes', fmt='(a7,i4)') 'zone i=', print_times
+    end if
+    close(u)
+  end subroutine props_header
+
+  ! One record of the total-mixture file (visualization.f90:1591-1646 with the
+  ! values of get_data_to_output :2296-2633 for one monatomic species: no
+  ! internal energy, and the directional temperatures, DOF, analytic
+  ! relaxation temperatures and moments written as zero in that file).
+  ! props(DVM_NPROP, ncell), ext(DVM_NEXT, ncell).
+  subroutine props_record(r, time, props, ext)
+    type(run_t), intent(in) :: r
+    real(c_double), intent(in) :: time, props(:,:), ext(:,:)
+    integer :: u, nx, ny, c, i, prop
+    real(c_double) :: xl, yl, val, dx, dy
+    open(newunit=u, file=trim(adjustl(r%props_file)) // '.dat', status='old', position='append', action='write')
+    if (r%cfg%nx > 1 .or. r%cfg%ny > 1) write(u, advance='yes', fmt='(a7,i5)') 'zone i=', r%cfg%nx * r%cfg%ny
+    dx = r%cfg%dx
+    dy = r%cfg%dy
+    xl = 0.5 * dx
+    do nx = 1, r%cfg%nx
+       yl = 0.5 * dy
+       do ny = 1, r%cfg%ny
+          c = nx + r%cfg%nx * (ny - 1)
+          write(u, advance='no', fmt='(3e24.15)') time, xl, yl
+          do prop = 1, 27
+             if (.not. r%to_output(prop)) cycle
+             select case (prop)
+             case (1); val = props(1, c)
+             case (2); val = props(2, c)
+             case (3); val = props(3, c)
+             case (4); val = props(4, c)
+             case (5); val = sqrt(props(2, c) * props(2, c) + props(3, c) * props(3, c) + props(4, c) * props(4, c))
+             case (6); val = props(2, c) / sqrt(0.5d0 * r%gamma * props(5, c) / r%mol_mass)
+             case (7); val = props(5, c)
+             case (10); val = props(6, c)
+             case (14, 17); val = 0.75d0 * props(1, c) * props(5, c)
+             case (18); val = ext(10, c)
+             case (19); val = ext(1, c)
+             case (20); val = ext(2, c)
+             case (21); val = ext(3, c)
+             case (27); val = 0.5d0 * props(1, c) * props(5, c)
+             case default; val = 0.0d0
+             end select
+             write(u, advance='no', fmt='(e24.16)') val
+          end do
+          do i = 1, r%n_stress
+             select case (r%stress_dirs(i))
+             case (11); val = ext(4, c)
+             case (22); val = ext(5, c)
+             case (33); val = ext(6, c)
+             case (12); val = ext(7, c)
+             case (13); val = ext(8, c)
+             case (23); val = ext(9, c)
+             case default; val = 0.0d0
+             end select
+             write(u, advance='no', fmt='(e24.16)') val
+          end do
+          do i = 1, r%n_moment
+             write(u, advance='no', fmt='(e24.16)') 0.0d0
+          end do
+          write(u, advance='yes', fmt='(1x)')
+          yl = yl + dy
+       end do
+       xl = xl + dx
+    end do
+    close(u)
+  end subroutine props_record
 
 end module DVMDeck

@@ -24,7 +24,7 @@ program dvm_gpu
   logical :: ok, check
   integer :: i, nargs, steps, rc, n, ncell, u, c, loc
   integer(c_int32_t) :: ntime
-  real(c_double), allocatable, target :: f(:,:), props(:,:)
+  real(c_double), allocatable, target :: f(:,:), props(:,:), ext(:,:)
   real(c_double), target :: phase(DVM_NPHASE)
   integer(c_int64_t) :: calls
   real(c_double) :: t0, t1
@@ -77,6 +77,7 @@ program dvm_gpu
   n = size(r%f0, 1)
   ncell = size(r%f0, 2)
   if (len_trim(dump_init) > 0) call dump(dump_init, r%f0)
+  if (r%props_vis) call props_header(r, steps)
   if (check) then
      call print_config()
   else
@@ -88,13 +89,14 @@ contains
   subroutine solve()
     rc = dvm_ctx_create(r%cfg, ctx)
     if (rc /= DVM_OK) call fail(dvm_error_message(c_null_ptr))
-    allocate(f(n, ncell), props(DVM_NPROP, ncell))
+    allocate(f(n, ncell), props(DVM_NPROP, ncell), ext(DVM_NEXT, ncell))
     rc = dvm_upload_f(ctx, 0_c_int32_t, r%f0)
     if (rc == DVM_OK) rc = dvm_moments(ctx, props)
     if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
     rc = dvm_timing_reset(ctx)
     loc = r%out_x + r%cfg%nx * (r%out_y - 1)
     if (r%verbosity >= 2) call screen(0)
+    if (r%props_vis) call write_props(0)
     call cpu_time(t0)
     do ntime = 1, steps
        rc = dvm_step(ctx, ntime)
@@ -103,6 +105,9 @@ contains
           rc = dvm_moments(ctx, props)
           if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
           call screen(int(ntime))
+       end if
+       if (r%props_vis) then
+          if (mod(int(ntime), steps) == 0 .or. mod(int(ntime), r%props_freq) == 0) call write_props(int(ntime))
        end if
     end do
     rc = dvm_moments(ctx, props)
@@ -127,6 +132,15 @@ contains
     end if
     call dvm_ctx_destroy(ctx)
   end subroutine solve
+
+  ! Moments and output moments of the resident f, then one record.
+  subroutine write_props(nt)
+    integer, intent(in) :: nt
+    rc = dvm_moments_ext(ctx, ext)
+    if (rc == DVM_OK) rc = dvm_moments(ctx, props)
+    if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+    call props_record(r, dble(nt) * r%cfg%dt, props, ext)
+  end subroutine write_props
 
   subroutine fail(m)
     character(len=*), intent(in) :: m

@@ -17,7 +17,7 @@ module DVMHip
   integer(c_int32_t), parameter, public :: DVM_PREC_FP64 = 0, DVM_PREC_FP32 = 1
   integer(c_int32_t), parameter, public :: DVM_SAMPLES_REFERENCE = 0, DVM_SAMPLES_EXPLICIT = 1
   integer(c_int32_t), parameter, public :: DVM_CONV_UPWIND1 = 1, DVM_CONV_MINMOD2 = 2
-  integer, parameter, public :: DVM_NPROP = 8, DVM_NPHASE = 7
+  integer, parameter, public :: DVM_NPROP = 8, DVM_NPHASE = 7, DVM_NEXT = 10
   integer, parameter, public :: DVM_P_DENS = 1, DVM_P_U = 2, DVM_P_V = 3, DVM_P_W = 4, DVM_P_TRTEMP = 5, &
        DVM_P_TEMP = 6, DVM_P_PRESSURE = 7, DVM_P_AVGSPEED = 8   ! 1-based positions in a props row
 
@@ -47,7 +47,7 @@ module DVMHip
   end type dvm_config
 
   public :: dvm_ctx_create, dvm_ctx_destroy, dvm_last_error, dvm_upload_f, dvm_download_f, &
-       dvm_moments, dvm_collide, dvm_convect, dvm_step, dvm_timing_reset, dvm_timing_read
+       dvm_moments, dvm_moments_ext, dvm_collide, dvm_convect, dvm_step, dvm_timing_reset, dvm_timing_read
   public :: dvm_batch_create, dvm_batch_destroy, dvm_batch_last_error, dvm_batch_admit, dvm_batch_retire, &
        dvm_batch_step, dvm_batch_synchronize, dvm_batch_download_f, dvm_batch_moments, dvm_batch_l1, &
        dvm_batch_tune
@@ -84,6 +84,11 @@ module DVMHip
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: props(*)
      end function dvm_moments
+     integer(c_int) function dvm_moments_ext(ctx, ext) bind(c, name="dvm_moments_ext")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ext(*)
+     end function dvm_moments_ext
      integer(c_int) function dvm_collide(ctx, ntime, samples) bind(c, name="dvm_collide")
        import :: c_int, c_int32_t, c_int64_t, c_ptr
        type(c_ptr), value :: ctx

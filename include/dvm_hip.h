@@ -75,6 +75,19 @@ extern "C" {
 #define DVM_P_AVGSPEED    7   /* sum f |v| / n                  */
 #define DVM_NPROP         8
 
+/* ---- per-cell output moments returned by dvm_moments_ext --------------- */
+#define DVM_X_QX          0   /* heat flux x                    */
+#define DVM_X_QY          1
+#define DVM_X_QZ          2
+#define DVM_X_TXX         3   /* viscous stress (pressure on the diagonal) */
+#define DVM_X_TYY         4
+#define DVM_X_TZZ         5
+#define DVM_X_TXY         6
+#define DVM_X_TXZ         7
+#define DVM_X_TYZ         8
+#define DVM_X_ENTROPY     9   /* -sum |f| log |f|               */
+#define DVM_NEXT         10
+
 /* Wall record indices: wall[side][...] with side 0 left, 1 right, 2 bottom, 3 top */
 #define DVM_W_DENS 0
 #define DVM_W_U    1
@@ -150,6 +163,14 @@ int dvm_download_f(dvm_ctx* ctx, int32_t species, double* f);
    Replaces: calculate_physical_properties (dist_funcs/physical_props.f90:285)
    called per cell at solver/dvm_solver.f90:277,336. */
 int dvm_moments(dvm_ctx* ctx, double* props_out);
+
+/* Output moments of the resident f per cell, ext[ncell][DVM_NEXT]: heat flux
+   (compute_heat_flux, dist_funcs/physical_props.f90:1914-1986), viscous
+   stress (compute_viscous_stress, :1988-2065) and entropy (compute_entropy,
+   :1871-1912), from f and its moments (which this call refreshes). Used by
+   the writers of input_output/visualization.f90:1591-1716 (get_data_to_output
+   :2296-2633); not on the time-step path. */
+int dvm_moments_ext(dvm_ctx* ctx, double* ext);
 
 /* One collision phase over every cell at time step ntime (>= 1): CDF build with
    the cached equilibrium, the three sampled VR integrals, phi := delta_phi.

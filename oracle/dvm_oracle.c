@@ -470,6 +470,55 @@ static void moments1(const ogrid* g, const double* f, double mass, double* p) {
   p[DVM_P_AVGSPEED] = sp;
 }
 
+/* Output moments per cell in the reference's loop order: compute_heat_flux
+   (physical_props.f90:1914-1986), compute_viscous_stress (:1988-2065),
+   compute_entropy (:1871-1912, C = 1). ext = {qx qy qz txx tyy tzz txy txz
+   tyz entropy}; uses the moments of the current f. */
+int orc_moments_ext(orc_ctx* c, double* ext) {
+  int rc = orc_moments(c, NULL);
+  if (rc) return rc;
+  const ogrid* g = &c->g;
+  const int N = g->num_points;
+  const double mass = c->cfg.mass;
+  for (int cell = 0; cell < c->ncell; ++cell) {
+    const double* f = c->f + (size_t)cell * N;
+    const double* p = c->props + (size_t)cell * DVM_NPROP;
+    const double dens = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W];
+    double q[3] = {0, 0, 0}, t[6] = {0, 0, 0, 0, 0, 0}, ent = 0.0;
+    int l = 0;
+    for (int k = 0; k < g->n[2]; ++k) {
+      const double z = g->ax[2][k];
+      const double czsq = (z - w) * (z - w);
+      for (int j = 0; j < g->n[1]; ++j) {
+        const double y = g->ax[1][j];
+        const double cysq = (y - v) * (y - v);
+        for (int i = 0; i < g->n[0]; ++i, ++l) {
+          const double x = g->ax[0][i];
+          const double cxsq = (x - u) * (x - u);
+          const double csq = cxsq + cysq + czsq;
+          const double fl = f[l];
+          q[0] = q[0] + (x - u) * csq * fl;
+          q[1] = q[1] + (y - v) * csq * fl;
+          q[2] = q[2] + (z - w) * csq * fl;
+          t[0] = t[0] - (x - u) * (x - u) * fl;
+          t[1] = t[1] - (y - v) * (y - v) * fl;
+          t[2] = t[2] - (z - w) * (z - w) * fl;
+          t[3] = t[3] - (x - u) * (y - v) * fl;
+          t[4] = t[4] - (x - u) * (z - w) * fl;
+          t[5] = t[5] - (y - v) * (z - w) * fl;
+          if (fl != 0.0) ent = ent - fabs(fl) * log(1.0 * fabs(fl));
+        }
+      }
+    }
+    double* o = ext + (size_t)cell * DVM_NEXT;
+    for (int a = 0; a < 3; ++a) o[a] = (dens == 0.0) ? 0.0 : q[a] * (0.5 * mass);
+    for (int a = 0; a < 3; ++a) o[3 + a] = p[DVM_P_PRESSURE] + mass * t[a];
+    for (int a = 3; a < 6; ++a) o[3 + a] = mass * t[a];
+    o[9] = ent;
+  }
+  return DVM_OK;
+}
+
 int orc_moments(orc_ctx* c, double* props_out) {
   if (!c->f_valid) return set_err(c, DVM_E_STATE, "moments before upload");
   size_t N = (size_t)c->g.num_points;

@@ -26,6 +26,7 @@ void orc_ctx_destroy(orc_ctx* ctx);
 const char* orc_last_error(const orc_ctx* ctx);
 int  orc_upload_f(orc_ctx* ctx, int32_t species, const double* f);
 int  orc_download_f(orc_ctx* ctx, int32_t species, double* f);
+int  orc_moments_ext(orc_ctx* ctx, double* ext);
 int  orc_moments(orc_ctx* ctx, double* props_out);
 int  orc_collide(orc_ctx* ctx, int32_t ntime, int64_t* samples_out);
 int  orc_convect(orc_ctx* ctx, int32_t species, int32_t ntime);

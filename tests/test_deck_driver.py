@@ -78,3 +78,15 @@ def test_initial_distribution_is_the_deck_state(pkg, tmp_path, initc):
             state = (0.8, 0.3, 0.0, 0.0, 1.4)
         want = pkg.maxwellian(cfg, *state, renormalise=True)
         assert np.allclose(f0[cell], want, rtol=1e-13, atol=0), cell
+
+
+def test_properties_file_header_follows_the_reference_layout(tmp_path):
+    run_driver(str(tmp_path), deck_text(), "--check")
+    lines = (tmp_path / "macro_properties.dat").read_text().splitlines()
+    # (a3,a14) on ", " and a blank-padded name: " , " then the name's leftmost 14 characters;
+    # the closing (1x) only moves the position, so nothing follows the last name
+    cols = "".join(" , " + n.ljust(14)[:14] for n in ['"Density"', '"X-Velocity"', '"Y-Velocity"',
+                                                       '"Trans. Temp."', '"Heat Flux X"', '"Heat Flux Y"',
+                                                       '"Pressure"'])
+    assert lines == ['title = "Macroscopic Properties"',
+                     'variables = "Time", "x-Location", "y-Location"' + cols]

@@ -141,6 +141,27 @@ def test_time_steps_match_oracle(pkg, case):
         assert ok, (n, err, scale)
 
 
+@pytest.mark.parametrize("case", ["cavity_diffuse", "homogeneous", "shock_1d"])
+def test_output_moments_match_oracle(pkg, case):
+    """Heat flux, viscous stress and entropy after two steps (1e-12 norm-wise
+    per quantity)."""
+    cfg, f = STEP_CASES[case](pkg)
+    print(f"[output moments] {case}", flush=True)
+    gpu, orc, _, _ = _pair(pkg, cfg, f)
+    for n in (1, 2):
+        gpu.step(n)
+        orc.step(n)
+    eg, eo = gpu.output_moments(), orc.output_moments()
+    p = orc.calculate_physical_properties()
+    # physical scales: stresses against the pressure, heat fluxes against
+    # pressure x thermal speed (some components vanish by symmetry)
+    pmax = np.max(p[:, 6])
+    scale = np.max(np.abs(eo), axis=0)
+    scale[0:3] = pmax * np.sqrt(np.max(p[:, 4]))
+    scale[3:9] = pmax
+    assert np.all(np.max(np.abs(eg - eo), axis=0) <= 1e-12 * scale), np.max(np.abs(eg - eo), axis=0) / scale
+
+
 def test_fused_step_equals_phases(pkg):
     cfg, f = pkg.lid_driven_cavity(nx=5, ny=5, n_v=9, vmax=3.0, samples=400)
     print("[fused] cavity 5x5 9^3", flush=True)
