@@ -242,6 +242,7 @@ static void build_host_grid(const dvm_config* cfg, HostGrid& h) {
     for (int i = 1; i < n[d] - 1; ++i) b[i] = (a[i + 1] - a[i]) / 2.0 + (a[i] - a[i - 1]) / 2.0;
     h.g.n[d] = n[d];
     h.g.vmin[d] = a[0];
+    h.g.vmax[d] = a[n[d] - 1];
     h.g.factor[d] = (double)(n[d] - 1) / (a[n[d] - 1] - a[0]);
     zero_point(a, h.g.zero[d], h.g.pos[d], h.g.neg[d]);
   }

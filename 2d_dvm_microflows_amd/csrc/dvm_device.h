@@ -26,6 +26,7 @@ struct Grid {
   int N;           // n0*n1*n2
   int nxy;         // n0*n1
   double vmin[3];  // first coordinate of each axis
+  double vmax[3];  // last coordinate of each axis
   double factor[3];// (n-1)/(vmax-vmin): first guess of the nearest-node search
   int zero[3], neg[3], pos[3];  // find_zero_point results per axis
   double inv_n0, inv_nxy;       // 1/n0 and 1/(n0*n1): node index decoding without integer division
@@ -95,32 +96,43 @@ __device__ __forceinline__ int referenced_search(const double* __restrict__ cumu
 // node to val, ties to the lower index, clamped to the axis ends. The
 // reference's bisection can only stop at the unique index with
 // a[index] < val <= a[index+1] (then it keeps the closer end), so its result
-// does not depend on the first guess; a uniform grid's guess is within one
-// node of that bracket, so the bracket is found by a short walk and the
-// bisection only runs when the walk fails (non-uniform axes).
-__device__ __forceinline__ int nearest_node(const double* __restrict__ a, int mx, double val, int index) {
-  if (a[0] > val) return 0;
-  if (a[mx] < val) return mx;
+// does not depend on the first guess. A uniform grid's guess is within one
+// node of that bracket, so the bracket is settled by one comparison at the
+// guess; a walk and then the bisection run only when it is not (non-uniform
+// axes).
+__device__ __forceinline__ int nearest_node(const double* __restrict__ a, int mx, double val, int index, double lo,
+                                            double hi) {
+  // lo = a[0], hi = a[mx], held in registers
+  if (lo > val) return 0;
+  if (hi < val) return mx;
   index = index < 0 ? 0 : (index > mx ? mx : index);
-  if (fabs(val - a[index]) < kDoubleTol) return index;
-  if (index == mx) index = mx - 1;
-  double a0 = a[index], a1 = a[index + 1];
-#pragma unroll
-  for (int w = 0; w < 2; ++w) {
-    if (val > a1 && index + 1 < mx) { ++index; a0 = a1; a1 = a[index + 1]; }
-    else if (!(val > a0) && index > 0) { --index; a1 = a0; a0 = a[index]; }
-  }
+  const double ag = a[index];
+  if (fabs(val - ag) < kDoubleTol) return index;
+  // the bracket is [index, index+1] or [index-1, index] when the guess is
+  // within one node (uniform axes): one compare and two independent reads
+  int i = (val > ag) ? index : index - 1;
+  i = i < 0 ? 0 : (i > mx - 1 ? mx - 1 : i);
+  double a0 = a[i], a1 = a[i + 1];
   if (!(val > a0 && val <= a1)) {
-    int left = 0, right = mx;
-    for (int it = 0; it < kSearchCap; ++it) {
-      a0 = a[index]; a1 = a[index + 1];
-      if (val > a0 && val <= a1) break;
-      if (val > a0) left = index; else right = index;
-      index = left + (right - left) / 2;
+    // farther guesses (non-uniform axes): walk, then bisect
+    index = i;
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      if (val > a1 && index + 1 < mx) { ++index; a0 = a1; a1 = a[index + 1]; }
+      else if (!(val > a0) && index > 0) { --index; a1 = a0; a0 = a[index]; }
     }
+    if (!(val > a0 && val <= a1)) {
+      int left = 0, right = mx;
+      for (int it = 0; it < kSearchCap; ++it) {
+        a0 = a[index]; a1 = a[index + 1];
+        if (val > a0 && val <= a1) break;
+        if (val > a0) left = index; else right = index;
+        index = left + (right - left) / 2;
+      }
+    }
+    i = index;
   }
-  if (fabs(val - a0) > fabs(val - a1)) index = index + 1;
-  return index;
+  return (fabs(val - a0) > fabs(val - a1)) ? i + 1 : i;
 }
 
 __device__ __forceinline__ int first_guess(double factor, double val, double vmin, int mx) {
@@ -158,11 +170,11 @@ __device__ __forceinline__ void remap(const Grid& g, double u, double v, double 
   const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
   const int imax = g.n[0] - 1, jmax = g.n[1] - 1, kmax = g.n[2] - 1;
   int corner = 0;
-  int oi = nearest_node(X, imax, u, first_guess(g.factor[0], u, g.vmin[0], imax));
+  int oi = nearest_node(X, imax, u, first_guess(g.factor[0], u, g.vmin[0], imax), g.vmin[0], g.vmax[0]);
   if (oi >= imax) { oi = imax; ++corner; } else if (oi <= 0) { oi = 0; ++corner; }
-  int oj = nearest_node(Y, jmax, v, first_guess(g.factor[1], v, g.vmin[1], jmax));
+  int oj = nearest_node(Y, jmax, v, first_guess(g.factor[1], v, g.vmin[1], jmax), g.vmin[1], g.vmax[1]);
   if (oj >= jmax) { oj = jmax; ++corner; } else if (oj <= 0) { oj = 0; ++corner; }
-  int ok = nearest_node(Z, kmax, w, first_guess(g.factor[2], w, g.vmin[2], kmax));
+  int ok = nearest_node(Z, kmax, w, first_guess(g.factor[2], w, g.vmin[2], kmax), g.vmin[2], g.vmax[2]);
   if (ok >= kmax) { ok = kmax; ++corner; } else if (ok <= 0) { ok = 0; ++corner; }
   if (corner == 3) {
     oi -= isign1(oi - 1);
