@@ -283,16 +283,20 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     double coeff = 0.0;
     if (dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
     // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
+    // f_eq needs no loads: each lane evaluates its own V consecutive nodes,
+    // stepping (i, j, k) along the node order
+    const int nx = g.n[0], ny = g.n[1];
     double carry = 0.0, carry_c = 0.0, prev_last = 0.0;
     if (threadIdx.x == 0) ec[0] = 0.0;
     for (int base = 0; base < N; base += CH) {
+      const int l0 = base + (int)threadIdx.x * V;
+      int i, j, k;
+      node_ijk(g, l0 < N ? l0 : 0, i, j, k);
+      double pv[V];
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        const int m = q * kCdfBS + threadIdx.x, l = base + m;
         double val = 0.0;
-        if (l < N) {
-          int i, j, k;
-          node_ijk(g, l, i, j, k);
+        if (l0 + q < N) {
           const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
           const double beta3 = g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
           double phi_eq = 0.0;
@@ -305,12 +309,9 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
           }
           val = phi_eq * beta3;
         }
-        tile[tile_pos(m)] = val;
+        pv[q] = val;
+        if (++i == nx) { i = 0; if (++j == ny) { j = 0; ++k; } }
       }
-      __syncthreads();
-      double pv[V];
-#pragma unroll
-      for (int q = 0; q < V; ++q) pv[q] = tile[own + q];
       carry = carry + chunk_scan<kCdfBS, V>(pv, red, carry);
       lastp[threadIdx.x] = pv[V - 1];
       __syncthreads();
@@ -337,7 +338,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
         const int m = q * kCdfBS + threadIdx.x, l = base + m;
         if (l < N) ec[l + 1] = tile[tile_pos(m)];
       }
-      __syncthreads();
+      // the next chunk writes the tile only after its first scan's barriers
     }
     build_buckets<kCdfBS>(ec, N, tab2, rights);
     if (threadIdx.x == 0) {
@@ -347,11 +348,24 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   // |f - f_eq| prefix
   double carry = 0.0;
   if (threadIdx.x == 0) nc[0] = 0.0;
+  // f and f_eq of the next chunk are loaded while this one is scanned
+  double nf[V], ne[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    const int l = q * kCdfBS + threadIdx.x;
+    nf[q] = (l < N) ? fc[l] : 0.0;
+    ne[q] = (l < N) ? ev[l] : 0.0;
+  }
   for (int base = 0; base < N; base += CH) {
 #pragma unroll
     for (int q = 0; q < V; ++q) {
       const int m = q * kCdfBS + threadIdx.x, l = base + m;
-      tile[tile_pos(m)] = (l < N) ? fabs(fc[l] - ev[l]) : 0.0;
+      tile[tile_pos(m)] = (l < N) ? fabs(nf[q] - ne[q]) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int l = base + CH + q * kCdfBS + threadIdx.x;
+      if (l < N) { nf[q] = fc[l]; ne[q] = ev[l]; }
     }
     __syncthreads();
     double d[V];
