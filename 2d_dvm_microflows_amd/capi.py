@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libdvmhip.so")
 
 DVM_OK = 0
-DVM_COLL_NONE, DVM_COLL_VR = 0, 3
+DVM_COLL_NONE, DVM_COLL_FMC, DVM_COLL_VR = 0, 2, 3
 DVM_BC_WALL, DVM_BC_INOUT, DVM_BC_ZEROGRAD, DVM_BC_COUETTE = 1, 2, 3, 4
 DVM_CS_CLARKE, DVM_CS_BIRD, DVM_CS_VKI = 1, 2, 3
 DVM_DIM_0D, DVM_DIM_1D, DVM_DIM_2D = 0, 1, 2

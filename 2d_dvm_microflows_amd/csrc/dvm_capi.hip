@@ -131,8 +131,8 @@ static int validate(const dvm_config* cfg) {
       if (!(a[d][i] > a[d][i - 1])) return fail(nullptr, DVM_E_ARG, "velocity axis %d is not increasing", d);
   }
   if (cfg->num_repl < 1) return fail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
-  if (cfg->coll_solver != DVM_COLL_NONE && cfg->coll_solver != DVM_COLL_VR)
-    return fail(nullptr, DVM_E_ARG, "coll_solver must be 0 (none) or 3 (variance reduction)");
+  if (cfg->coll_solver != DVM_COLL_NONE && cfg->coll_solver != DVM_COLL_FMC && cfg->coll_solver != DVM_COLL_VR)
+    return fail(nullptr, DVM_E_ARG, "coll_solver must be 0 (none), 2 (full Monte Carlo) or 3 (variance reduction)");
   if (cfg->precision != DVM_PREC_FP64 && cfg->precision != DVM_PREC_FP32)
     return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
   if (cfg->sample_mode != DVM_SAMPLES_REFERENCE && cfg->sample_mode != DVM_SAMPLES_EXPLICIT)
@@ -291,6 +291,7 @@ static void species_params(const dvm_config* cfg, double b3min, CdfParams& cp, C
   cp.beta3 = 0.5 * (b3a + b3a);
   cp.sample_mode = cfg->sample_mode;
   cp.samples_per_cell = cfg->samples_per_cell;
+  cp.fmc = cfg->coll_solver == DVM_COLL_FMC;
   co.seed = cfg->seed;
   co.mass = m;
   co.vhs_exponent = 2.0 - 2.0 * omega_ab;
@@ -350,6 +351,9 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     return rc;
   HIP_TRY(c, hipMemset(c->err, 0, 4 * sizeof(int)));
   HIP_TRY(c, hipMemset(c->counts, 0, (size_t)c->ncell * 4 * sizeof(int)));
+  // the full Monte Carlo kernel samples with f_eq = 0: its partner signs are
+  // dsgn(f - 0) = dsgn(f)
+  HIP_TRY(c, hipMemset(c->eqv, 0, CN * sizeof(double)));
   // collision accumulator placement: the cell's slice in LDS when it fits
   const bool f32 = cfg->precision == DVM_PREC_FP32;
   if (const char* e = getenv("DVM_COLL_BS")) {
@@ -557,6 +561,11 @@ static void end_slot(dvm_ctx* c) {
 }
 
 // ---- phases
+// Collision phases run for the variance-reduction and full Monte Carlo solvers.
+static bool collides(const dvm_ctx* c) {
+  return c->cfg.coll_solver == DVM_COLL_VR || c->cfg.coll_solver == DVM_COLL_FMC;
+}
+
 static int moments_async(dvm_ctx* c) {
   hipLaunchKernelGGL(k_moments, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass);
   HIP_TRY(c, hipGetLastError());
@@ -565,7 +574,7 @@ static int moments_async(dvm_ctx* c) {
 }
 
 static int cdf_async(dvm_ctx* c) {
-  if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
+  if (!collides(c)) return DVM_OK;
   hipLaunchKernelGGL(k_cdf, dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f, c->props, c->eqv,
                      c->cdf, c->tabs, c->eqpar, c->counts, c->factors);
   HIP_TRY(c, hipGetLastError());
@@ -575,7 +584,7 @@ static int cdf_async(dvm_ctx* c) {
 // fd = f (FP64) or dacc = 0 (FP32) before a collision that spills from its
 // LDS window to global atomics.
 static int prep_async(dvm_ctx* c) {
-  if (c->cfg.coll_solver != DVM_COLL_VR || (c->lds_collide && c->split == 1)) return DVM_OK;
+  if (!collides(c) || (c->lds_collide && c->split == 1)) return DVM_OK;
   if (c->cfg.precision == DVM_PREC_FP32)
     hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->dacc, (long long)c->ncell * c->N, 0.0);
   else
@@ -592,7 +601,7 @@ static bool separate_conserve(const dvm_ctx* c) {
 }
 
 static int collide_async(dvm_ctx* c, int ntime) {
-  if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
+  if (!collides(c)) return DVM_OK;
   void* args[] = {&c->g,    &c->collp, &ntime,   &c->split, &c->wd,     &c->props,   &c->f,  &c->fd,
                   &c->dacc, &c->eqv,   &c->cdf,  &c->tabs,  &c->counts, &c->factors, &c->err};
   HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(c->coll_bs), args,
@@ -606,7 +615,7 @@ static int collide_async(dvm_ctx* c, int ntime) {
 // FP32 path beyond LDS: fd = f + dacc with the FP64 conservation correction,
 // then phi := delta_phi.
 static int conserve_async(dvm_ctx* c) {
-  if (c->cfg.coll_solver != DVM_COLL_VR || !separate_conserve(c)) return DVM_OK;
+  if (!collides(c) || !separate_conserve(c)) return DVM_OK;
   hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;
@@ -717,7 +726,7 @@ extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
   if (int rc = run_phases(c, ntime, 0, 3)) return rc;
   if (samples_out) {
     std::vector<int> h((size_t)c->ncell * 4, 0);
-    if (c->cfg.coll_solver == DVM_COLL_VR)
+    if (collides(c))
       HIP_TRY(c, hipMemcpyAsync(h.data(), c->counts, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int q = 0; q < c->ncell; ++q) samples_out[q] = h[(size_t)q * 4 + 3];
@@ -796,7 +805,7 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
   // The CDF rebuild below commits the equilibrium cache exactly as the next
   // collide would (same f, same moments), so the trace leaves later steps
   // unchanged.
-  if (c->cfg.coll_solver != DVM_COLL_VR) { *n_out = 0; return DVM_OK; }
+  if (!collides(c)) { *n_out = 0; return DVM_OK; }
   if (int rc = cdf_async(c)) return rc;
   int h[4];
   HIP_TRY(c, hipMemcpyAsync(h, c->counts + (size_t)cell * 4, sizeof h, hipMemcpyDeviceToHost, c->stream));

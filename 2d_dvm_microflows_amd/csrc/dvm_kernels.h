@@ -22,6 +22,7 @@ struct CdfParams {
   double rms_sq, beta3;          // reference sample-count rule
   int sample_mode;
   long long samples_per_cell;
+  int fmc;                       // full Monte Carlo kernel: |f| CDF, one integral
 };
 
 struct CollParams {

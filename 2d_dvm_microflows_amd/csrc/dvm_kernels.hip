@@ -273,8 +273,8 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     const double tol = cp.cdf_tolerance;
     // the equilibrium cache test: w is not compared; a negative u or v makes
     // its test always true
-    s_new = (fabs(dens - par[0]) > tol * dens || fabs(u - par[1]) > tol * u ||
-             fabs(v - par[2]) > tol * v || fabs(temp - par[4]) > tol * temp) ? 1 : 0;
+    s_new = (!cp.fmc && (fabs(dens - par[0]) > tol * dens || fabs(u - par[1]) > tol * u ||
+                         fabs(v - par[2]) > tol * v || fabs(temp - par[4]) > tol * temp)) ? 1 : 0;
   }
   __syncthreads();
   const bool new_eq = s_new != 0;
@@ -345,7 +345,9 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
     }
   }
-  // |f - f_eq| prefix
+  // |f - f_eq| prefix; |f| prefix for the full Monte Carlo kernel
+  // (full_mc.f90:110-180), which takes f_eq = 0
+  const bool fmc = cp.fmc != 0;
   double carry = 0.0;
   if (threadIdx.x == 0) nc[0] = 0.0;
   // f and f_eq of the next chunk are loaded while this one is scanned
@@ -354,7 +356,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   for (int q = 0; q < V; ++q) {
     const int l = q * kCdfBS + threadIdx.x;
     nf[q] = (l < N) ? fc[l] : 0.0;
-    ne[q] = (l < N) ? ev[l] : 0.0;
+    ne[q] = (l < N && !fmc) ? ev[l] : 0.0;
   }
   for (int base = 0; base < N; base += CH) {
 #pragma unroll
@@ -365,7 +367,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
 #pragma unroll
     for (int q = 0; q < V; ++q) {
       const int l = base + CH + q * kCdfBS + threadIdx.x;
-      if (l < N) { nf[q] = fc[l]; ne[q] = ev[l]; }
+      if (l < N) { nf[q] = fc[l]; ne[q] = fmc ? 0.0 : ev[l]; }
     }
     __syncthreads();
     double d[V];
@@ -383,6 +385,31 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     __syncthreads();
   }
   build_buckets<kCdfBS>(nc, N, tab2 + kNBucket, rights);
+  if (fmc) {
+    // full_monte_carlo_kernel (full_mc.f90:240-317): abs_dens = dens - 2 neg_dens
+    // with neg_dens = (dens - sum |f|) / 2; compute_num_coll_pairs with both
+    // density ratios abs_dens / dens; factor halved for like species. All
+    // samples are integral-3 samples (both partners from the |f| CDF).
+    if (threadIdx.x == 0) {
+      const double neg_dens = 0.5 * (dens - nc[N]);
+      const double abs_dens = dens - 2.0 * neg_dens;
+      double num;
+      if (cp.sample_mode == DVM_SAMPLES_EXPLICIT) {
+        num = (double)cp.samples_per_cell;
+      } else {
+        const double ratio = abs_dens / dens;
+        const double ct = pow(0.5 * (p[DVM_P_TEMP] + p[DVM_P_TEMP]), 2.0 / 3.0);
+        num = round((0.5 * cp.dt * ct * ratio * ratio) / (cp.kn * cp.rms_sq * cp.beta3));
+      }
+      if (!(num > 0.0) || !(abs_dens > 0.0)) num = 0.0;
+      if (num > 2147483647.0) num = 2147483647.0;
+      const int n = (int)num;
+      cnt4[0] = 0; cnt4[1] = 0; cnt4[2] = n; cnt4[3] = n;
+      fac3[0] = 0.0; fac3[1] = 0.0;
+      fac3[2] = (n > 0) ? 0.5 * ((cp.dt / cp.kn) * cp.sigma * abs_dens * abs_dens / (double)n) : 0.0;
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
     const double neq_dens = nc[N], eq_dens = ec[N];
     const double h = 0.5 * neq_dens / dens;

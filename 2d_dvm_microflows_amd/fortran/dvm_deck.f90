@@ -11,7 +11,7 @@
 !
 ! Supported: num_species = 1 of He, Ne, Ar, Xe (species_data.f90:245-524),
 ! csection_method 1-4, csection_coeff 1-3, uniform velocity grids
-! (x/y/z_vel_grid_type = 1), iColl 0 or 3, iConv 0 or 1, InitC 1, 2, 3, 5
+! (x/y/z_vel_grid_type = 1), iColl 0, 2 or 3, iConv 0 or 1, InitC 1, 2, 3, 5
 ! with InitDF = 1 (Maxwellian), LWallBC / RWallBC 1-4. Anything else stops
 ! with a message, as the reference stops on an invalid deck value.
 !
@@ -326,9 +326,10 @@ contains
     icoll = deck_int(d, 'iColl', 3)
     select case (icoll)
     case (0); cfg%coll_solver = DVM_COLL_NONE
+    case (2); cfg%coll_solver = DVM_COLL_FMC
     case (3); cfg%coll_solver = DVM_COLL_VR
     case default
-       msg = 'iColl must be 0 (none) or 3 (variance reduction)'
+       msg = 'iColl must be 0 (none), 2 (full Monte Carlo) or 3 (variance reduction)'
        return
     end select
     cfg%cdf_tolerance = deck_real(d, 'cdf_tolerance', 0.01d0)

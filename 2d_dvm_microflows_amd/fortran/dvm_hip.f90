@@ -9,7 +9,7 @@ module DVMHip
   private
 
   integer(c_int), parameter, public :: DVM_OK = 0
-  integer(c_int32_t), parameter, public :: DVM_COLL_NONE = 0, DVM_COLL_VR = 3
+  integer(c_int32_t), parameter, public :: DVM_COLL_NONE = 0, DVM_COLL_FMC = 2, DVM_COLL_VR = 3
   integer(c_int32_t), parameter, public :: DVM_BC_WALL = 1, DVM_BC_INOUT = 2, DVM_BC_ZEROGRAD = 3, &
        DVM_BC_COUETTE = 4
   integer(c_int32_t), parameter, public :: DVM_CS_CLARKE = 1, DVM_CS_BIRD = 2, DVM_CS_VKI = 3

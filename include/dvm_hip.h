@@ -38,6 +38,7 @@ extern "C" {
 /* ---- enumerations (values equal the reference's deck integers) --------- */
 /* collision solver, collisions/collision_integral.f90:35-38 */
 #define DVM_COLL_NONE     0
+#define DVM_COLL_FMC      2   /* full Monte Carlo, collisions/full_mc.f90:110-641 */
 #define DVM_COLL_VR       3
 /* side boundary types, conditions/boundary_conditions.f90:56-59 */
 #define DVM_BC_WALL       1
@@ -112,7 +113,7 @@ typedef struct dvm_config {
   double  mass, diam, omega, alpha, viscosity, t_viscosity;
   int32_t csection_coeff;   /* DVM_CS_*                                         */
   /* collision integral */
-  int32_t coll_solver;      /* DVM_COLL_NONE or DVM_COLL_VR                     */
+  int32_t coll_solver;      /* DVM_COLL_NONE, DVM_COLL_FMC or DVM_COLL_VR       */
   double  cdf_tolerance;    /* set_cdf_tolerance                                */
   int32_t num_repl;         /* set_num_repl_locations (NRepl)                   */
   double  coln_rms;         /* ColnRMS                                          */

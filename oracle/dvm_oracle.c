@@ -534,6 +534,7 @@ typedef struct {
   double dens, neq_dens, eq_dens, temp;
   int num[3];
   double factor[3];
+  int fmc; /* full Monte Carlo: integral-3 samples from the |f| CDF, signs dsgn(f) */
 } cell_coll;
 
 /* variance_reduction.f90:234-496 compute_cdf_vr for the like-species pair
@@ -639,6 +640,63 @@ static void pair_counts(orc_ctx* c, int cell, cell_coll* cc) {
   }
 }
 
+/* full_mc.f90:110-180 compute_cdf_fmc: sequential prefix of |f| in node order
+   and its 1000-bucket table (the same table rule as compute_cdf_vr). Held in
+   the deviation slots (neqc, neql, neqr); the equilibrium slots are zeroed. */
+static void cdf_fmc(orc_ctx* c, int cell) {
+  int N = c->g.num_points;
+  const double* f = c->f + (size_t)cell * N;
+  double* eqc = c->eqc;
+  double* neqc = c->neqc;
+  eqc[0] = 0.0;
+  neqc[0] = 0.0;
+  for (int l = 1; l <= N; ++l) {
+    neqc[l] = neqc[l - 1] + fabs(f[l - 1]);
+    eqc[l] = 0.0;
+  }
+  orc_build_buckets(eqc, N, c->eql, c->eqr);
+  orc_build_buckets(neqc, N, c->neql, c->neqr);
+}
+
+/* full_mc.f90:240-317: abs_dens = dens - 2 neg_dens, neg_dens = (dens - sum|f|)/2;
+   compute_num_coll_pairs (collision_utilities.f90:106-140) with both density
+   ratios abs_dens/dens (or the explicit count); factor
+   (dt/Kn) sigma abs_dens^2 / num_colls, halved for like species (:319-321).
+   Every sample is an integral-3 sample. */
+static void fmc_counts(orc_ctx* c, int cell, cell_coll* cc) {
+  const dvm_config* cfg = &c->cfg;
+  const double* p = c->props + (size_t)cell * DVM_NPROP;
+  int N = c->g.num_points;
+  memset(cc, 0, sizeof *cc);
+  cc->fmc = 1;
+  cc->dens = p[DVM_P_DENS];
+  cc->temp = p[DVM_P_TEMP];
+  cc->neq_dens = c->neqc[N];
+  double neg_dens = 0.5 * (cc->dens - c->neqc[N]);
+  double abs_dens = cc->dens - 2.0 * neg_dens;
+  long long num;
+  if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT) {
+    num = cfg->samples_per_cell;
+  } else {
+    double ratio = abs_dens / cc->dens;
+    double rms_sq = cfg->coln_rms * cfg->coln_rms;
+    double b3a = c->g.beta3_min * pow(cfg->mass, 3.0 / 2.0);
+    double b3b = c->g.beta3_min * pow(cfg->mass, 3.0 / 2.0);
+    double beta3 = 0.5 * (b3a + b3b);
+    double ct = 0.5 * (cc->temp + cc->temp);
+    ct = pow(ct, 2.0 / 3.0);
+    double x = (0.5 * cfg->dt * ct * ratio * ratio) / (cfg->kn * rms_sq * beta3);
+    num = (x > 0.0) ? fnint(x) : 0; /* the reference's nint of NaN (empty cell) is left undefined */
+  }
+  if (!(abs_dens > 0.0) || num < 0) num = 0;
+  if (num > 0x7fffffffLL) num = 0x7fffffffLL;
+  cc->num[2] = (int)num;
+  if (num > 0) {
+    double factor = (cfg->dt / cfg->kn) * c->sigma * abs_dens * abs_dens / (double)num;
+    cc->factor[2] = 0.5 * factor;
+  }
+}
+
 /* One VR sample (variance_reduction.f90:686-816 / 838-961 / 983-1116 with
    vr_depletion_routine :1127-1237, replenish_collision
    replenishing_collisions.f90:134-354). Scatters into fd; when rec is
@@ -679,8 +737,9 @@ static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_c
   }
   double g_sigma = pow(gg, c->vhs_exponent);
   /* signs of deviation partners: cdf_neq%sign(gl) = dsgn(f - phi_eq) at that node */
-  double signA = dsgn(f[lA] - (eq_df[lA + 1] - eq_df[lA]));
-  double signB = dsgn(f[lB] - (eq_df[lB + 1] - eq_df[lB]));
+  /* full Monte Carlo (full_mc.f90:337-338, 434-437): dsgn of phi_old at the picks */
+  double signA = cc->fmc ? dsgn(f[lA]) : dsgn(f[lA] - (eq_df[lA + 1] - eq_df[lA]));
+  double signB = cc->fmc ? dsgn(f[lB]) : dsgn(f[lB] - (eq_df[lB + 1] - eq_df[lB]));
   double sumA = (t == 0) ? 1.0 : signA;
   double sumB = (t == 1) ? 1.0 : signB;
   double factor = cc->factor[t];
@@ -762,11 +821,23 @@ static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_c
   return 0;
 }
 
+/* The cell's CDFs and sample counts for the configured solver (the
+   equilibrium cache is committed when commit = 1). */
+static void cell_cdf_counts(orc_ctx* c, int cell, int commit, double* eq_df_out, cell_coll* cc) {
+  if (c->cfg.coll_solver == DVM_COLL_FMC) {
+    cdf_fmc(c, cell);
+    fmc_counts(c, cell, cc);
+  } else {
+    cdf_vr(c, cell, commit, eq_df_out);
+    pair_counts(c, cell, cc);
+    cc->fmc = 0;
+  }
+}
+
 static int collide_cell(orc_ctx* c, int cell, int ntime, int64_t* nsamp) {
   int N = c->g.num_points;
-  cdf_vr(c, cell, 1, NULL);
   cell_coll cc;
-  pair_counts(c, cell, &cc);
+  cell_cdf_counts(c, cell, 1, NULL, &cc);
   double* fd = c->fd + (size_t)cell * N;
   memcpy(fd, c->f + (size_t)cell * N, sizeof(double) * N);
   int64_t tot = 0;
@@ -794,7 +865,7 @@ int orc_collide_cells(orc_ctx* c, int32_t ntime, int32_t c0, int32_t c1) {
 
 int orc_collide(orc_ctx* c, int32_t ntime, int64_t* samples_out) {
   if (!c->props_valid) return set_err(c, DVM_E_STATE, "collide before moments");
-  if (c->cfg.coll_solver != DVM_COLL_VR) return DVM_OK;
+  if (c->cfg.coll_solver != DVM_COLL_VR && c->cfg.coll_solver != DVM_COLL_FMC) return DVM_OK;
   for (int cell = 0; cell < c->ncell; ++cell) {
     int rc = collide_cell(c, cell, ntime, samples_out ? samples_out + cell : NULL);
     if (rc) return rc;
@@ -812,9 +883,8 @@ int orc_collide_trace(orc_ctx* c, int32_t ntime, int32_t cell, int64_t max_recor
   double par[5];
   memcpy(save, c->eq_df + (size_t)cell * (N + 1), sizeof(double) * (N + 1));
   memcpy(par, c->eq_par + (size_t)cell * 5, sizeof par);
-  cdf_vr(c, cell, 1, NULL);
   cell_coll cc;
-  pair_counts(c, cell, &cc);
+  cell_cdf_counts(c, cell, 1, NULL, &cc);
   int64_t q = 0;
   for (int t = 0; t < 3; ++t)
     for (int s = 0; s < cc.num[t]; ++s, ++q)
@@ -831,10 +901,9 @@ int orc_cell_cdf(orc_ctx* c, int32_t cell, double* eq_cumul, double* neq_cumul, 
   if (!c->props_valid) return set_err(c, DVM_E_STATE, "cdf before moments");
   int N = c->g.num_points;
   double* tmp = (double*)malloc(sizeof(double) * (N + 1));
-  cdf_vr(c, cell, 0, tmp);
-  free(tmp);
   cell_coll cc;
-  pair_counts(c, cell, &cc);
+  cell_cdf_counts(c, cell, 0, tmp, &cc);
+  free(tmp);
   memcpy(eq_cumul, c->eqc, sizeof(double) * (N + 1));
   memcpy(neq_cumul, c->neqc, sizeof(double) * (N + 1));
   memcpy(eq_left, c->eql, sizeof c->eql);

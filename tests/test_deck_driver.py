@@ -54,7 +54,7 @@ def test_extension_keys(tmp_path):
     assert c["dim"] == 1 and c["nx"] == 12 and c["conv_order"] == 2 and (c["bc_left"], c["bc_right"]) == (2, 3)
 
 
-@pytest.mark.parametrize("key, value, text", [("iColl", 2, "iColl"), ("csection_method", 7, "csection_method"),
+@pytest.mark.parametrize("key, value, text", [("iColl", 1, "iColl"), ("csection_method", 7, "csection_method"),
                                               ("species_codes", "'H2'", "monatomic"), ("InitC", "'4'", "InitC")])
 def test_unsupported_decks_stop_with_a_message(tmp_path, key, value, text):
     r = run_driver(str(tmp_path), deck_text(**{key: value}), "--check", check=False)
@@ -90,3 +90,8 @@ def test_properties_file_header_follows_the_reference_layout(tmp_path):
                                                        '"Pressure"'])
     assert lines == ['title = "Macroscopic Properties"',
                      'variables = "Time", "x-Location", "y-Location"' + cols]
+
+
+def test_full_monte_carlo_deck(tmp_path):
+    c = parse_check(run_driver(str(tmp_path), deck_text(iColl=2), "--check").stdout)
+    assert c["coll_solver"] == 2
