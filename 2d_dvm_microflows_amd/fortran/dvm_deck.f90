@@ -54,6 +54,16 @@ module DVMDeck
      integer :: n_stress = 0, n_moment = 0
      integer :: stress_dirs(6) = 0, moment_nums(16) = 0
      real(c_double) :: gamma = 5.0d0 / 3.0d0, mol_mass = 1.0d0
+     ! distribution-function output (visualization.f90:497-560, 920-1215)
+     logical :: df_vis = .false.
+     character(len=128) :: df_file = 'dist_func'
+     integer :: df_freq = 1, df_dim = 3, n_loc = 0
+     integer :: loc_x(64) = 1, loc_y(64) = 1
+     integer :: i_zero = 0, j_zero = 0, k_zero = 0
+     ! restart (restart.f90:66-80, read_input_2D.f90:745-760)
+     logical :: read_rst = .false., write_rst = .false.
+     integer :: write_freq = 0
+     character(len=128) :: read_rst_file = 'restart.rst', write_rst_file = 'restart.rst'
   end type run_t
 
   ! deck switches of the 27 output properties, in the order of
@@ -73,7 +83,7 @@ module DVMDeck
        '"Vib. Temp Analytic"', '"Trans. Temp Analytic"', '"Pressure"']
 
   public :: deck_read, deck_has, deck_real, deck_int, deck_str, run_from_deck, write_reference_values
-  public :: props_header, props_record
+  public :: props_header, props_record, write_restart, read_restart, restart_due, df_headers, df_records
 
 contains
 
@@ -384,6 +394,62 @@ contains
     coeff = opt * (3.0d0 * 0.5d0 + 1.0d0) / opt
     r%gamma = coeff / (coeff - 1.0d0)
     r%mol_mass = opt / deck_real(d, 'dens_RW', 1.0d0)
+    ! distribution-function output of a single monatomic species (no rotational
+    ! or vibrational levels to write)
+    r%df_vis = deck_int(d, 'enable_vel_df_vis', 0) /= 0
+    if (deck_int(d, 'enable_rot_df_vis', 0) /= 0 .or. deck_int(d, 'enable_vib_df_vis', 0) /= 0) then
+       msg = 'enable_rot_df_vis / enable_vib_df_vis need internal energy levels (monatomic species only)'
+       return
+    end if
+    if (r%df_vis) then
+       r%df_file = deck_str(d, 'df_vis_filename', 'dist_func')
+       r%df_freq = max(1, deck_int(d, 'df_vis_dump_freq', 1))
+       r%df_dim = deck_int(d, 'df_output_dimension', 3)
+       r%n_loc = deck_int(d, 'num_output_locations', 1)
+       if (r%df_dim < 1 .or. r%df_dim > 3) then
+          msg = 'df_output_dimension must be 1, 2 or 3'
+          return
+       end if
+       if (r%n_loc < 0 .or. r%n_loc > size(r%loc_x)) then
+          msg = 'num_output_locations must be 0..64'
+          return
+       end if
+       ! locations outside the physical grid (out-of-bounds cells in the
+       ! reference) are skipped with a warning
+       mid = 0
+       do k = 1, r%n_loc
+          i = deck_int(d, 'x_output_locations', 1, k)
+          j = deck_int(d, 'y_output_locations', 1, k)
+          if (i < 1 .or. i > nx .or. j < 1 .or. j > ny) then
+             write(*, '(a,i0,a,i0,a)') 'Warning: distribution-function output location (', i, ',', j, &
+                  ') is outside the physical grid and is skipped'
+             cycle
+          end if
+          mid = mid + 1
+          r%loc_x(mid) = i
+          r%loc_y(mid) = j
+       end do
+       r%n_loc = mid
+       ! the planes of the 1-D and 2-D outputs: first x (y) node with
+       ! |x| < double_tol (visualization.f90:998-1012) and the z zero point
+       ! (find_zero_point, velocity_grid.f90:653-689)
+       r%i_zero = zero_node(r%vx)
+       r%j_zero = zero_node(r%vy)
+       r%k_zero = 0
+       do k = 1, size(r%vz)
+          if (r%vz(k) <= 0.0d0) r%k_zero = k
+       end do
+       if (r%df_dim == 1 .and. r%j_zero == 0 .or. r%df_dim < 3 .and. r%k_zero == 0) then
+          msg = 'df_output_dimension 1 needs a y node at zero, 1 and 2 a z node at or below zero'
+          return
+       end if
+    end if
+    ! restart files
+    r%read_rst = deck_int(d, 'read_rst', 0) /= 0
+    r%write_rst = deck_int(d, 'write_rst', 0) /= 0
+    r%write_freq = deck_int(d, 'write_freq', 0)
+    r%read_rst_file = deck_str(d, 'read_restart_filename', 'restart.rst')
+    r%write_rst_file = deck_str(d, 'write_restart_filename', 'restart.rst')
     ! screen output
     r%verbosity = deck_int(d, 'verbosity', 2)
     r%display_freq = max(1, deck_int(d, 'display_freq', 1))
@@ -452,6 +518,19 @@ contains
     r%cfg = cfg
     ok = .true.
   end subroutine run_from_deck
+
+  ! First node with |a| < double_tol (1e-13), 0 if none.
+  integer function zero_node(a)
+    real(c_double), intent(in) :: a(:)
+    integer :: i
+    zero_node = 0
+    do i = 1, size(a)
+       if (abs(a(i)) < 1.0d-13) then
+          zero_node = i
+          return
+       end if
+    end do
+  end function zero_node
 
   ! velocity_grid.f90:486-540 compute_beta
   real(c_double) function beta(a, i)
@@ -590,5 +669,198 @@ contains
     end do
     close(u)
   end subroutine props_record
+
+  ! write_restart (restart.f90:92-145): unformatted sequential records,
+  ! ntime first, then the distribution of every cell (one record of
+  ! num_points values each, nodes in (i, j, k) order with i fastest). The
+  ! reference indexes phi(species, nx) (1-D); cells here run x fastest, then
+  ! y, which is the same order for ny = 1.
+  subroutine write_restart(filename, ntime, f)
+    character(len=*), intent(in) :: filename
+    integer, intent(in) :: ntime
+    real(c_double), intent(in) :: f(:,:)
+    integer :: u, c
+    open(newunit=u, file=trim(filename), status='replace', form='unformatted', action='write')
+    write(u) ntime
+    do c = 1, size(f, 2)
+       write(u) f(:, c)
+    end do
+    close(u)
+  end subroutine write_restart
+
+  ! read_restart (restart.f90:147-250) for one monatomic species: ntime, then
+  ! one record per cell into f (the caller's shape fixes the sizes).
+  subroutine read_restart(filename, ntime, f, ok, msg)
+    character(len=*), intent(in) :: filename
+    integer, intent(out) :: ntime
+    real(c_double), intent(inout) :: f(:,:)
+    logical, intent(out) :: ok
+    character(len=*), intent(out) :: msg
+    integer :: u, c, ios
+    ok = .false.
+    ntime = 0
+    open(newunit=u, file=trim(filename), status='old', form='unformatted', action='read', iostat=ios)
+    if (ios /= 0) then
+       msg = 'cannot open restart file ' // trim(filename)
+       return
+    end if
+    read(u, iostat=ios) ntime
+    do c = 1, size(f, 2)
+       if (ios /= 0) exit
+       read(u, iostat=ios) f(:, c)
+    end do
+    close(u)
+    if (ios /= 0) then
+       msg = 'restart file ' // trim(filename) // ' is shorter than the deck''s grid'
+       return
+    end if
+    ok = .true.
+  end subroutine read_restart
+
+  ! write_restart's schedule: every write_freq steps and at the last step
+  ! (write_freq = 0: the last step only).
+  logical function restart_due(r, ntime, last)
+    type(run_t), intent(in) :: r
+    integer, intent(in) :: ntime, last
+    restart_due = .false.
+    if (.not. r%write_rst) return
+    if (ntime == last) then
+       restart_due = .true.
+    else if (r%write_freq > 0) then
+       restart_due = mod(ntime, r%write_freq) == 0
+    end if
+  end function restart_due
+
+  ! File of output location q: <df_vis_filename>_<species>_<x>_<y>.dat
+  ! (visualization.f90:505-508, int2char = I31 left-adjusted).
+  function df_name(r, q) result(fn)
+    type(run_t), intent(in) :: r
+    integer, intent(in) :: q
+    character(len=256) :: fn
+    character(len=32) :: a, b
+    write(a, '(i0)') r%loc_x(q)
+    write(b, '(i0)') r%loc_y(q)
+    fn = trim(adjustl(r%df_file)) // '_1_' // trim(a) // '_' // trim(b) // '.dat'
+  end function df_name
+
+  ! write_headers, distribution-function part (visualization.f90:497-560).
+  subroutine df_headers(r)
+    type(run_t), intent(in) :: r
+    integer :: u, q
+    if (.not. r%df_vis) return
+    do q = 1, r%n_loc
+       open(newunit=u, file=trim(df_name(r, q)), status='replace', action='write')
+       write(u, advance='yes', fmt='(a)') 'title = "Distribution Function"'
+       select case (r%df_dim)
+       case (1)
+          write(u, advance='no', fmt='(a)') 'variables = "X-Velocity", "Beta3", "Time"'
+       case (2)
+          write(u, advance='no', fmt='(a)') 'variables = "X-Velocity", "Y-Velocity", "Beta3", "Time"'
+       case default
+          write(u, advance='no', fmt='(a)') 'variables = "X-Velocity", "Y-Velocity", "Z-Velocity", "Beta3", "Time"'
+       end select
+       write(u, advance='no', fmt='(a)') ', "Vel DistFunc", "Deviation"'
+       write(u, advance='yes', fmt='(1x)')
+       close(u)
+    end do
+  end subroutine df_headers
+
+  ! write_df (visualization.f90:920-1215) for one species: per output cell a
+  ! zone of f and f - f_eq on the x line (1), the z = 0 plane (2) or the whole
+  ! grid (3); f_eq is the Maxwellian of the cell's density, velocity and
+  ! mixture temperature with the reference's pi, times beta^3, evaluated at
+  ! (x, 0, 0) or (x, y, 0) for the line and plane. props: (DVM_NPROP, ncell).
+  subroutine df_records(r, time, f, props)
+    type(run_t), intent(in) :: r
+    real(c_double), intent(in) :: time, f(:,:), props(:,:)
+    integer :: u, q, c, i, j, k, nvx, nvy, nvz, l
+    real(c_double) :: mass, dens, uu, vv, ww, temp, coeff, x, y, z, b3, feq
+    if (.not. r%df_vis) return
+    nvx = size(r%vx)
+    nvy = size(r%vy)
+    nvz = size(r%vz)
+    mass = r%cfg%mass
+    do q = 1, r%n_loc
+       c = r%loc_x(q) + r%cfg%nx * (r%loc_y(q) - 1)
+       dens = props(DVM_P_DENS, c)
+       uu = props(DVM_P_U, c)
+       vv = props(DVM_P_V, c)
+       ww = props(DVM_P_W, c)
+       temp = props(DVM_P_TEMP, c)
+       if (dens > 0.0d0) then
+          coeff = dens * sqrt(mass * mass * mass) / sqrt(pi_ref * pi_ref * pi_ref * temp * temp * temp)
+       else
+          coeff = 0.0d0
+       end if
+       open(newunit=u, file=trim(df_name(r, q)), status='old', position='append', action='write')
+       select case (r%df_dim)
+       case (1)
+          write(u, advance='yes', fmt='(a7,i4)') 'zone i=', nvx
+          j = r%j_zero
+          k = r%k_zero
+          do i = 1, nvx
+             x = r%vx(i)
+             b3 = beta(r%vx, i) * beta(r%vy, j) * beta(r%vz, k)
+             feq = 0.0d0
+             if (temp > 0.0d0) feq = maxwell(x, 0.0d0, 0.0d0, coeff, mass, uu, vv, ww, temp)
+             feq = feq * b3
+             l = i + nvx * ((j - 1) + nvy * (k - 1))
+             write(u, advance='no', fmt='(3e24.16)') x, b3, time
+             write(u, advance='no', fmt='(2e24.16)') f(l, c), f(l, c) - feq
+             write(u, advance='yes', fmt='(1x)')
+          end do
+       case (2)
+          write(u, advance='yes', fmt='(a7,i3,a4,i3)') 'zone i=', nvx, ', j=', nvy
+          k = r%k_zero
+          do j = 1, nvy
+             y = r%vy(j)
+             do i = 1, nvx
+                x = r%vx(i)
+                b3 = beta(r%vx, i) * beta(r%vy, j) * beta(r%vz, k)
+                feq = 0.0d0
+                if (temp > 0.0d0) feq = maxwell(x, y, 0.0d0, coeff, mass, uu, vv, ww, temp)
+                feq = feq * b3
+                l = i + nvx * ((j - 1) + nvy * (k - 1))
+                write(u, advance='no', fmt='(4e24.16)') x, y, b3, time
+                write(u, advance='no', fmt='(2e24.16)') f(l, c), f(l, c) - feq
+                write(u, advance='yes', fmt='(1x)')
+             end do
+          end do
+       case default
+          write(u, advance='yes', fmt='(a7,i4,a9,i4,a9,i4)') 'zone i=', nvx, ', j=', nvy, ', k=', nvz
+          do k = 1, nvz
+             z = r%vz(k)
+             do j = 1, nvy
+                y = r%vy(j)
+                do i = 1, nvx
+                   x = r%vx(i)
+                   b3 = beta(r%vx, i) * beta(r%vy, j) * beta(r%vz, k)
+                   feq = 0.0d0
+                   if (temp > 0.0d0) feq = maxwell(x, y, z, coeff, mass, uu, vv, ww, temp)
+                   feq = feq * b3
+                   l = i + nvx * ((j - 1) + nvy * (k - 1))
+                   write(u, advance='no', fmt='(5e24.16)') x, y, z, b3, time
+                   write(u, advance='no', fmt='(2e24.16)') f(l, c), f(l, c) - feq
+                   write(u, advance='yes', fmt='(1x)')
+                end do
+             end do
+          end do
+       end select
+       close(u)
+    end do
+  end subroutine df_records
+
+  ! compute_maxwellian (dist_func.f90:317-344)
+  real(c_double) function maxwell(x, y, z, coeff, mass, u, v, w, temp)
+    real(c_double), intent(in) :: x, y, z, coeff, mass, u, v, w, temp
+    real(c_double) :: cx, cy, cz
+    cx = (x - u)
+    cx = cx * cx
+    cy = (y - v)
+    cy = cy * cy
+    cz = (z - w)
+    cz = cz * cz
+    maxwell = coeff * exp(-(cx + cy + cz) * mass / temp)
+  end function maxwell
 
 end module DVMDeck

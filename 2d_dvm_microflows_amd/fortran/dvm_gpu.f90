@@ -8,7 +8,12 @@
 !
 !   --check      parse the deck, write reference_values.txt and print the
 !                derived configuration ("key value" lines); no device work
-!   --steps N    run N steps instead of the deck's ntstep
+!   --steps N    last step N instead of the deck's ntstep
+!
+! With read_rst = 1 the run starts from the deck's restart file (its step
+! count and distribution) and steps ntime_init+1 .. N, as the solve() loop
+! does (dvm_solver.f90:265-299); with write_rst = 1 it writes the restart
+! file every write_freq steps and at step N (restart.f90:92-145).
 !   --dump-init  raw FP64 initial distribution [cell][node]
 !   --dump       raw FP64 distribution after the last step
 !   --props      final moments, one line per cell (density u v w T_tr T p <|v|>)
@@ -22,7 +27,7 @@ program dvm_gpu
   type(c_ptr) :: ctx
   character(len=512) :: deck_file, arg, dump_init, dump_final, props_file, msg
   logical :: ok, check
-  integer :: i, nargs, steps, rc, n, ncell, u, c, loc
+  integer :: i, nargs, steps, rc, n, ncell, u, c, loc, ntime_init
   integer(c_int32_t) :: ntime
   real(c_double), allocatable, target :: f(:,:), props(:,:), ext(:,:)
   real(c_double), target :: phase(DVM_NPHASE)
@@ -76,8 +81,14 @@ program dvm_gpu
   call write_reference_values(r, 'reference_values.txt')
   n = size(r%f0, 1)
   ncell = size(r%f0, 2)
+  ntime_init = 0
+  if (r%read_rst) then
+     call read_restart(r%read_rst_file, ntime_init, r%f0, ok, msg)
+     if (.not. ok) call fail(msg)
+  end if
   if (len_trim(dump_init) > 0) call dump(dump_init, r%f0)
   if (r%props_vis) call props_header(r, steps)
+  call df_headers(r)
   if (check) then
      call print_config()
   else
@@ -95,12 +106,18 @@ contains
     if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
     rc = dvm_timing_reset(ctx)
     loc = r%out_x + r%cfg%nx * (r%out_y - 1)
-    if (r%verbosity >= 2) call screen(0)
-    if (r%props_vis) call write_props(0)
+    if (r%verbosity >= 2) call screen(ntime_init)
+    if (r%props_vis) call write_props(ntime_init)
+    if (r%df_vis) call write_df(ntime_init)
     call cpu_time(t0)
-    do ntime = 1, steps
+    do ntime = ntime_init + 1, steps
        rc = dvm_step(ctx, ntime)
        if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+       if (restart_due(r, int(ntime), steps)) then
+          rc = dvm_download_f(ctx, 0_c_int32_t, f)
+          if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+          call write_restart(r%write_rst_file, int(ntime), f)
+       end if
        if (r%verbosity >= 2 .and. mod(int(ntime), r%display_freq) == 0) then
           rc = dvm_moments(ctx, props)
           if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
@@ -109,13 +126,16 @@ contains
        if (r%props_vis) then
           if (mod(int(ntime), steps) == 0 .or. mod(int(ntime), r%props_freq) == 0) call write_props(int(ntime))
        end if
+       if (r%df_vis) then
+          if (mod(int(ntime), steps) == 0 .or. mod(int(ntime), r%df_freq) == 0) call write_df(int(ntime))
+       end if
     end do
     rc = dvm_moments(ctx, props)
     if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
     rc = dvm_timing_read(ctx, phase, calls)
     call cpu_time(t1)
     if (r%verbosity >= 1) then
-       write(*, '(a,i0,a,f10.3,a)') 'steps: ', steps, '   wall (host cpu_time): ', t1 - t0, ' s'
+       write(*, '(a,i0,a,f10.3,a)') 'steps: ', max(0, steps - ntime_init), '   wall (host cpu_time): ', t1 - t0, ' s'
        write(*, '(a,7f10.4)') 'device ms (cdf prep collide conserve walls sweep moments): ', phase
     end if
     if (len_trim(dump_final) > 0) then
@@ -141,6 +161,15 @@ contains
     if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
     call props_record(r, dble(nt) * r%cfg%dt, props, ext)
   end subroutine write_props
+
+  ! Distribution-function records of the resident f and its moments.
+  subroutine write_df(nt)
+    integer, intent(in) :: nt
+    rc = dvm_moments(ctx, props)
+    if (rc == DVM_OK) rc = dvm_download_f(ctx, 0_c_int32_t, f)
+    if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+    call df_records(r, dble(nt) * r%cfg%dt, f, props)
+  end subroutine write_df
 
   subroutine fail(m)
     character(len=*), intent(in) :: m
@@ -207,6 +236,7 @@ contains
        write(*, '(a,i0,5(1x,es25.16e3))') 'wall', k - 1, r%cfg%wall(:, k)
     end do
     write(*, '(a,1x,i0)') 'ntstep', r%ntstep
+    write(*, '(a,1x,i0)') 'ntime_init', ntime_init
     write(*, '(a,1x,a)') 'species', trim(r%species)
   end subroutine print_config
 

@@ -95,3 +95,73 @@ def test_properties_file_header_follows_the_reference_layout(tmp_path):
 def test_full_monte_carlo_deck(tmp_path):
     c = parse_check(run_driver(str(tmp_path), deck_text(iColl=2), "--check").stdout)
     assert c["coll_solver"] == 2
+
+
+def fortran_records(*payloads):
+    """Unformatted sequential file: each record framed by its 4-byte length."""
+    import struct
+    out = b""
+    for p in payloads:
+        out += struct.pack("<i", len(p)) + p + struct.pack("<i", len(p))
+    return out
+
+
+def read_fortran_records(path):
+    import struct
+    data, recs, i = open(path, "rb").read(), [], 0
+    while i < len(data):
+        (n,) = struct.unpack_from("<i", data, i)
+        recs.append(data[i + 4:i + 4 + n])
+        assert struct.unpack_from("<i", data, i + 4 + n)[0] == n
+        i += n + 8
+    return recs
+
+
+def test_restart_file_is_read_in_record_order(pkg, tmp_path):
+    """read_rst = 1: ntime from the first record, then one record of f per
+    cell (restart.f90:147-250 order); the run would start at ntime + 1."""
+    over = dict(nx_space="'3'", ny_space="'2'", read_rst=1, read_restart_filename="r.rst")
+    c = parse_check(run_driver(str(tmp_path), deck_text(**{k: v for k, v in over.items() if "rst" not in k}),
+                               "--check").stdout)
+    N, ncell = c["nvx"] * c["nvy"] * c["nvz"], 6
+    f = np.random.default_rng(2).random((ncell, N))
+    (tmp_path / "r.rst").write_bytes(fortran_records(np.int32(7).tobytes(), *(row.tobytes() for row in f)))
+    r = run_driver(str(tmp_path), deck_text(**over), "--check", "--dump-init", "f0.bin")
+    assert parse_check(r.stdout)["ntime_init"] == 7
+    assert np.array_equal(np.fromfile(tmp_path / "f0.bin").reshape(ncell, N), f)
+    # a file shorter than the deck's grid stops with a message
+    (tmp_path / "r.rst").write_bytes(fortran_records(np.int32(7).tobytes(), f[0].tobytes()))
+    r = run_driver(str(tmp_path), deck_text(**over), "--check", check=False)
+    assert r.returncode != 0 and "shorter" in r.stdout
+
+
+def test_distribution_function_headers(tmp_path):
+    """enable_vel_df_vis: one file per output location, named
+    <df_vis_filename>_<species>_<x>_<y>.dat, with the header of
+    visualization.f90:497-560 for the chosen dimension."""
+    over = dict(enable_vel_df_vis=1, df_output_dimension=2, num_output_locations=2, x_output_locations="'3 5'",
+                y_output_locations="'2 4'", df_vis_filename="dfout")
+    run_driver(str(tmp_path), deck_text(**over), "--check")
+    for name in ("dfout_1_3_2.dat", "dfout_1_5_4.dat"):
+        assert (tmp_path / name).read_text().splitlines() == [
+            'title = "Distribution Function"',
+            'variables = "X-Velocity", "Y-Velocity", "Beta3", "Time", "Vel DistFunc", "Deviation"']
+    over.update(df_output_dimension=3)
+    run_driver(str(tmp_path), deck_text(**over), "--check")
+    assert (tmp_path / "dfout_1_3_2.dat").read_text().splitlines()[1] == (
+        'variables = "X-Velocity", "Y-Velocity", "Z-Velocity", "Beta3", "Time", "Vel DistFunc", "Deviation"')
+
+
+@pytest.mark.parametrize("over, text", [
+    (dict(enable_vel_df_vis=1, df_output_dimension=1), "df_output_dimension 1"),      # even y grid: no y = 0 node
+    (dict(enable_rot_df_vis=1), "internal energy levels")])
+def test_distribution_function_output_errors(tmp_path, over, text):
+    r = run_driver(str(tmp_path), deck_text(**over), "--check", check=False)
+    assert r.returncode != 0 and text in r.stdout
+
+
+def test_distribution_function_location_outside_the_grid_is_skipped(tmp_path):
+    r = run_driver(str(tmp_path), deck_text(num_output_locations=2, x_output_locations="'11 2'",
+                                            y_output_locations="'3 3'"), "--check")
+    assert "(11,3) is outside the physical grid" in r.stdout
+    assert not (tmp_path / "dist_func_1_11_3.dat").exists() and (tmp_path / "dist_func_1_2_3.dat").exists()

@@ -84,3 +84,78 @@ def test_properties_file_matches_the_oracle(pkg, tmp_path):
         scale[[10, 11]] = pmax * cth
         scale[[13, 14]] = pmax
         assert np.all(np.max(np.abs(got - want), axis=0) <= 1e-12 * scale), (step, np.max(np.abs(got - want), axis=0) / scale)
+
+
+def test_restart_continues_the_run(pkg, tmp_path):
+    """write_rst = 1 writes the restart file at the last step (ntime, then one
+    record per cell equal to the resident f); a run that reads it and steps
+    on to 5 equals a straight 5-step run (1e-12 per cell: the order of the
+    floating-point atomics may differ between runs)."""
+    from tests.test_deck_driver import read_fortran_records
+    small = dict(nx_space="'4'", ny_space="'3'", dvm_samples=300)
+    print("[deck] restart: 3 steps + 2 steps vs 5 steps", flush=True)
+    run_driver(str(tmp_path), deck_text(write_rst=1, write_restart_filename="a.rst", **small), "--steps", "3",
+               "--dump", "f3.bin")
+    recs = read_fortran_records(tmp_path / "a.rst")
+    assert np.frombuffer(recs[0], dtype=np.int32)[0] == 3 and len(recs) == 1 + 12
+    f3 = np.fromfile(tmp_path / "f3.bin").reshape(12, -1)
+    assert np.array_equal(np.stack([np.frombuffer(r, dtype=np.float64) for r in recs[1:]]), f3)
+    r = run_driver(str(tmp_path), deck_text(read_rst=1, read_restart_filename="a.rst", **small), "--steps", "5",
+                   "--dump", "f5r.bin")
+    assert "steps: 2" in r.stdout
+    run_driver(str(tmp_path), deck_text(**small), "--steps", "5", "--dump", "f5.bin")
+    a = np.fromfile(tmp_path / "f5r.bin").reshape(12, -1)
+    b = np.fromfile(tmp_path / "f5.bin").reshape(12, -1)
+    err = np.max(np.abs(a - b), axis=1) / np.max(np.abs(b), axis=1)
+    assert np.all(err <= 1e-12), err.max()
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_distribution_function_records(pkg, tmp_path, dim):
+    """write_df records of a 2-step run at every step: zones at t = 0, dt,
+    2 dt; f equals the resident distribution (16 printed digits) and the
+    deviation is f - beta^3 f_M(n, u, T) with the reference's pi at the
+    cell's moments, taken at z = 0 on the plane output
+    (visualization.f90:1085-1135)."""
+    over = dict(enable_vel_df_vis=1, df_output_dimension=dim, num_output_locations=1, x_output_locations="'4'",
+                y_output_locations="'7'", df_vis_dump_freq=1, nx_space="'6'", ny_space="'8'", dvm_samples=300)
+    print(f"[deck] df output dim {dim}", flush=True)
+    run_driver(str(tmp_path), deck_text(**over), "--steps", "2", "--dump-init", "f0.bin", "--dump", "f2.bin")
+    text = (tmp_path / "dist_func_1_4_7.dat").read_text()          # --check below rewrites the header
+    cfg = solver_config(pkg, parse_check(run_driver(str(tmp_path), deck_text(**over), "--check").stdout))
+    zones, rows = [], []
+    for ln in text.splitlines()[2:]:
+        if ln.startswith("zone"):
+            zones.append(ln)
+            rows.append([])
+        else:
+            rows[-1].append([float(x) for x in ln.split()])
+    assert len(zones) == 3
+    cell, N, nv = 3 + cfg.nx * 6, cfg.num_points, cfg.nvx
+    v = pkg.node_velocities(cfg)
+    beta = [np.concatenate([[a[1] - a[0]], (a[2:] - a[1:-1]) / 2 + (a[1:-1] - a[:-2]) / 2, [a[-1] - a[-2]]])
+            for a in (cfg.vx, cfg.vy, cfg.vz)]
+    b3 = (beta[2][:, None, None] * beta[1][None, :, None] * beta[0][None, None, :]).reshape(-1)
+    kz = max(k for k in range(cfg.nvz) if cfg.vz[k] <= 0.0)
+    sel = np.arange(N) if dim == 3 else kz * nv * cfg.nvy + np.arange(nv * cfg.nvy)
+    orc = oracle_solver(cfg)
+    for step, name in ((0, "f0.bin"), (2, "f2.bin")):
+        rec = np.array(rows[step])
+        assert zones[step] == ("zone i=  18     , j=  18     , k=  18" if dim == 3 else "zone i= 18, j= 18")
+        assert rec.shape == (len(sel), dim + 4)
+        assert np.all(rec[:, dim + 1] == step * cfg.dt)
+        assert np.allclose(rec[:, dim], b3[sel], rtol=1e-15, atol=0)
+        fall = np.fromfile(tmp_path / name).reshape(-1, N)
+        f = fall[cell]
+        assert np.allclose(rec[:, dim + 2], f[sel], rtol=1e-15, atol=1e-300)
+        orc.upload_f(fall)
+        p = orc.calculate_physical_properties()[cell]
+        dens, u, w_, T = p[0], p[1:4], None, p[5]
+        coeff = dens * np.sqrt(cfg.mass ** 3) / np.sqrt(pkg.config.PI_REF ** 3 * T ** 3)
+        at = v[sel].copy()
+        if dim == 2:
+            at[:, 2] = 0.0
+        feq = coeff * np.exp(-((at - u) ** 2).sum(1) * cfg.mass / T) * b3[sel]
+        dev = rec[:, dim + 3]
+        assert np.max(np.abs(dev - (f[sel] - feq))) <= 1e-12 * np.max(np.abs(f[sel]))
+    orc.close()
