@@ -234,6 +234,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-shock", action="store_true")
+    ap.add_argument("--no-fmc", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default tiles (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=3)
@@ -292,6 +293,16 @@ def main():
         out["speedup_fp32_vs_fp64"] = w64 / wall
         out["fp32_moment_rel_error_vs_fp64"] = moment_error(runs["fp32"][2], runs["fp64"][2])
         out["fp32_moment_error_after_steps"] = args.warmup + args.steps
+    if not args.no_fmc:
+        # the same cavity with the full Monte Carlo kernel (iColl = 2): one
+        # integral of samples drawn from the |f| CDF
+        c = cfg.with_(coll_solver=pkg.capi.DVM_COLL_FMC,
+                      precision=pkg.capi.DVM_PREC_FP32 if head == "fp32" else pkg.capi.DVM_PREC_FP64)
+        wf, pf, _ = run_gpu(pkg, c, f, args.steps, args.warmup)
+        out["fmc_path"] = {"value": updates / wf, "ms_per_step": wf / args.steps * 1e3, "precision": head,
+                           "collision_samples_per_s": samples_per_step * args.steps / wf,
+                           "phase_ms": {k: round(v, 4) for k, v in pf.items()}}
+        print(f"[bench] fmc {head}: {wf / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_shock:
         out["shock_1d"] = run_shock(pkg, args.steps, args.warmup)
         print(f"[bench] shock 1-D: {out['shock_1d']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
