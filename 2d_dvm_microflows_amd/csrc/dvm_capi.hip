@@ -46,6 +46,9 @@ struct dvm_ctx {
   int coll_bs = 1024;            // lanes per collision workgroup (256, 512 or 1024)
   int pick_k = 1;                // samples per lane picked in lockstep (1, 2 or 4)
   int split = 1;                 // workgroups per cell in the collision kernel
+  int heat_every = 0;            // wall heat fluxes every heat_every steps (0: off)
+  int heat_ntime = -1;           // step of the values in d_heat
+  double* d_heat = nullptr;      // [4][max(nx,ny)] wall heat fluxes
   double* dacc = nullptr;        // [C][N] collision change of the FP32 path
   bool f_valid = false, props_valid = false;
   int trace_ready = -1;
@@ -155,7 +158,7 @@ extern "C" const char* dvm_last_error(const dvm_ctx* ctx) { return ctx ? ctx->ms
 
 extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   if (!c) return;
-  void* ptrs[] = {c->d_axes, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->eqpar, c->props,
+  void* ptrs[] = {c->d_heat, c->d_axes, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->eqpar, c->props,
                   c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -624,7 +627,7 @@ static int conserve_async(dvm_ctx* c) {
 
 static bool convects(const dvm_ctx* c) { return c->cfg.enable_convection && c->cfg.dim != DVM_DIM_0D; }
 
-static int walls_async(dvm_ctx* c) {
+static int walls_async(dvm_ctx* c, int ntime) {
   if (!convects(c)) return DVM_OK;
   ConvParams& p = c->convp;
   const int N = c->N;
@@ -646,6 +649,14 @@ static int walls_async(dvm_ctx* c) {
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->spec, ns, 0.0);
     hipLaunchKernelGGL(k_spec_ghost, dim3((N + 255) / 256, p.max_along, 4), dim3(256), 0, c->stream, c->g, p, c->f,
                        c->spec, c->err);
+  }
+  if (c->heat_every > 0 && ntime % c->heat_every == 0) {
+    WallVel wv;
+    for (int s = 0; s < 4; ++s)
+      for (int d = 0; d < 3; ++d) wv.u[s][d] = c->cfg.wall[s][DVM_W_U + d];
+    hipLaunchKernelGGL(k_wall_heat, dim3(p.max_along, 4), dim3(kMomBS), 0, c->stream, c->g, p, wv, c->cfg.mass,
+                       c->f, c->props, c->wallM, c->ratio, c->spec, c->d_heat);
+    c->heat_ntime = ntime;
   }
   HIP_TRY(c, hipGetLastError());
   return DVM_OK;
@@ -680,7 +691,7 @@ static int run_phases(dvm_ctx* c, int ntime, int first, int last) {
       case 1: rc = prep_async(c); break;
       case 2: rc = collide_async(c, ntime); break;
       case 3: rc = conserve_async(c); break;
-      case 4: rc = walls_async(c); break;
+      case 4: rc = walls_async(c, ntime); break;
       case 5: rc = sweep_async(c); break;
       case 6: rc = moments_async(c); break;
     }
@@ -793,6 +804,33 @@ extern "C" int dvm_remap(dvm_ctx* c, int64_t n, const double* uvw, double f_off,
   (void)hipFree(d_m);
   (void)hipFree(d_st);
   if (e != hipSuccess) return fail(c, DVM_E_HIP, "dvm_remap: %s", hipGetErrorString(e));
+  return DVM_OK;
+}
+
+extern "C" int dvm_set_wall_heat_flux_freq(dvm_ctx* c, int32_t every) {
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (every < 0) return fail(c, DVM_E_ARG, "every must be >= 0");
+  if (every > 0 && c->cfg.dim != DVM_DIM_2D) return fail(c, DVM_E_ARG, "wall heat fluxes need a 2-D grid");
+  if (every > 0 && !c->d_heat) {
+    const size_t n = (size_t)4 * c->convp.max_along;
+    if (int rc = dalloc(c, &c->d_heat, n)) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->d_heat, 0, n * sizeof(double), c->stream));
+  }
+  c->heat_every = every;
+  return DVM_OK;
+}
+
+extern "C" int dvm_wall_heat_flux(dvm_ctx* c, double* q, int32_t* ntime_out) {
+  if (!c || !q) return fail(c, DVM_E_ARG, "null argument");
+  const size_t n = (size_t)4 * (c->cfg.nx > c->cfg.ny ? c->cfg.nx : c->cfg.ny);
+  if (!c->d_heat || c->heat_ntime < 0) {
+    for (size_t i = 0; i < n; ++i) q[i] = 0.0;
+    if (ntime_out) *ntime_out = -1;
+    return DVM_OK;
+  }
+  HIP_TRY(c, hipMemcpyAsync(q, c->d_heat, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (ntime_out) *ntime_out = c->heat_ntime;
   return DVM_OK;
 }
 

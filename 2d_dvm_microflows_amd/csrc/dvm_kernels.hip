@@ -1072,6 +1072,63 @@ __global__ void __launch_bounds__(kMomBS) k_wall_ratio(Grid g, ConvParams p, con
   if (threadIdx.x == 0) ratio[side * p.max_along + pos] = s / p.flux_out[side];
 }
 
+// Heat flux into each wall cell's wall (boundary_conditions.f90:872-923 and
+// the right / bottom / top analogues at :1069-1115, :1262-1307, :1450-1495):
+// q = m/2 (sum over the incoming half of v_n f |c - u_cell|^2 + sum over the
+// outgoing half of v_n g |c - u_wall|^2), g the wall ghost the sweep uses
+// (diffuse part ratio * wallM, specular part spec), u_cell the cell's bulk
+// velocity from the moments the step started with. The zero-velocity plane
+// belongs to neither half. blockIdx.y = side.
+struct WallVel {
+  double u[4][3];
+};
+
+__global__ void __launch_bounds__(kMomBS) k_wall_heat(Grid g, ConvParams p, WallVel wv, double mass,
+                                                      const double* __restrict__ f, const double* __restrict__ props,
+                                                      const double* __restrict__ wallM,
+                                                      const double* __restrict__ ratio,
+                                                      const double* __restrict__ spec, double* __restrict__ q) {
+  __shared__ double red[kMomBS / 64];
+  const int side = blockIdx.y, pos = blockIdx.x;
+  const int along = (side < 2) ? p.ny : p.nx;
+  if (pos >= along || !p.wall_side[side]) return;
+  const int ix = (side == 0) ? 0 : (side == 1 ? p.nx - 1 : pos);
+  const int iy = (side == 2) ? 0 : (side == 3 ? p.ny - 1 : pos);
+  const size_t cell = (size_t)ix + (size_t)p.nx * iy;
+  const double* fc = f + cell * g.N;
+  const double* pc = props + cell * DVM_NPROP;
+  const double xb = pc[DVM_P_U], yb = pc[DVM_P_V], zb = pc[DVM_P_W];
+  const double uw = wv.u[side][0], vw = wv.u[side][1], ww = wv.u[side][2];
+  const double rt = (p.acc > 0.0) ? ratio[side * p.max_along + pos] : 0.0;
+  const double* wm = wallM + (size_t)side * g.N;
+  const double* sp = spec + ((size_t)side * p.max_along + pos) * g.N;
+  double hin = 0.0, hout = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(g, l, i, j, k);
+    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+    bool in, out;
+    if (side == 0) { in = i <= g.neg[0]; out = i >= g.pos[0]; }
+    else if (side == 1) { in = i >= g.pos[0]; out = i <= g.neg[0]; }
+    else if (side == 2) { in = j <= g.neg[1]; out = j >= g.pos[1]; }
+    else { in = j >= g.pos[1]; out = j <= g.neg[1]; }
+    const double vn = (side < 2) ? x : y;
+    if (in) {
+      const double cx = (x - xb) * (x - xb), cy = (y - yb) * (y - yb), cz = (z - zb) * (z - zb);
+      hin += vn * fc[l] * (cx + cy + cz);
+    } else if (out) {
+      double gv = 0.0;
+      if (p.acc > 0.0) gv = rt * wm[l];
+      if (p.acc < 1.0) gv += sp[l];
+      const double cx = (x - uw) * (x - uw), cy = (y - vw) * (y - vw), cz = (z - ww) * (z - ww);
+      hout += vn * gv * (cx + cy + cz);
+    }
+  }
+  hin = block_sum<kMomBS>(hin, red);
+  hout = block_sum<kMomBS>(hout, red);
+  if (threadIdx.x == 0) q[side * p.max_along + pos] = 0.5 * mass * (hin + hout);
+}
+
 // Specular part of the wall ghosts (boundary_conditions.f90:832-858 and
 // analogues): (1-acc)*f at each incoming node, reflected and remapped onto the
 // ghost with apply_bc_remapping (:1852-1929). One lane per (incoming node).

@@ -126,6 +126,21 @@ class Solver:
         self._check(self._fn("moments_ext")(self.h, _dptr(out)))
         return out
 
+    def set_wall_heat_flux_freq(self, every):
+        """Record the wall heat fluxes during the convection of every step whose
+        ntime is a multiple of `every` (set_wall_heat_flux_dump_freq,
+        boundary_conditions.f90:108-116; 0 stops)."""
+        self._check(self._fn("set_wall_heat_flux_freq")(self.h, int(every)))
+
+    def wall_heat_flux(self):
+        """The last recorded wall heat fluxes, [4, max(nx, ny)] (left, right,
+        bottom, top; index along the wall), and their step (-1: none);
+        boundary_conditions.f90:872-923 and analogues."""
+        out = np.zeros((4, max(self.cfg.nx, self.cfg.ny)), dtype=np.float64)
+        nt = C.c_int32(-1)
+        self._check(self._fn("wall_heat_flux")(self.h, _dptr(out), C.byref(nt)))
+        return out, nt.value
+
     def compute_collision_integral(self, ntime):
         """Collision phase of every cell (collision_integral.f90:163); returns
         the samples drawn per cell."""

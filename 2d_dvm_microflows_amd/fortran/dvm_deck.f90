@@ -60,6 +60,8 @@ module DVMDeck
      integer :: df_freq = 1, df_dim = 3, n_loc = 0
      integer :: loc_x(64) = 1, loc_y(64) = 1
      integer :: i_zero = 0, j_zero = 0, k_zero = 0
+     ! wall heat-flux files (boundary_conditions.f90:108-116, 550-575, 872-923)
+     integer :: heat_freq = 0
      ! restart (restart.f90:66-80, read_input_2D.f90:745-760)
      logical :: read_rst = .false., write_rst = .false.
      integer :: write_freq = 0
@@ -84,6 +86,7 @@ module DVMDeck
 
   public :: deck_read, deck_has, deck_real, deck_int, deck_str, run_from_deck, write_reference_values
   public :: props_header, props_record, write_restart, read_restart, restart_due, df_headers, df_records
+  public :: wall_headers, wall_records
 
 contains
 
@@ -444,6 +447,7 @@ contains
           return
        end if
     end if
+    r%heat_freq = max(0, deck_int(d, 'wall_heat_flux_dump_freq', 0))
     ! restart files
     r%read_rst = deck_int(d, 'read_rst', 0) /= 0
     r%write_rst = deck_int(d, 'write_rst', 0) /= 0
@@ -862,5 +866,53 @@ contains
     cz = cz * cz
     maxwell = coeff * exp(-(cx + cy + cz) * mass / temp)
   end function maxwell
+
+  ! The four wall files, created with their headers when the physical grid is
+  ! 2-D (initialize_solver, dvm_solver.f90:208-226).
+  subroutine wall_headers(r)
+    type(run_t), intent(in) :: r
+    character(len=6), parameter :: wname(4) = [character(len=6) :: 'left', 'right', 'bottom', 'top']
+    integer :: u, s
+    if (.not. (r%cfg%nx > 1 .and. r%cfg%ny > 1)) return
+    do s = 1, 4
+       open(newunit=u, file=trim(wname(s)) // '_wall.dat', status='replace', action='write')
+       write(u, fmt='(a)') 'title = "' // trim(wname(s)) // ' wall fluxes"'
+       write(u, fmt='(a)') 'variables = "Time", "Location", "Heat Flux"'
+       close(u)
+    end do
+  end subroutine wall_headers
+
+  ! One dump step: a zone line in every wall file (apply_boundary_conditions,
+  ! boundary_conditions.f90:563-575), then a record per cell along each
+  ! diffuse / specular wall (time, cell-centre location, heat flux;
+  ! :920-922 and analogues). q(pos, side) as returned by dvm_wall_heat_flux.
+  subroutine wall_records(r, ntime, q)
+    type(run_t), intent(in) :: r
+    integer, intent(in) :: ntime
+    real(c_double), intent(in) :: q(:,:)
+    character(len=6), parameter :: wname(4) = [character(len=6) :: 'left', 'right', 'bottom', 'top']
+    integer :: u, s, n, k
+    logical :: wall
+    if (.not. (r%cfg%nx > 1 .and. r%cfg%ny > 1)) return
+    do s = 1, 4
+       n = r%cfg%ny
+       if (s > 2) n = r%cfg%nx
+       open(newunit=u, file=trim(wname(s)) // '_wall.dat', status='old', position='append', action='write')
+       write(u, advance='yes', fmt='(a7,i4)') 'zone i=', n
+       wall = .true.
+       if (s == 1) wall = r%cfg%bc_left == DVM_BC_WALL
+       if (s == 2) wall = r%cfg%bc_right == DVM_BC_WALL
+       if (wall) then
+          do k = 1, n
+             if (s <= 2) then
+                write(u, fmt='(5e24.14)') ntime * r%cfg%dt, (k - 1) * r%cfg%dy + r%cfg%dy * 0.5d0, q(k, s)
+             else
+                write(u, fmt='(5e24.14)') ntime * r%cfg%dt, (k - 1) * r%cfg%dx + r%cfg%dx * 0.5d0, q(k, s)
+             end if
+          end do
+       end if
+       close(u)
+    end do
+  end subroutine wall_records
 
 end module DVMDeck

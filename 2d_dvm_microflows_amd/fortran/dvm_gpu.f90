@@ -29,7 +29,8 @@ program dvm_gpu
   logical :: ok, check
   integer :: i, nargs, steps, rc, n, ncell, u, c, loc, ntime_init
   integer(c_int32_t) :: ntime
-  real(c_double), allocatable, target :: f(:,:), props(:,:), ext(:,:)
+  real(c_double), allocatable, target :: f(:,:), props(:,:), ext(:,:), heat(:,:)
+  integer(c_int32_t) :: heat_nt
   real(c_double), target :: phase(DVM_NPHASE)
   integer(c_int64_t) :: calls
   real(c_double) :: t0, t1
@@ -89,6 +90,7 @@ program dvm_gpu
   if (len_trim(dump_init) > 0) call dump(dump_init, r%f0)
   if (r%props_vis) call props_header(r, steps)
   call df_headers(r)
+  call wall_headers(r)
   if (check) then
      call print_config()
   else
@@ -101,6 +103,11 @@ contains
     rc = dvm_ctx_create(r%cfg, ctx)
     if (rc /= DVM_OK) call fail(dvm_error_message(c_null_ptr))
     allocate(f(n, ncell), props(DVM_NPROP, ncell), ext(DVM_NEXT, ncell))
+    allocate(heat(max(r%cfg%nx, r%cfg%ny), 4))
+    if (r%heat_freq > 0 .and. r%cfg%nx > 1 .and. r%cfg%ny > 1) then
+       rc = dvm_set_wall_heat_flux_freq(ctx, int(r%heat_freq, c_int32_t))
+       if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+    end if
     rc = dvm_upload_f(ctx, 0_c_int32_t, r%f0)
     if (rc == DVM_OK) rc = dvm_moments(ctx, props)
     if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
@@ -113,6 +120,13 @@ contains
     do ntime = ntime_init + 1, steps
        rc = dvm_step(ctx, ntime)
        if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+       if (r%heat_freq > 0 .and. r%cfg%nx > 1 .and. r%cfg%ny > 1) then
+          if (mod(int(ntime), r%heat_freq) == 0) then
+             rc = dvm_wall_heat_flux(ctx, heat, heat_nt)
+             if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
+             call wall_records(r, int(ntime), heat)
+          end if
+       end if
        if (restart_due(r, int(ntime), steps)) then
           rc = dvm_download_f(ctx, 0_c_int32_t, f)
           if (rc /= DVM_OK) call fail(dvm_error_message(ctx))

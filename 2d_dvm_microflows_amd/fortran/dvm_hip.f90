@@ -47,7 +47,8 @@ module DVMHip
   end type dvm_config
 
   public :: dvm_ctx_create, dvm_ctx_destroy, dvm_last_error, dvm_upload_f, dvm_download_f, &
-       dvm_moments, dvm_moments_ext, dvm_collide, dvm_convect, dvm_step, dvm_timing_reset, dvm_timing_read
+       dvm_moments, dvm_moments_ext, dvm_collide, dvm_convect, dvm_step, dvm_timing_reset, dvm_timing_read, &
+       dvm_set_wall_heat_flux_freq, dvm_wall_heat_flux
   public :: dvm_batch_create, dvm_batch_destroy, dvm_batch_last_error, dvm_batch_admit, dvm_batch_retire, &
        dvm_batch_step, dvm_batch_synchronize, dvm_batch_download_f, dvm_batch_moments, dvm_batch_l1, &
        dvm_batch_tune
@@ -89,6 +90,17 @@ module DVMHip
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: ext(*)
      end function dvm_moments_ext
+     integer(c_int) function dvm_set_wall_heat_flux_freq(ctx, every) bind(c, name="dvm_set_wall_heat_flux_freq")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: every
+     end function dvm_set_wall_heat_flux_freq
+     integer(c_int) function dvm_wall_heat_flux(ctx, q, ntime) bind(c, name="dvm_wall_heat_flux")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: q(*)
+       integer(c_int32_t), intent(out) :: ntime
+     end function dvm_wall_heat_flux
      integer(c_int) function dvm_collide(ctx, ntime, samples) bind(c, name="dvm_collide")
        import :: c_int, c_int32_t, c_int64_t, c_ptr
        type(c_ptr), value :: ctx

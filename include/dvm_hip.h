@@ -173,6 +173,23 @@ int dvm_moments(dvm_ctx* ctx, double* props_out);
    :2296-2633); not on the time-step path. */
 int dvm_moments_ext(dvm_ctx* ctx, double* ext);
 
+/* Wall heat fluxes of a 2-D grid, replacing the four wall files written
+   inside set_boundary_nodes (conditions/boundary_conditions.f90:872-923,
+   1069-1115, 1262-1307, 1450-1495) at the interval of
+   set_wall_heat_flux_dump_freq (:108-116). During the convection of every
+   step whose ntime is a multiple of `every` (0: never), each cell along a
+   diffuse / specular wall records
+     q = m/2 * ( sum over the incoming half of v_n f |c - u_cell|^2
+               + sum over the outgoing half of v_n g |c - u_wall|^2 ),
+   f the post-collision distribution, g the wall ghost, u_cell the bulk
+   velocity of the moments the step started with, v_n the wall-normal
+   velocity component (x for left / right, y for bottom / top). */
+int dvm_set_wall_heat_flux_freq(dvm_ctx* ctx, int32_t every);
+/* The last recorded fluxes q[side][pos] (side 0 left, 1 right, 2 bottom,
+   3 top; pos the cell index along the wall, q holds 4 * max(nx, ny) values,
+   sides that are not walls hold 0) and the step they belong to (-1: none). */
+int dvm_wall_heat_flux(dvm_ctx* ctx, double* q, int32_t* ntime_out);
+
 /* One collision phase over every cell at time step ntime (>= 1): CDF build with
    the cached equilibrium, the three sampled VR integrals, phi := delta_phi.
    samples_out[ncell] (may be NULL) receives tot_colls per cell.

@@ -70,6 +70,9 @@ struct orc_ctx {
   double* neqc;       /* [N+1] */
   int32_t eql[NBUCKET], eqr[NBUCKET], neql[NBUCKET], neqr[NBUCKET];
   double sigma, vhs_exponent;
+  /* wall heat fluxes (set_wall_heat_flux_dump_freq, boundary_conditions.f90:108) */
+  int heat_every, heat_on, heat_ntime;
+  double* heat;       /* [4][max(nx,ny)] */
   char err[256];
 };
 
@@ -393,7 +396,7 @@ void orc_ctx_destroy(orc_ctx* c) {
   if (!c) return;
   for (int d = 0; d < 3; ++d) { free(c->g.ax[d]); free(c->g.beta[d]); }
   free(c->f); free(c->fd); free(c->props); free(c->eq_df); free(c->eq_par);
-  free(c->eqc); free(c->neqc);
+  free(c->eqc); free(c->neqc); free(c->heat);
   free(c);
 }
 
@@ -926,6 +929,77 @@ static double maxwell(double x, double y, double z, double coeff, double mass, d
   return coeff * exp(-csq * mass / temp);
 }
 
+/* Heat flux into the wall on `side` of a wall cell (boundary_conditions.f90:
+   872-923 left, 1069-1115 right, 1262-1307 bottom, 1450-1495 top), in the
+   reference's loop order: incoming half of f against the cell's bulk velocity,
+   outgoing half of the ghost gh against the wall velocity; pc the cell's
+   moments from the start of the step. */
+static void wall_heat(orc_ctx* c, int side, int pos, const double* f0, const double* gh, const double* pc) {
+  const ogrid* g = &c->g;
+  int nvx = g->n[0], nvy = g->n[1], nvz = g->n[2];
+  const double *X = g->ax[0], *Y = g->ax[1], *Z = g->ax[2];
+  double xb = pc[DVM_P_U], yb = pc[DVM_P_V], zb = pc[DVM_P_W];
+  const double* wl = c->cfg.wall[side];
+  double u = wl[DVM_W_U], v = wl[DVM_W_V], w = wl[DVM_W_W];
+  /* index ranges of the incoming and outgoing halves along the wall normal */
+  int in0, in1, out0, out1;
+  if (side == 0 || side == 2) {
+    int neg = g->neg[side / 2], posi = g->pos[side / 2], mx = g->n[side / 2] - 1;
+    in0 = 0; in1 = neg; out0 = posi; out1 = mx;
+  } else {
+    int neg = g->neg[side / 2], posi = g->pos[side / 2], mx = g->n[side / 2] - 1;
+    in0 = posi; in1 = mx; out0 = 0; out1 = neg;
+  }
+  double hin = 0.0, hout = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    double ub = pass ? u : xb, vb = pass ? v : yb, wb = pass ? w : zb;
+    const double* src = pass ? gh : f0;
+    int a0 = pass ? out0 : in0, a1 = pass ? out1 : in1;
+    double h = 0.0;
+    for (int k = 0; k < nvz; ++k) {
+      double z = Z[k], czsq = (z - wb) * (z - wb);
+      if (side < 2) {
+        for (int j = 0; j < nvy; ++j) {
+          double y = Y[j], cysq = (y - vb) * (y - vb);
+          for (int i = a0; i <= a1; ++i) {
+            double x = X[i], cxsq = (x - ub) * (x - ub);
+            double csq = cxsq + cysq + czsq;
+            h = h + x * src[i + nvx * (j + nvy * k)] * csq;
+          }
+        }
+      } else {
+        for (int j = a0; j <= a1; ++j) {
+          double y = Y[j], cysq = (y - vb) * (y - vb);
+          for (int i = 0; i < nvx; ++i) {
+            double x = X[i], cxsq = (x - ub) * (x - ub);
+            double csq = cxsq + cysq + czsq;
+            h = h + y * src[i + nvx * (j + nvy * k)] * csq;
+          }
+        }
+      }
+    }
+    if (pass) hout = h; else hin = h;
+  }
+  int along = c->cfg.nx > c->cfg.ny ? c->cfg.nx : c->cfg.ny;
+  c->heat[side * along + pos] = 0.5 * c->cfg.mass * (hin + hout);
+}
+
+int orc_set_wall_heat_flux_freq(orc_ctx* c, int32_t every) {
+  if (every < 0) return set_err(c, DVM_E_ARG, "every must be >= 0");
+  if (every > 0 && c->cfg.dim != DVM_DIM_2D) return set_err(c, DVM_E_ARG, "wall heat fluxes need a 2-D grid");
+  int along = c->cfg.nx > c->cfg.ny ? c->cfg.nx : c->cfg.ny;
+  if (every > 0 && !c->heat) c->heat = (double*)calloc(4 * (size_t)along, sizeof(double));
+  c->heat_every = every;
+  return DVM_OK;
+}
+
+int orc_wall_heat_flux(orc_ctx* c, double* q, int32_t* ntime_out) {
+  int along = c->cfg.nx > c->cfg.ny ? c->cfg.nx : c->cfg.ny;
+  for (int i = 0; i < 4 * along; ++i) q[i] = (c->heat && c->heat_ntime > 0) ? c->heat[i] : 0.0;
+  if (ntime_out) *ntime_out = (c->heat && c->heat_ntime > 0) ? c->heat_ntime : -1;
+  return DVM_OK;
+}
+
 /* boundary_conditions.f90:630-1656 set_boundary_nodes: builds the 4 ghosts of
    boundary cell (ix,iy) (0-based) and writes its upwind update into fc. */
 static void boundary_cell_2d(orc_ctx* c, int ix, int iy, double* fc, double* ghost /* [4][N] */) {
@@ -1104,6 +1178,13 @@ static void boundary_cell_2d(orc_ctx* c, int ix, int iy, double* fc, double* gho
     memcpy(g4, c->f + (size_t)(ix + nxs * (iy + 1)) * N, sizeof(double) * N);
     if (j_zero_flag)
       for (int k = 0; k < nvz; ++k) for (int i = 0; i < nvx; ++i) g4[IDX(i, j_zero, k)] = 0.0;
+  }
+  if (c->heat_on) {
+    const double* pc = c->props + (size_t)(ix + nxs * iy) * DVM_NPROP;
+    if (ix == 0 && cfg->bc_left == DVM_BC_WALL) wall_heat(c, 0, iy, f0, g1, pc);
+    if (ix == nxs - 1 && cfg->bc_right == DVM_BC_WALL) wall_heat(c, 1, iy, f0, g2, pc);
+    if (iy == 0) wall_heat(c, 2, ix, f0, g3, pc);
+    if (iy == nys - 1) wall_heat(c, 3, ix, f0, g4, pc);
   }
   /* ---- boundary-cell upwind update (boundary_conditions.f90:1548-1643) */
   for (int k = 0; k < nvz; ++k) {
@@ -1327,9 +1408,10 @@ static int convect_1d(orc_ctx* c, double* fc) {
 }
 
 int orc_convect(orc_ctx* c, int32_t species, int32_t ntime) {
-  (void)ntime;
   if (species != 0) return set_err(c, DVM_E_ARG, "only species 0 is supported");
   if (!c->cfg.enable_convection || c->cfg.dim == DVM_DIM_0D) return DVM_OK;
+  c->heat_on = c->heat && c->heat_every > 0 && ntime % c->heat_every == 0;
+  if (c->heat_on) c->heat_ntime = ntime;
   int N = c->g.num_points;
   double* fc = c->fd;
   if (c->cfg.dim == DVM_DIM_1D) {

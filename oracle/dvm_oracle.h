@@ -35,6 +35,8 @@ int  orc_remap(orc_ctx* ctx, int64_t n, const double* uvw, double f_off,
                int32_t* stencil, double* mass);
 int  orc_collide_trace(orc_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_records,
                        int32_t* irec, double* drec, int64_t* n_out);
+int  orc_set_wall_heat_flux_freq(orc_ctx* ctx, int32_t every);
+int  orc_wall_heat_flux(orc_ctx* ctx, double* q, int32_t* ntime_out);
 /* Collision phase restricted to cells [c0, c1) (CPU-baseline sampling). */
 int  orc_collide_cells(orc_ctx* ctx, int32_t ntime, int32_t c0, int32_t c1);
 

@@ -165,3 +165,15 @@ def test_distribution_function_location_outside_the_grid_is_skipped(tmp_path):
                                             y_output_locations="'3 3'"), "--check")
     assert "(11,3) is outside the physical grid" in r.stdout
     assert not (tmp_path / "dist_func_1_11_3.dat").exists() and (tmp_path / "dist_func_1_2_3.dat").exists()
+
+
+def test_wall_files_have_headers_on_2d_grids(tmp_path):
+    run_driver(str(tmp_path), deck_text(), "--check")
+    for s in ("left", "right", "bottom", "top"):
+        assert (tmp_path / f"{s}_wall.dat").read_text().splitlines() == [
+            f'title = "{s} wall fluxes"', 'variables = "Time", "Location", "Heat Flux"']
+
+
+def test_no_wall_files_on_1d_grids(tmp_path):
+    run_driver(str(tmp_path), deck_text(nx_space="'6'", ny_space="'1'", dvm_dim=1), "--check")
+    assert not (tmp_path / "left_wall.dat").exists()

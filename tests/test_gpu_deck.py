@@ -159,3 +159,43 @@ def test_distribution_function_records(pkg, tmp_path, dim):
         dev = rec[:, dim + 3]
         assert np.max(np.abs(dev - (f[sel] - feq))) <= 1e-12 * np.max(np.abs(f[sel]))
     orc.close()
+
+
+def test_wall_heat_flux_files(pkg, tmp_path):
+    """wall_heat_flux_dump_freq = 2 over 4 steps: each of the four wall files
+    holds its header, then per dump step a zone line and one record per cell
+    along the wall (time, cell-centre location, heat flux) equal to the
+    oracle's fluxes from the same initial f (1e-12 of the half-space sums the
+    flux is the difference of, m/2 sum |v_n| f |v|^2)."""
+    over = dict(wall_heat_flux_dump_freq=2, nx_space="'5'", ny_space="'4'", dvm_samples=300)
+    print("[deck] wall heat-flux files, 4 steps", flush=True)
+    run_driver(str(tmp_path), deck_text(**over), "--steps", "4", "--dump-init", "f0.bin")
+    files = {s: (tmp_path / f"{s}_wall.dat").read_text().splitlines() for s in ("left", "right", "bottom", "top")}
+    cfg = solver_config(pkg, parse_check(run_driver(str(tmp_path), deck_text(**over), "--check").stdout))
+    f0 = np.fromfile(tmp_path / "f0.bin").reshape(cfg.nx * cfg.ny, -1)
+    orc = oracle_solver(cfg)
+    orc.initialize(f0)
+    orc.set_wall_heat_flux_freq(2)
+    want = {}
+    for n in range(1, 5):
+        orc.step(n)
+        if n % 2 == 0:
+            want[n] = orc.wall_heat_flux()[0]
+    orc.close()
+    for side, (name, lines) in enumerate(files.items()):
+        along = cfg.ny if side < 2 else cfg.nx
+        assert lines[:2] == [f'title = "{name} wall fluxes"', 'variables = "Time", "Location", "Heat Flux"']
+        body = lines[2:]
+        assert len(body) == 2 * (1 + along)
+        for z, n in enumerate((2, 4)):
+            block = body[z * (1 + along):(z + 1) * (1 + along)]
+            assert block[0] == f"zone i={along:4d}"
+            rec = np.array([[float(x) for x in ln.split()] for ln in block[1:]])
+            h = cfg.dy if side < 2 else cfg.dx
+            assert np.allclose(rec[:, 0], n * cfg.dt, rtol=1e-13)
+            assert np.allclose(rec[:, 1], (np.arange(along) + 0.5) * h, rtol=1e-13)
+            q = want[n][side, :along]
+            v = pkg.node_velocities(cfg)
+            scale = 0.5 * cfg.mass * np.max(np.abs(f0) @ (np.abs(v[:, 0 if side < 2 else 1]) * (v ** 2).sum(1)))
+            # records carry 15 significant digits
+            assert np.max(np.abs(rec[:, 2] - q)) <= 1e-12 * scale + 1e-14 * np.max(np.abs(q))
