@@ -41,6 +41,9 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_NODE = 2 * 8
 COLLIDE_BYTES_PER_NODE = 5 * 8
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+VALU_TABLE = os.path.join(ROOT, "profiles", "valu_per_launch.txt")
+SIMDS = 1024        # MI355X: 256 CUs x 4 SIMDs
+CLOCK_GHZ = 2.4     # MI355X peak engine clock
 METRIC = "cell-node updates/s, FP32-collision vs plain FP64 at 1e-5; p50/p99 solve latency"
 
 
@@ -117,12 +120,32 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def measured_valu(kernel, workload):
+    """Wave-level VALU instructions per launch from the rocprofv3 counter
+    table, or None when it has not been measured."""
+    if not os.path.exists(VALU_TABLE):
+        return None
+    for line in open(VALU_TABLE):
+        f = line.split()
+        if len(f) >= 3 and not line.startswith("#") and f[0] == kernel and f[1] == workload:
+            return float(f[2])
+    return None
+
+
 def roofline(kernel, workload, alg_bytes, ms):
     achieved = alg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
-    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": measured_traffic(kernel, workload), "algorithmic_bytes_per_launch": alg_bytes,
-            "launch_ms": ms}
+    out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+           "traffic": measured_traffic(kernel, workload), "algorithmic_bytes_per_launch": alg_bytes,
+           "launch_ms": ms}
+    # the issue rate that actually bounds the collision kernels: measured VALU
+    # instructions per launch against one wave64 VALU instruction per 4 cycles
+    # on each of the 1024 SIMDs at the 2.4 GHz peak engine clock
+    valu = measured_valu(kernel, workload)
+    if valu and ms > 0:
+        out["valu_issue"] = {"instructions_per_launch": valu, "simds": SIMDS, "clock_ghz": CLOCK_GHZ,
+                             "frac": valu * 4.0 / (SIMDS * CLOCK_GHZ * 1e9 * ms * 1e-3)}
+    return out
 
 
 def run_batched(pkg, count, steps, precision, tune=True):
