@@ -86,7 +86,7 @@ module DVMDeck
 
   public :: deck_read, deck_has, deck_real, deck_int, deck_str, run_from_deck, write_reference_values
   public :: props_header, props_record, write_restart, read_restart, restart_due, df_headers, df_records
-  public :: wall_headers, wall_records
+  public :: wall_headers, wall_records, species_header, species_record
 
 contains
 
@@ -914,5 +914,192 @@ contains
        close(u)
     end do
   end subroutine wall_records
+
+  ! Header of the per-species file <properties_vis_filename>_1.dat
+  ! (visualization.f90:660-700): names trimmed, not padded as in the total
+  ! file; the same 1-D zone line.
+  subroutine species_header(r, end_time)
+    type(run_t), intent(in) :: r
+    integer, intent(in) :: end_time
+    integer :: u, i, print_times
+    open(newunit=u, file=trim(adjustl(r%props_file)) // '_1.dat', status='replace', action='write')
+    write(u, advance='yes', fmt='(a)') 'title = "Macroscopic Properties"'
+    write(u, advance='no', fmt='(a)') 'variables = "Time", "x-Location", "y-Location"'
+    do i = 1, 27
+       if (r%to_output(i)) write(u, advance='no', fmt='(a2,a)') ', ', trim(adjustl(prop_names(i)))
+    end do
+    do i = 1, r%n_stress
+       write(u, advance='no', fmt='(a7,i2,a1)') ', "Tau ', r%stress_dirs(i), '"'
+    end do
+    do i = 1, r%n_moment
+       write(u, advance='no', fmt='(a10,i2,a1)') ', "Moment ', r%moment_nums(i), '"'
+    end do
+    write(u, advance='yes', fmt='(1x)')
+    if (r%cfg%nx == 1) then
+       if (mod(end_time, r%props_freq) == 0) then
+          print_times = end_time / r%props_freq + 1
+       else
+          print_times = end_time / r%props_freq + 2
+       end if
+       write(u, advance='yes', fmt='(a7,i4)') 'zone i=', print_times
+    end if
+    close(u)
+  end subroutine species_header
+
+  ! One record of the per-species file (visualization.f90:1666-1713) with the
+  ! species values of get_data_to_output (:2296-2633) for one monatomic
+  ! species: as the total file, plus the directional temperatures
+  ! (compute_directional_temperature, physical_props.f90:1811-1869), the
+  ! analytic relaxation temperatures and compute_moment (:2073-2150), summed
+  ! from f in the reference's loop order. Monatomic species leave the rotational
+  ! and vibrational collision numbers cZr / cZv unset; columns 24 and 25 take
+  ! the limit of a zero collision number, T_eq. Locations start at 0 (:1677-1680)
+  ! and the zone line is written only when nx > 1. f(N, ncell).
+  subroutine species_record(r, time, f, props, ext)
+    type(run_t), intent(in) :: r
+    real(c_double), intent(in) :: time, f(:,:), props(:,:), ext(:,:)
+    integer :: u, nx, ny, c, i, prop
+    real(c_double) :: xl, yl, val, dens, uu, vv, ww, ttr, tr_e, teq, sd24, sd25, td(3)
+    open(newunit=u, file=trim(adjustl(r%props_file)) // '_1.dat', status='old', position='append', action='write')
+    if (r%cfg%nx > 1) write(u, advance='yes', fmt='(a7,i5)') 'zone i=', r%cfg%nx * r%cfg%ny
+    xl = 0.0d0
+    do nx = 1, r%cfg%nx
+       yl = 0.0d0
+       do ny = 1, r%cfg%ny
+          c = nx + r%cfg%nx * (ny - 1)
+          dens = props(DVM_P_DENS, c)
+          uu = props(DVM_P_U, c)
+          vv = props(DVM_P_V, c)
+          ww = props(DVM_P_W, c)
+          ttr = props(DVM_P_TRTEMP, c)
+          tr_e = 0.75d0 * dens * ttr
+          teq = 4.0d0 * (tr_e + 0.0d0) / (dens * (3.0d0 + 0.0d0))
+          sd24 = 0.0d0
+          sd25 = 0.0d0
+          if (abs(time) >= 1.0d-13) then
+             sd24 = teq
+             sd25 = teq
+          end if
+          if (r%to_output(11) .or. r%to_output(12) .or. r%to_output(13)) call dir_temps(r, f(:, c), dens, uu, vv, ww, td)
+          write(u, advance='no', fmt='(3e24.15)') time, xl, yl
+          do prop = 1, 27
+             if (.not. r%to_output(prop)) cycle
+             select case (prop)
+             case (1); val = dens
+             case (2); val = uu
+             case (3); val = vv
+             case (4); val = ww
+             case (5); val = sqrt(uu * uu + vv * vv + ww * ww)
+             case (6); val = uu / sqrt(0.5d0 * r%gamma * ttr / r%mol_mass)
+             case (7); val = ttr
+             case (10); val = props(DVM_P_TEMP, c)
+             case (11); val = td(1)
+             case (12); val = td(2)
+             case (13); val = td(3)
+             case (14, 17); val = tr_e
+             case (18); val = ext(10, c)
+             case (19); val = ext(1, c)
+             case (20); val = ext(2, c)
+             case (21); val = ext(3, c)
+             case (24); val = sd24
+             case (25); val = sd25
+             case (26)
+                if (abs(time) < 1.0d-13) then
+                   val = ttr
+                else
+                   val = (4.0d0 / 3.0d0) * (tr_e - (0.0d0 / 4.0d0) * sd24 - (0.0d0 / 4.0d0) * sd25)
+                end if
+             case (27); val = 0.5d0 * dens * ttr
+             case default; val = 0.0d0
+             end select
+             write(u, advance='no', fmt='(e24.16)') val
+          end do
+          do i = 1, r%n_stress
+             select case (r%stress_dirs(i))
+             case (11); val = ext(4, c)
+             case (22); val = ext(5, c)
+             case (33); val = ext(6, c)
+             case (12); val = ext(7, c)
+             case (13); val = ext(8, c)
+             case (23); val = ext(9, c)
+             case default; val = 0.0d0
+             end select
+             write(u, advance='no', fmt='(e24.16)') val
+          end do
+          do i = 1, r%n_moment
+             write(u, advance='no', fmt='(e24.16)') moment(r, f(:, c), r%moment_nums(i), dens, uu, vv, ww, ttr)
+          end do
+          write(u, advance='yes', fmt='(1x)')
+          yl = yl + r%cfg%dy
+       end do
+       xl = xl + r%cfg%dx
+    end do
+    close(u)
+  end subroutine species_record
+
+  ! compute_directional_temperature (physical_props.f90:1811-1869) for x, y, z.
+  subroutine dir_temps(r, fc, dens, u, v, w, td)
+    type(run_t), intent(in) :: r
+    real(c_double), intent(in) :: fc(:), dens, u, v, w
+    real(c_double), intent(out) :: td(3)
+    integer :: i, j, k, l
+    real(c_double) :: cs
+    td = 0.0d0
+    l = 0
+    do k = 1, size(r%vz)
+       do j = 1, size(r%vy)
+          do i = 1, size(r%vx)
+             l = l + 1
+             cs = (r%vx(i) - u) * (r%vx(i) - u)
+             td(1) = td(1) + fc(l) * cs
+             cs = (r%vy(j) - v) * (r%vy(j) - v)
+             td(2) = td(2) + fc(l) * cs
+             cs = (r%vz(k) - w) * (r%vz(k) - w)
+             td(3) = td(3) + fc(l) * cs
+          end do
+       end do
+    end do
+    td = (2.0d0 / 3.0d0) * r%cfg%mass * td / dens
+  end subroutine dir_temps
+
+  ! compute_moment (physical_props.f90:2073-2150): normalised n-th moment of
+  ! the peculiar speed against its Maxwellian value.
+  real(c_double) function moment(r, fc, n, dens, u, v, w, temp)
+    type(run_t), intent(in) :: r
+    real(c_double), intent(in) :: fc(:), dens, u, v, w, temp
+    integer, intent(in) :: n
+    integer :: i, j, k, l, in
+    real(c_double) :: d, nfact, mom_mb, czsq, cysq, cxsq, cc
+    d = dble(n)
+    nfact = 1.0d0
+    in = 1
+    do
+       if (n + in >= 1) then
+          nfact = nfact * dble(n + in)
+          in = in - 2
+       else
+          exit
+       end if
+    end do
+    mom_mb = nfact * temp**(1.5d0 + d) * (2.0d0**(-0.5d0 * d))
+    moment = 0.0d0
+    l = 0
+    do k = 1, size(r%vz)
+       czsq = (r%vz(k) - w)
+       czsq = czsq * czsq
+       do j = 1, size(r%vy)
+          cysq = (r%vy(j) - v)
+          cysq = cysq * cysq
+          do i = 1, size(r%vx)
+             l = l + 1
+             cxsq = (r%vx(i) - u)
+             cxsq = cxsq * cxsq
+             cc = sqrt(cxsq + cysq + czsq)
+             moment = moment + cc**d * fc(l)
+          end do
+       end do
+    end do
+    moment = r%cfg%mass**(0.5d0 * d) * moment / (dens * mom_mb)
+  end function moment
 
 end module DVMDeck

@@ -88,7 +88,10 @@ program dvm_gpu
      if (.not. ok) call fail(msg)
   end if
   if (len_trim(dump_init) > 0) call dump(dump_init, r%f0)
-  if (r%props_vis) call props_header(r, steps)
+  if (r%props_vis) then
+     call props_header(r, steps)
+     call species_header(r, steps)
+  end if
   call df_headers(r)
   call wall_headers(r)
   if (check) then
@@ -167,13 +170,16 @@ contains
     call dvm_ctx_destroy(ctx)
   end subroutine solve
 
-  ! Moments and output moments of the resident f, then one record.
+  ! Moments and output moments of the resident f, then one record of the total
+  ! and the per-species property files.
   subroutine write_props(nt)
     integer, intent(in) :: nt
     rc = dvm_moments_ext(ctx, ext)
     if (rc == DVM_OK) rc = dvm_moments(ctx, props)
+    if (rc == DVM_OK) rc = dvm_download_f(ctx, 0_c_int32_t, f)
     if (rc /= DVM_OK) call fail(dvm_error_message(ctx))
     call props_record(r, dble(nt) * r%cfg%dt, props, ext)
+    call species_record(r, dble(nt) * r%cfg%dt, f, props, ext)
   end subroutine write_props
 
   ! Distribution-function records of the resident f and its moments.

@@ -177,3 +177,15 @@ def test_wall_files_have_headers_on_2d_grids(tmp_path):
 def test_no_wall_files_on_1d_grids(tmp_path):
     run_driver(str(tmp_path), deck_text(nx_space="'6'", ny_space="'1'", dvm_dim=1), "--check")
     assert not (tmp_path / "left_wall.dat").exists()
+
+
+def test_species_properties_file_header(tmp_path):
+    """<properties_vis_filename>_1.dat: the total file's header with the names
+    trimmed (visualization.f90:660-700)."""
+    run_driver(str(tmp_path), deck_text(write_moment=2, moment_numbers="'3 4'"), "--check")
+    lines = (tmp_path / "macro_properties_1.dat").read_text().splitlines()
+    names = ['"Density"', '"X-Velocity"', '"Y-Velocity"', '"Trans. Temp."', '"Heat Flux X"', '"Heat Flux Y"',
+             '"Pressure"']
+    assert lines == ['title = "Macroscopic Properties"',
+                     'variables = "Time", "x-Location", "y-Location"' + "".join(", " + n for n in names)
+                     + ', "Moment  3", "Moment  4"']

@@ -199,3 +199,52 @@ def test_wall_heat_flux_files(pkg, tmp_path):
             scale = 0.5 * cfg.mass * np.max(np.abs(f0) @ (np.abs(v[:, 0 if side < 2 else 1]) * (v ** 2).sum(1)))
             # records carry 15 significant digits
             assert np.max(np.abs(rec[:, 2] - q)) <= 1e-12 * scale + 1e-14 * np.max(np.abs(q))
+
+
+def test_species_properties_file(pkg, tmp_path):
+    """macro_properties_1.dat of a 2-step run with dumps every step: zones at
+    t = 0, dt, 2 dt (written because nx > 1) of 12 cells with locations from
+    0; directional temperatures (2/3) m sum (c_d - u_d)^2 f / n, the analytic
+    relaxation columns (t = 0: T_rot = 0 and T_tr; later T_eq = T_tr and
+    (4/3) E_tr) and the speed moments m^(k/2) sum |c - u|^k f / (n (k+1)!! T^(3/2+k)
+    2^(-k/2)) equal the oracle's moments of the same f (1e-12)."""
+    over = dict(nx_space="'4'", ny_space="'3'", dvm_samples=300, properties_vis_dump_freq=1, write_temp_x=1,
+                write_temp_y=1, write_temp_z=1, write_rot_temp_relax=1, write_kin_temp_relax=1, write_moment=2,
+                moment_numbers="'2 3'")
+    print("[deck] per-species properties file, 2 steps", flush=True)
+    run_driver(str(tmp_path), deck_text(**over), "--steps", "2", "--dump-init", "f0.bin", "--dump", "f2.bin")
+    text = (tmp_path / "macro_properties_1.dat").read_text().splitlines()
+    cfg = solver_config(pkg, parse_check(run_driver(str(tmp_path), deck_text(**over), "--check").stdout))
+    records = []
+    for ln in text[2:]:
+        if ln.startswith("zone"):
+            assert ln == "zone i=   12"
+            records.append([])
+        else:
+            records[-1].append([float(x) for x in ln.split()])
+    assert [len(r) for r in records] == [12, 12, 12]
+    v = pkg.node_velocities(cfg)
+    orc = oracle_solver(cfg)
+    order = [ix + cfg.nx * iy for ix in range(cfg.nx) for iy in range(cfg.ny)]
+    for step, name in ((0, "f0.bin"), (2, "f2.bin")):
+        f = np.fromfile(tmp_path / name).reshape(-1, cfg.num_points)
+        orc.upload_f(f)
+        p = orc.calculate_physical_properties()
+        got = np.array(records[step])
+        assert np.allclose(got[:, 1], np.repeat(np.arange(cfg.nx) * cfg.dx, cfg.ny), rtol=1e-14, atol=1e-300)
+        assert np.allclose(got[:, 2], np.tile(np.arange(cfg.ny) * cfg.dy, cfg.nx), rtol=1e-14, atol=1e-300)
+        for row, c in zip(got, order):
+            n, u, T = p[c, 0], p[c, 1:4], p[c, 4]
+            cc = v - u
+            tdir = (2.0 / 3.0) * cfg.mass * (f[c][:, None] * cc ** 2).sum(0) / n
+            mom = []
+            for k, nf in ((2, 3.0 * 1.0), (3, 4.0 * 2.0)):
+                mb = nf * T ** (1.5 + k) * 2.0 ** (-0.5 * k)
+                mom.append(cfg.mass ** (0.5 * k) * (np.sqrt((cc ** 2).sum(1)) ** k * f[c]).sum() / (n * mb))
+            relax = [0.0, T] if step == 0 else [T, (4.0 / 3.0) * 0.75 * n * T]
+            # columns: t x y | dens u v | T_tr | Tx Ty Tz | hf x y | rot relax, kin relax | p | moments
+            want = np.concatenate([[n, u[0], u[1], T], tdir, [relax[0], relax[1], 0.5 * n * T], mom])
+            sel = np.r_[3:7, 7:10, 12:15, 15:17]
+            have = row[sel]
+            assert np.allclose(have, want, rtol=1e-12, atol=1e-14 * np.max(np.abs(want))), (step, c, have, want)
+    orc.close()
