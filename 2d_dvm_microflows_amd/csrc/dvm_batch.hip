@@ -453,6 +453,8 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
   BHIP(b, hipMemset(b->P.err, 0, 4 * sizeof(int)));
   BHIP(b, hipMemset(b->P.counts, 0, CC * 4 * sizeof(int)));
   BHIP(b, hipMemset(b->d_state, 0, (size_t)max_slots * sizeof(SlotState)));
+  // the full Monte Carlo solver never writes eqv: its deviation signs are dsgn(f - 0)
+  BHIP(b, hipMemset(b->P.eqv, 0, NC * sizeof(double)));
   b->slots.assign(max_slots, SlotDesc{});
   b->hs.assign(max_slots, HostSlot{});
   b->nodes.init(node_cap);
@@ -474,8 +476,8 @@ extern "C" int dvm_batch_create(const dvm_config* species, int32_t max_slots, in
     return bfail(nullptr, DVM_E_ARG, "max_slots, node_capacity and cell_capacity must be >= 1");
   if (!(species->mass > 0.0)) return bfail(nullptr, DVM_E_ARG, "mass must be > 0");
   if (species->num_repl < 1) return bfail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
-  if (species->coll_solver != DVM_COLL_VR)
-    return bfail(nullptr, DVM_E_ARG, "batched solves use the variance-reduction collision solver");
+  if (species->coll_solver != DVM_COLL_VR && species->coll_solver != DVM_COLL_FMC)
+    return bfail(nullptr, DVM_E_ARG, "batched solves use the variance-reduction or full Monte Carlo collision solver");
   if (species->precision != DVM_PREC_FP64 && species->precision != DVM_PREC_FP32)
     return bfail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
   if (species->sample_mode != DVM_SAMPLES_REFERENCE && species->sample_mode != DVM_SAMPLES_EXPLICIT)

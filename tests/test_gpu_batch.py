@@ -151,3 +151,19 @@ def test_fp32_batch_matches_fp32_standalone(pkg):
         b.step(2)
         for (c, f), s in zip(insts, slots):
             assert cellwise(b.download_f(s), standalone(pkg, c, f, 3)) <= 1e-6, c.name
+
+
+def test_full_monte_carlo_instances_match_standalone(pkg):
+    """The batch with the full Monte Carlo solver (iColl = 2): an LDS mixture,
+    a windowed 29^3 grid and a 1-D shock equal standalone contexts."""
+    insts = [(c.with_(coll_solver=pkg.capi.DVM_COLL_FMC), f) for c, f in instances(pkg)]
+    insts = [insts[0], insts[2], insts[3]]
+    print("[batch] full Monte Carlo: 11^3, 29^3, 1-D shock; 3 steps", flush=True)
+    with make_batch(pkg, insts) as b:
+        slots = [b.admit(c, f) for c, f in insts]
+        b.step(3)
+        b.synchronize()
+        for (c, f), s in zip(insts, slots):
+            got = b.download_f(s)
+            want = standalone(pkg, c, f, 3)
+            assert cellwise(got, want) <= RTOL, (c.name, cellwise(got, want))
