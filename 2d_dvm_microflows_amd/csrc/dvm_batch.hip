@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(kMomBS) k_moments_cells(const SlotDesc* __rest
                                                           const SlotState* __restrict__ state,
                                                           const CellRef* __restrict__ cells, double mass,
                                                           BatchPools P) {
-  __shared__ double red[6][kMomBS / 64];
+  __shared__ double red[1][6][kMomBS / 64];
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
   const int cur = state[r.slot].cur;

@@ -570,7 +570,8 @@ static bool collides(const dvm_ctx* c) {
 }
 
 static int moments_async(dvm_ctx* c) {
-  hipLaunchKernelGGL(k_moments, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass);
+  hipLaunchKernelGGL(k_moments, dim3((c->ncell + kMomCells - 1) / kMomCells), dim3(kMomBS), 0, c->stream, c->g,
+                     c->f, c->props, c->cfg.mass, c->ncell);
   HIP_TRY(c, hipGetLastError());
   c->props_valid = true;
   return DVM_OK;
@@ -668,8 +669,7 @@ static int sweep_async(dvm_ctx* c) {
   const unsigned chunks = (unsigned)(((c->N + kConvBS - 1) / kConvBS + 7) / 8 * 8);
   const unsigned ys = (unsigned)(c->ncell < 65535 ? c->ncell : 65535);
   if (c->cfg.dim == DVM_DIM_2D)
-    hipLaunchKernelGGL(k_convect2d, dim3(chunks, c->cfg.nx), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f,
-                       c->fd, c->wallM, c->ratio, c->spec);
+    launch_convect2d(c->g, c->convp, c->f, c->fd, c->wallM, c->ratio, c->spec, c->stream);
   else
     hipLaunchKernelGGL(k_convect1d, dim3(chunks, ys), dim3(kConvBS), 0, c->stream, c->g, c->convp, c->f, c->fd,
                        c->inflow, c->wallM, c->ratio, c->spec);

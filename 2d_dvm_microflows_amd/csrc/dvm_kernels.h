@@ -8,10 +8,23 @@
 namespace dvm {
 
 constexpr int kMomBS = 256;      // moments / wall-flux workgroup
+constexpr int kMomCells = 4;     // cells per k_moments workgroup
 constexpr int kCdfBS = 256;      // CDF workgroup
 constexpr int kCdfV = 8;         // consecutive nodes per lane in the CDF scan
 constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell's LDS slice
 constexpr int kConvBS = 256;     // convection lanes per workgroup
+#ifndef DVM_CONV_STRIP
+#define DVM_CONV_STRIP 4
+#endif
+#ifndef DVM_CONV_DEPTH
+#define DVM_CONV_DEPTH 2
+#endif
+#ifndef DVM_CONV_SEGROWS
+#define DVM_CONV_SEGROWS 8
+#endif
+constexpr int kConvStrip = DVM_CONV_STRIP;      // 2-D sweep: adjacent columns per workgroup
+constexpr int kConvDepth = DVM_CONV_DEPTH;      // 2-D sweep: rows loaded ahead
+constexpr int kConvSegRows = DVM_CONV_SEGROWS;  // 2-D sweep: rows per workgroup
 constexpr int kAxisSlots = 384;  // LDS doubles reserved for the three axes (<= 128 points each)
 constexpr int kMaxAxis = 128;
 // LDS bytes available to one workgroup on gfx950
@@ -56,7 +69,8 @@ struct ConvParams {
   int order;                     // 1-D sweep order (1 or 2)
 };
 
-__global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass);
+__global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass,
+                          int ncell);
 __global__ void k_moments_ext(Grid g, const double* __restrict__ f, const double* __restrict__ props, double mass,
                               double* __restrict__ ext);
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
@@ -74,9 +88,8 @@ __global__ void k_wall_ratio(Grid g, ConvParams p, const double* __restrict__ f,
 __global__ void k_spec_ghost(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ spec,
                              int* __restrict__ err);
 __global__ void k_fill(double* __restrict__ a, long long n, double v);
-__global__ void k_convect2d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ fo,
-                            const double* __restrict__ wallM, const double* __restrict__ ratio,
-                            const double* __restrict__ spec);
+void launch_convect2d(const Grid& g, const ConvParams& p, const double* f, double* fo, const double* wallM,
+                      const double* ratio, const double* spec, hipStream_t st);
 __global__ void k_convect1d(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ fo,
                             const double* __restrict__ inflow, const double* __restrict__ wallM,
                             const double* __restrict__ ratio, const double* __restrict__ spec);

@@ -16,8 +16,9 @@
 //   k_collide    one workgroup per cell; samples spread over the lanes, the
 //                cell's increments accumulated in LDS (ds_add_f64) when the
 //                slice fits, else global atomics into fd
-//   k_convect2d  one lane per (cell,node); interior donor-cell upwind and the
-//                trunk boundary-cell update with wall / neighbour ghosts
+//   k_convect2d  one lane per node of a strip of adjacent columns, walking y;
+//                interior donor-cell upwind and the trunk boundary-cell update
+//                with wall / neighbour ghosts
 //   k_convect1d  one lane per (cell,node); zone-grid 1-D upwind and in/out flow
 //   k_wall_ratio, k_spec_ghost  diffuse flux balance and specular remap ghosts
 #include "dvm_kernels.h"
@@ -76,52 +77,89 @@ __device__ __forceinline__ void node_ijk(const Grid& g, int l, int& i, int& j, i
 // monatomic species: n, u, v, w, <|v|>, T and the derived fields, in one pass
 // over f. The reference's second pass sum f |c - u|^2 is formed as
 // sum f |c|^2 - u . sum f c (algebraically equal; the distribution is read once).
-__device__ __forceinline__ void moments_cell(const Grid& g, const double* __restrict__ fc, double* __restrict__ p,
-                                             double mass, double (*red)[kMomBS / 64]) {
+// Moments of one cell from its six sums {n, n u, n v, n w, sum f |c|^2,
+// sum f |c|}.
+__device__ __forceinline__ void moments_finish(const double* tot, double* __restrict__ p, double mass) {
+  const double dens = tot[0];
+  if (dens == 0.0) {
+    for (int q = 0; q < DVM_NPROP; ++q) p[q] = 0.0;
+    return;
+  }
+  const double u = tot[1] / dens, v = tot[2] / dens, w = tot[3] / dens;
+  const double st = tot[4] - (u * tot[1] + v * tot[2] + w * tot[3]);
+  const double t = 2.0 * mass * st / (3.0 * dens);
+  p[DVM_P_DENS] = dens;
+  p[DVM_P_U] = u;
+  p[DVM_P_V] = v;
+  p[DVM_P_W] = w;
+  p[DVM_P_TRTEMP] = t;
+  p[DVM_P_TEMP] = (3.0 * t + 0.0 + 0.0) / 3.0;  // compute_overall_temperature, no internal dof
+  p[DVM_P_PRESSURE] = 0.5 * dens * t;
+  p[DVM_P_AVGSPEED] = tot[5] / dens;
+}
+
+// Sums of C cells f[c * stride + l] over the nodes, one workgroup of kMomBS
+// lanes: lane t takes nodes t, t + kMomBS, ..., so the node's coordinates and
+// speed are computed once for the C cells and the C loads of an iteration are
+// independent. Each cell's sums are added in the same order whatever C is.
+template <int C>
+__device__ __forceinline__ void moments_sums(const Grid& g, const double* __restrict__ f, size_t stride, int nc,
+                                             double (*red)[6][kMomBS / 64]) {
   const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[C][6];
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[c][q] = 0.0;
   for (int l = threadIdx.x; l < g.N; l += kMomBS) {
     int i, j, k;
     node_ijk(g, l, i, j, k);
-    const double v = fc[l], x = X[i], y = Y[j], z = Z[k];
+    const double x = X[i], y = Y[j], z = Z[k];
     const double c2 = x * x + y * y + z * z;
-    acc[0] += v;
-    acc[1] += v * x;
-    acc[2] += v * y;
-    acc[3] += v * z;
-    acc[4] += v * c2;
-    acc[5] += v * sqrt(c2);
+    const double sp = sqrt(c2);
+    double v[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[c] = (c < nc) ? f[(size_t)c * stride + l] : 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      acc[c][0] += v[c];
+      acc[c][1] += v[c] * x;
+      acc[c][2] += v[c] * y;
+      acc[c][3] += v[c] * z;
+      acc[c][4] += v[c] * c2;
+      acc[c][5] += v[c] * sp;
+    }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const double t = wave_sum(acc[q]);
-    if (lane == 0) red[q][wid] = t;
-  }
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double t = wave_sum(acc[c][q]);
+      if (lane == 0) red[c][q][wid] = t;
+    }
   __syncthreads();
+}
+
+__device__ __forceinline__ void moments_total(const double (*red)[kMomBS / 64], double* tot) {
+  for (int q = 0; q < 6; ++q) {
+    double t = 0.0;
+    for (int w = 0; w < kMomBS / 64; ++w) t += red[q][w];
+    tot[q] = t;
+  }
+}
+
+// calculate_physical_properties (physical_props.f90:285-485) for one
+// monatomic species: n, u, v, w, <|v|>, T and the derived fields, in one pass
+// over f. The reference's second pass sum f |c - u|^2 is formed as
+// sum f |c|^2 - u . sum f c (algebraically equal; the distribution is read once).
+__device__ __forceinline__ void moments_cell(const Grid& g, const double* __restrict__ fc, double* __restrict__ p,
+                                             double mass, double (*red)[6][kMomBS / 64]) {
+  moments_sums<1>(g, fc, 0, 1, red);
   if (threadIdx.x == 0) {
     double tot[6];
-    for (int q = 0; q < 6; ++q) {
-      double t = 0.0;
-      for (int w = 0; w < kMomBS / 64; ++w) t += red[q][w];
-      tot[q] = t;
-    }
-    const double dens = tot[0];
-    if (dens == 0.0) {
-      for (int q = 0; q < DVM_NPROP; ++q) p[q] = 0.0;
-      return;
-    }
-    const double u = tot[1] / dens, v = tot[2] / dens, w = tot[3] / dens;
-    const double st = tot[4] - (u * tot[1] + v * tot[2] + w * tot[3]);
-    const double t = 2.0 * mass * st / (3.0 * dens);
-    p[DVM_P_DENS] = dens;
-    p[DVM_P_U] = u;
-    p[DVM_P_V] = v;
-    p[DVM_P_W] = w;
-    p[DVM_P_TRTEMP] = t;
-    p[DVM_P_TEMP] = (3.0 * t + 0.0 + 0.0) / 3.0;  // compute_overall_temperature, no internal dof
-    p[DVM_P_PRESSURE] = 0.5 * dens * t;
-    p[DVM_P_AVGSPEED] = tot[5] / dens;
+    moments_total(red[0], tot);
+    moments_finish(tot, p, mass);
   }
 }
 
@@ -171,11 +209,18 @@ __global__ void __launch_bounds__(kMomBS) k_moments_ext(Grid g, const double* __
   }
 }
 
+// kMomCells cells per workgroup (the last workgroup may hold fewer).
 __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
-                                                    double* __restrict__ props, double mass) {
-  __shared__ double red[6][kMomBS / 64];
-  const int cell = blockIdx.x;
-  moments_cell(g, f + (size_t)cell * g.N, props + (size_t)cell * DVM_NPROP, mass, red);
+                                                    double* __restrict__ props, double mass, int ncell) {
+  __shared__ double red[kMomCells][6][kMomBS / 64];
+  const int c0 = blockIdx.x * kMomCells;
+  const int nc = (ncell - c0 < kMomCells) ? ncell - c0 : kMomCells;
+  moments_sums<kMomCells>(g, f + (size_t)c0 * g.N, (size_t)g.N, nc, red);
+  if ((int)threadIdx.x < nc) {
+    double tot[6];
+    moments_total(red[threadIdx.x], tot);
+    moments_finish(tot, props + (size_t)(c0 + threadIdx.x) * DVM_NPROP, mass);
+  }
 }
 
 // ------------------------------------------------------------------- CDF
@@ -1195,22 +1240,32 @@ __device__ __forceinline__ double ghost(const ConvParams& p, const double* __res
 }
 
 // 2-D upwind sweep: interior cells (convection.f90:119-276) and boundary cells
-// (boundary_conditions.f90:1548-1643) in one pass. Each lane owns one velocity
-// node of one column ix and walks the column in y, keeping the rows iy-1, iy,
-// iy+1 of its node in registers, so the y-upwind value is never re-read; the
-// x-upwind value comes from the neighbouring column's lane of the same node.
-// Grid: x = node chunks of kConvBS nodes, padded to a multiple of 8, y = column.
-// Workgroups are dispatched to the 8 XCDs round-robin, so with a chunk count
-// divisible by 8 every column's chunk j runs on XCD j % 8, next to its
-// neighbouring columns, and the x-upwind re-read hits that XCD's L2.
+// (boundary_conditions.f90:1548-1643) in one pass. A workgroup takes one chunk
+// of kConvBS velocity nodes over a strip of S adjacent columns and a segment of
+// rows: each lane owns one node in every column of the strip and walks the
+// segment in y, holding the rows iy-1, iy and iy+1 of its node in registers, so
+// the y-upwind value is read once per segment. The x-upwind value of a column
+// is the neighbouring column's register, except at the strip's upwind edge,
+// where one halo value per row is loaded (the column left of the strip for
+// x >= 0 nodes, right of it for x < 0). Loads run D rows ahead of the row being
+// computed, so (S + 1) D loads per lane are in flight.
+// Grid: x = node chunks padded to a multiple of 8, y = strips, z = row
+// segments. Workgroups are dispatched to the 8 XCDs round-robin, so chunk j of
+// every strip and segment runs on XCD j % 8, and a strip's halo column (the edge
+// column of the neighbouring strip, read by the same chunk) comes through that
+// XCD's L2.
+template <int S, int D>
 __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, const double* __restrict__ f,
                                                        double* __restrict__ fo, const double* __restrict__ wallM,
                                                        const double* __restrict__ ratio,
-                                                       const double* __restrict__ spec) {
+                                                       const double* __restrict__ spec, int seg_rows) {
   const int N = g.N;
   const int l = blockIdx.x * kConvBS + threadIdx.x;
-  const int ix = blockIdx.y;
-  if (l >= N || ix >= p.nx) return;
+  const int ix0 = blockIdx.y * S;
+  const int y0 = blockIdx.z * seg_rows;
+  if (l >= N || ix0 >= p.nx || y0 >= p.ny) return;
+  const int y1 = (p.ny - y0 < seg_rows) ? p.ny : y0 + seg_rows;  // rows [y0, y1)
+  const int ylim = (y1 < p.ny) ? y1 + 1 : p.ny;                    // rows read: [y0 - 1, ylim)
   int i, j, k;
   node_ijk(g, l, i, j, k);
   const double xv = g.ax[0][i], yv = g.ax[1][j];
@@ -1220,36 +1275,87 @@ __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, con
   int xr = -1, yr = -1;
   if (i <= g.neg[0]) xr = 0; else if (p.izflag && i == g.zero[0]) xr = 1; else if (i >= g.pos[0]) xr = 2;
   if (j <= g.neg[1]) yr = 0; else if (p.jzflag && j == g.zero[1]) yr = 1; else if (j >= g.pos[1]) yr = 2;
-  const size_t row = (size_t)p.nx * N;  // distance between rows iy and iy+1
-  const long long xoff = (xv < 0.0) ? (long long)N : -(long long)N;
-  const bool xin = (ix > 0 && ix < p.nx - 1);
-  size_t idx = (size_t)ix * N + l;
-  double fm = 0.0, fc = f[idx];
-  for (int iy = 0; iy < p.ny; ++iy, idx += row) {
-    const double fp = (iy + 1 < p.ny) ? f[idx + row] : 0.0;
-    if (xin && iy > 0 && iy < p.ny - 1) {
-      const double px = f[(long long)idx + xoff];
-      const double py = (yv < 0.0) ? fp : fm;
-      fo[idx] = fc + ax_ * (px - fc) + ay_ * (py - fc);
-    } else if (xr < 0 || yr < 0) {
-      fo[idx] = 0.0;
-    } else {
-      double out = fc;
-      if (xr != 1) {
-        const double gx = ghost(p, f, wallM, ratio, spec, N, xr == 0 ? 1 : 0, ix, iy, l, idx);
-        out = out + ax_ * (gx - fc);
-      }
-      if (yr != 1) {
-        const int side = yr == 0 ? 3 : 2;
-        const bool wall = (side == 2) ? iy == 0 : iy == p.ny - 1;
-        const double gy = wall ? ghost(p, f, wallM, ratio, spec, N, side, ix, iy, l, idx) : (side == 2 ? fm : fp);
-        out = out + ay_ * (gy - fc);
-      }
-      fo[idx] = out;
-    }
-    fm = fc;
-    fc = fp;
+  const int ns = (p.nx - ix0 < S) ? p.nx - ix0 : S;  // columns of this strip
+  const size_t row = (size_t)p.nx * N;              // distance between rows iy and iy+1
+  const bool xneg = xv < 0.0;                       // x-upwind neighbour is ix+1
+  const int hx = xneg ? ix0 + S : ix0 - 1;          // halo column
+  const bool hval = hx >= 0 && hx < p.nx;
+  const long long hoff = (long long)(hx - ix0) * N;
+  const size_t col0 = (size_t)ix0 * N + l;          // column ix0 of row 0
+  // r[0] row iy-1, r[1] row iy, r[1 + d] row iy + d; h[d] the halo of row iy + d
+  double r[D + 2][S], h[D + 1];
+#pragma unroll
+  for (int c = 0; c < S; ++c) r[0][c] = (y0 > 0 && c < ns) ? f[col0 + (size_t)(y0 - 1) * row + (size_t)c * N] : 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int yl = y0 + d;
+    const size_t rb = col0 + (size_t)yl * row;
+#pragma unroll
+    for (int c = 0; c < S; ++c) r[1 + d][c] = (yl < ylim && c < ns) ? f[rb + (size_t)c * N] : 0.0;
+    h[d] = (yl < ylim && hval) ? f[(long long)rb + hoff] : 0.0;
   }
+  for (int iy = y0; iy < y1; ++iy) {
+    {
+      const int yl = iy + D;
+      const size_t rb = col0 + (size_t)yl * row;
+#pragma unroll
+      for (int c = 0; c < S; ++c) r[D + 1][c] = (yl < ylim && c < ns) ? f[rb + (size_t)c * N] : 0.0;
+      h[D] = (yl < ylim && hval) ? f[(long long)rb + hoff] : 0.0;
+    }
+    const size_t rb = col0 + (size_t)iy * row;
+    const double hc = h[0];
+    const bool yin = iy > 0 && iy < p.ny - 1;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      if (c < ns) {
+        const int ix = ix0 + c;
+        const size_t idx = rb + (size_t)c * N;
+        const double fc = r[1][c];
+        double out;
+        if (yin && ix > 0 && ix < p.nx - 1) {
+          const double px = xneg ? (c + 1 < S ? r[1][c + 1 < S ? c + 1 : c] : hc)
+                                 : (c > 0 ? r[1][c > 0 ? c - 1 : c] : hc);
+          const double py = (yv < 0.0) ? r[2][c] : r[0][c];
+          out = fc + ax_ * (px - fc) + ay_ * (py - fc);
+        } else if (xr < 0 || yr < 0) {
+          out = 0.0;
+        } else {
+          out = fc;
+          if (xr != 1) {
+            const double gx = ghost(p, f, wallM, ratio, spec, N, xr == 0 ? 1 : 0, ix, iy, l, idx);
+            out = out + ax_ * (gx - fc);
+          }
+          if (yr != 1) {
+            const int side = yr == 0 ? 3 : 2;
+            const bool wall = (side == 2) ? iy == 0 : iy == p.ny - 1;
+            const double gy = wall ? ghost(p, f, wallM, ratio, spec, N, side, ix, iy, l, idx)
+                                   : (side == 2 ? r[0][c] : r[2][c]);
+            out = out + ay_ * (gy - fc);
+          }
+        }
+        fo[idx] = out;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d <= D; ++d) {
+#pragma unroll
+      for (int c = 0; c < S; ++c) r[d][c] = r[d + 1][c];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) h[d] = h[d + 1];
+  }
+}
+
+// Host launch of the 2-D sweep.
+void launch_convect2d(const Grid& g, const ConvParams& p, const double* f, double* fo, const double* wallM,
+                      const double* ratio, const double* spec, hipStream_t st) {
+  constexpr int S = kConvStrip, D = kConvDepth;
+  // node chunks padded to a multiple of 8 (one XCD per chunk index)
+  const unsigned chunks = (unsigned)(((g.N + kConvBS - 1) / kConvBS + 7) / 8 * 8);
+  const int strips = (p.nx + S - 1) / S;
+  const int segs = (p.ny + kConvSegRows - 1) / kConvSegRows;
+  hipLaunchKernelGGL((k_convect2d<S, D>), dim3(chunks, strips, segs), dim3(kConvBS), 0, st, g, p, f, fo, wallM,
+                     ratio, spec, kConvSegRows);
 }
 
 // 1-D sweep of the zone-grid solver (grid_ref_code convection.f90:111-256)
