@@ -1265,7 +1265,7 @@ __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, con
   const int y0 = blockIdx.z * seg_rows;
   if (l >= N || ix0 >= p.nx || y0 >= p.ny) return;
   const int y1 = (p.ny - y0 < seg_rows) ? p.ny : y0 + seg_rows;  // rows [y0, y1)
-  const int ylim = (y1 < p.ny) ? y1 + 1 : p.ny;                    // rows read: [y0 - 1, ylim)
+  const int ylim = (y1 < p.ny) ? y1 + 1 : p.ny;                    // rows [y0 - 1, ylim) around the segment
   int i, j, k;
   node_ijk(g, l, i, j, k);
   const double xv = g.ax[0][i], yv = g.ax[1][j];
@@ -1282,25 +1282,33 @@ __global__ void __launch_bounds__(kConvBS) k_convect2d(Grid g, ConvParams p, con
   const bool hval = hx >= 0 && hx < p.nx;
   const long long hoff = (long long)(hx - ix0) * N;
   const size_t col0 = (size_t)ix0 * N + l;          // column ix0 of row 0
+  // the row below the segment is the y-upwind row of y >= 0 nodes, the row
+  // above it that of y < 0 nodes; each node loads only its own
+  const bool yneg = yv < 0.0;
+  const int ylo = yneg ? y0 : y0 - 1;
+  const int yhi = yneg ? ylim : y1;                 // rows read: [ylo, yhi)
   // r[0] row iy-1, r[1] row iy, r[1 + d] row iy + d; h[d] the halo of row iy + d
   double r[D + 2][S], h[D + 1];
 #pragma unroll
-  for (int c = 0; c < S; ++c) r[0][c] = (y0 > 0 && c < ns) ? f[col0 + (size_t)(y0 - 1) * row + (size_t)c * N] : 0.0;
+  for (int c = 0; c < S; ++c)
+    r[0][c] = (ylo < y0 && y0 > 0 && c < ns) ? f[col0 + (size_t)(y0 - 1) * row + (size_t)c * N] : 0.0;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const int yl = y0 + d;
     const size_t rb = col0 + (size_t)yl * row;
+    const bool ld = yl < yhi;
 #pragma unroll
-    for (int c = 0; c < S; ++c) r[1 + d][c] = (yl < ylim && c < ns) ? f[rb + (size_t)c * N] : 0.0;
-    h[d] = (yl < ylim && hval) ? f[(long long)rb + hoff] : 0.0;
+    for (int c = 0; c < S; ++c) r[1 + d][c] = (ld && c < ns) ? f[rb + (size_t)c * N] : 0.0;
+    h[d] = (yl < y1 && hval) ? f[(long long)rb + hoff] : 0.0;
   }
   for (int iy = y0; iy < y1; ++iy) {
     {
       const int yl = iy + D;
       const size_t rb = col0 + (size_t)yl * row;
+      const bool ld = yl < yhi;
 #pragma unroll
-      for (int c = 0; c < S; ++c) r[D + 1][c] = (yl < ylim && c < ns) ? f[rb + (size_t)c * N] : 0.0;
-      h[D] = (yl < ylim && hval) ? f[(long long)rb + hoff] : 0.0;
+      for (int c = 0; c < S; ++c) r[D + 1][c] = (ld && c < ns) ? f[rb + (size_t)c * N] : 0.0;
+      h[D] = (yl < y1 && hval) ? f[(long long)rb + hoff] : 0.0;
     }
     const size_t rb = col0 + (size_t)iy * row;
     const double hc = h[0];

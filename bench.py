@@ -107,6 +107,51 @@ def cpu_baseline(pkg, cfg_full, steps=12):
                       f"{cfg.samples_per_cell} samples/cell, {steps} full steps, {dt:.1f} s"}
 
 
+def cpu_baseline_threads(pkg, cfg_full, steps=6):
+    """The same bounded sample on every host core the process may use (at
+    most 16): one oracle context per thread, each stepping its own 16x16-cell
+    cavity (the oracle keeps per-context scratch, so contexts are the unit of
+    parallelism); ctypes releases the GIL during each step. Rate in
+    cell-node updates/s over all threads."""
+    import threading
+    from tests._oracle import oracle_solver
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    cfg, f = pkg.lid_driven_cavity(nx=16, ny=16, n_v=cfg_full.nvx, vmax=float(cfg_full.vx[-1]),
+                                   samples=cfg_full.samples_per_cell, kn=cfg_full.kn)
+    solvers = [oracle_solver(cfg) for _ in range(threads)]
+    for s in solvers:
+        s.initialize(f)
+    errors = []
+
+    def run(s):
+        try:
+            for n in range(1, steps + 1):
+                s.step(n)
+        except Exception as e:  # reported below; a failed thread voids the figure
+            errors.append(e)
+
+    ts = [threading.Thread(target=run, args=(s,)) for s in solvers]
+    t0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    dt = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    if errors:
+        raise errors[0]
+    return {"value": threads * cfg.nx * cfg.ny * cfg.num_points * steps / dt, "unit": "cell-node updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{threads} threads, each the CPU oracle (serial C, -O2) on its own 16x16-cell cavity, "
+                      f"{cfg.nvx}^3 nodes, {cfg.samples_per_cell} samples/cell, {steps} full steps, {dt:.1f} s "
+                      f"({cores} cores in the affinity mask)"}
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of a kernel from the table of rocprofv3 PMC
     measurements (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration),
@@ -346,6 +391,7 @@ def main():
               file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pkg, cfg)
+        out["cpu_baseline"]["all_cores"] = cpu_baseline_threads(pkg, cfg)
     print(json.dumps(out))
 
 
