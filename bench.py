@@ -68,6 +68,38 @@ def copy_calibration(nbytes=512 << 20):
     hip.hipFree(b)
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """HBM bandwidth of a plain device-to-device copy on this GPU (hipMemcpyAsync
+    of 1 GiB, timed with HIP events): 2 x bytes / time, in GB/s. The practical
+    ceiling for a read-once / write-once kernel beside the 8 TB/s nominal."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+        assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+        assert hip.hipMemcpyAsync(b, a, nbytes, 3, None) == 0  # warm-up, hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            assert hip.hipMemcpyAsync(b, a, nbytes, 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def run_gpu(pkg, cfg, f, steps, warmup):
     """Times `steps` fused steps after `warmup` ones; returns wall seconds, mean
     per-phase device ms (HIP events on the library's stream) and the moments."""
@@ -354,6 +386,9 @@ def main():
         "roofline": roofline("k_collide", workload, COLLIDE_BYTES_PER_NODE * ncell * N, phase["collide"]),
         "roofline_sweep": roofline("k_convect2d", workload, SWEEP_BYTES_PER_NODE * ncell * N, phase["sweep"]),
     }
+    copy_gbs = copy_bandwidth()
+    out["device_copy_gbs"] = copy_gbs
+    out["roofline_sweep"]["frac_of_device_copy"] = out["roofline_sweep"]["achieved"] / copy_gbs
     if "fp64" in runs and "fp32" in runs:
         w64 = runs["fp64"][0]
         out["fp64_path"] = {"value": updates / w64, "ms_per_step": w64 / args.steps * 1e3,
