@@ -10,13 +10,14 @@
 // admitted and retired between steps while the others stay resident.
 //
 // One step of every active slot:
-//   k_zero_int         the work-queue head (a fill kernel, graph-safe)
+//   k_zero_int         the work-queue heads (a fill kernel, graph-safe)
 //   k_cdf_cells        one workgroup per resident cell
 //   k_prep_cells       fd = f (FP64) or dacc = 0 (FP32) for cells that take
 //                      global atomics (window smaller than the grid, or
 //                      samples split over several work items)
 //   k_collide_slots    persistent workgroups pull work items (a cell's sample
-//                      range) from a queue sorted by cost; each accumulates in
+//                      range) from per-XCD queues, costliest cells first, a
+//                      cell's items in one queue; each accumulates in
 //                      an LDS window (the whole grid when it fits, else a box
 //                      around the cell's bulk velocity, spilling the rest to
 //                      global FP64 atomics)
@@ -66,11 +67,15 @@ struct WorkItem {
   int slot, cell, q0, q1;  // samples [q0, q1) of the cell (q1 clamped to its count)
 };
 
+// The collision work items form kXcds queues, one per XCD: queue x is
+// items[qstart[x], qstart[x + 1]) and its head is queue[x].
+constexpr int kXcds = 8;
+
 struct BatchPools {
   double* f[2];
   double *eqv, *dacc, *cdf, *eqpar, *props, *factors, *inflow;
   int2* tabs;
-  int *counts, *err, *queue;
+  int *counts, *err, *queue, *qstart;
 };
 
 __device__ __forceinline__ size_t cell_node0(const SlotDesc& d, int cell) { return (size_t)d.off + (size_t)cell * d.g.N; }
@@ -119,24 +124,34 @@ __global__ void k_prep_cells(const SlotDesc* __restrict__ slots, const SlotState
   }
 }
 
-// Persistent collision workgroups: each takes the next work item (costliest
-// first) until the queue is drained; every workgroup reaches the exit once the
-// head passes n_items.
+// Persistent collision workgroups: each takes the next work item of its home
+// queue (the queue of the XCD it runs on: workgroups are dispatched to the
+// XCDs round-robin) until that queue is drained, then of the other queues in
+// turn; every workgroup reaches the exit once it has found all kXcds queues
+// drained. A cell's items sit in one queue, so the workgroups that share its
+// CDFs share one L2.
 template <bool kF32, int BS, int K>
 __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict__ slots,
                                                       const SlotState* __restrict__ state,
-                                                      const WorkItem* __restrict__ items, int n_items,
-                                                      CollParams base, BatchPools P) {
+                                                      const WorkItem* __restrict__ items, CollParams base,
+                                                      BatchPools P) {
   extern __shared__ double dyn[];
   __shared__ ConserveShared<BS> csh;
   __shared__ int s_item;
+  const int home = (int)(blockIdx.x % kXcds);
+  int visited = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(P.queue, 1);
+    const int qx = (home + visited) % kXcds;
+    if (threadIdx.x == 0) s_item = atomicAdd(P.queue + qx, 1);
     __syncthreads();
     // uniform across the workgroup: the item, its slot descriptor, pointers
     // and window bounds then live in scalar registers
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    if (it >= n_items) return;
+    const int it = P.qstart[qx] + __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= P.qstart[qx + 1]) {
+      __syncthreads();  // every lane has read s_item before the next head is taken
+      if (++visited == kXcds) return;
+      continue;
+    }
     const WorkItem w = items[it];
     const SlotDesc& d = slots[w.slot];
     Grid g = d.g;
@@ -407,7 +422,8 @@ extern "C" void dvm_batch_destroy(dvm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   void* ptrs[] = {b->d_slots, b->d_state, b->d_axes, b->P.f[0], b->P.f[1], b->P.eqv, b->P.dacc, b->P.cdf,
                   b->P.eqpar, b->P.props, b->P.factors, b->P.inflow, b->P.tabs, b->P.counts, b->P.err,
-                  b->P.queue, b->d_cells, b->d_cells1d, b->d_adm, b->d_active, b->d_active0, b->d_items, b->d_l1};
+                  b->P.queue, b->P.qstart, b->d_cells, b->d_cells1d, b->d_adm, b->d_active, b->d_active0,
+                  b->d_items, b->d_l1};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -444,7 +460,8 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
       (rc = balloc(b, &b->P.cdf, 2 * (NC + CC))) || (rc = balloc(b, &b->P.eqpar, CC * 5)) ||
       (rc = balloc(b, &b->P.props, CC * DVM_NPROP)) || (rc = balloc(b, &b->P.factors, CC * 3)) ||
       (rc = balloc(b, &b->P.tabs, CC * 2 * kNBucket)) || (rc = balloc(b, &b->P.counts, CC * 4)) ||
-      (rc = balloc(b, &b->P.err, 4)) || (rc = balloc(b, &b->P.queue, 4)) || (rc = balloc(b, &b->d_cells, CC)) ||
+      (rc = balloc(b, &b->P.err, 4)) || (rc = balloc(b, &b->P.queue, kXcds)) ||
+      (rc = balloc(b, &b->P.qstart, kXcds + 1)) || (rc = balloc(b, &b->d_cells, CC)) ||
       (rc = balloc(b, &b->d_cells1d, CC)) || (rc = balloc(b, &b->d_adm, CC)) ||
       (rc = balloc(b, &b->d_active, max_slots)) || (rc = balloc(b, &b->d_active0, max_slots)) ||
       (rc = balloc(b, &b->d_l1, max_slots)))
@@ -630,11 +647,13 @@ extern "C" int dvm_batch_retire(dvm_batch* b, int32_t slot) {
 static int batch_lists(dvm_batch* b) {
   std::vector<CellRef> cl, cl1;
   std::vector<int> act, act0;
-  struct Costed {
+  // work items of each cell, contiguous in `items`
+  struct CellWork {
     long long cost;
-    WorkItem w;
+    int first, count;
   };
-  std::vector<Costed> items;
+  std::vector<WorkItem> items;
+  std::vector<CellWork> cw;
   const bool explicit_mode = b->species.sample_mode == DVM_SAMPLES_EXPLICIT;
   b->max_n = 0;
   b->max_n1d = 0;
@@ -670,18 +689,39 @@ static int batch_lists(dvm_batch* b) {
     for (int c = 0; c < h.ncell; ++c) {
       cl.push_back(CellRef{s, c});
       if (d.convect) cl1.push_back(CellRef{s, c});
+      cw.push_back(CellWork{est, (int)items.size(), d.split});
       for (int q = 0; q < d.split; ++q) {
         const long long q0 = (long long)q * step;
         const bool last = q == d.split - 1;
-        const long long cost = d.split == 1 ? est : (last ? est - q0 : step);
-        items.push_back({cost, WorkItem{s, c, (int)q0, last ? 0x7fffffff : (int)(q0 + step)}});
+        items.push_back(WorkItem{s, c, (int)q0, last ? 0x7fffffff : (int)(q0 + step)});
       }
     }
     if (d.convect) b->max_n1d = h.N > b->max_n1d ? h.N : b->max_n1d;
   }
-  std::stable_sort(items.begin(), items.end(), [](const Costed& x, const Costed& y) { return x.cost > y.cost; });
-  std::vector<WorkItem> wi(items.size());
-  for (size_t q = 0; q < items.size(); ++q) wi[q] = items[q].w;
+  // cells to the XCD queues, costliest first, each to the least loaded queue
+  // (longest-processing-time balancing); a queue lists its cells' items in
+  // that order, so the costliest items run first
+  std::stable_sort(cw.begin(), cw.end(), [](const CellWork& x, const CellWork& y) { return x.cost > y.cost; });
+  std::vector<int> home(cw.size());
+  {
+    long long load[kXcds] = {0};
+    for (size_t q = 0; q < cw.size(); ++q) {
+      int x = 0;
+      for (int y = 1; y < kXcds; ++y)
+        if (load[y] < load[x]) x = y;
+      home[q] = x;
+      load[x] += cw[q].cost;
+    }
+  }
+  std::vector<WorkItem> wi;
+  wi.reserve(items.size());
+  int qstart[kXcds + 1];
+  for (int x = 0; x < kXcds; ++x) {
+    qstart[x] = (int)wi.size();
+    for (size_t q = 0; q < cw.size(); ++q)
+      if (home[q] == x) wi.insert(wi.end(), items.begin() + cw[q].first, items.begin() + cw[q].first + cw[q].count);
+  }
+  qstart[kXcds] = (int)wi.size();
   if ((long long)wi.size() > b->items_cap) {
     if (b->d_items) BHIP(b, hipFree(b->d_items));
     b->d_items = nullptr;
@@ -697,6 +737,7 @@ static int batch_lists(dvm_batch* b) {
   if (!act0.empty())
     BHIP(b, hipMemcpyAsync(b->d_active0, act0.data(), act0.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
   if (!wi.empty()) BHIP(b, hipMemcpyAsync(b->d_items, wi.data(), wi.size() * sizeof(WorkItem), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->P.qstart, qstart, sizeof(qstart), hipMemcpyHostToDevice, b->stream));
   // descriptors carry the split chosen above
   BHIP(b, hipMemcpyAsync(b->d_slots, b->slots.data(), b->slots.size() * sizeof(SlotDesc), hipMemcpyHostToDevice,
                          b->stream));
@@ -720,7 +761,7 @@ static int batch_lists(dvm_batch* b) {
 // Enqueue one step of every resident slot on the batch stream.
 static int batch_enqueue_step(dvm_batch* b) {
   const bool f32 = batch_f32(b);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, b->stream, b->P.queue, 1);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, b->stream, b->P.queue, kXcds);
   hipLaunchKernelGGL(k_cdf_cells, dim3(b->n_cells), dim3(kCdfBS), 0, b->stream, b->d_slots, b->d_state, b->d_cells,
                      b->cdfp, b->P);
   if (b->any_global) {
@@ -733,10 +774,9 @@ static int batch_enqueue_step(dvm_batch* b) {
     const SlotDesc* sl = b->d_slots;
     const SlotState* st = b->d_state;
     const WorkItem* it = b->d_items;
-    int n_items = b->n_items;
     CollParams cp = b->collp;
     BatchPools P = b->P;
-    void* args[] = {&sl, &st, &it, &n_items, &cp, &P};
+    void* args[] = {&sl, &st, &it, &cp, &P};
     BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
                             b->stream));
   }
