@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
     const int bad = collide_item<kF32, BS, K>(g, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
-                                              P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh);
+                                              P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed});
     if (bad) atomicOr(P.err, 1);
     __syncthreads();  // LDS axes, accumulator and s_item are reused by the next item
   }

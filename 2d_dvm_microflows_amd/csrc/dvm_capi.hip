@@ -50,6 +50,11 @@ struct dvm_ctx {
   int heat_ntime = -1;           // step of the values in d_heat
   double* d_heat = nullptr;      // [4][max(nx,ny)] wall heat fluxes
   double* dacc = nullptr;        // [C][N] collision change of the FP32 path
+  // explicit uniform list (dvm_set_uniform_stream): su_n values, read from
+  // su_cursor on; sbase [C] per-cell offsets of the current collision
+  double* d_su = nullptr;
+  long long su_n = 0, su_cursor = 0;
+  long long *d_sbase = nullptr, *d_stotal = nullptr;
   bool f_valid = false, props_valid = false;
   int trace_ready = -1;
   char msg[512] = {0};
@@ -159,7 +164,8 @@ extern "C" const char* dvm_last_error(const dvm_ctx* ctx) { return ctx ? ctx->ms
 extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_heat, c->d_axes, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->eqpar, c->props,
-                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc};
+                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc,
+                  c->d_su, c->d_sbase, c->d_stotal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& row : c->ev)
@@ -171,9 +177,24 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
 
 template <int BS, int K>
 static const void* collide_kernel_bsk(bool lds, bool f32) {
-  if (!lds) return f32 ? (const void*)k_collide<false, true, BS, K> : (const void*)k_collide<false, false, BS, K>;
-  return f32 ? (const void*)k_collide<true, true, BS, K> : (const void*)k_collide<true, false, BS, K>;
+  if (!lds)
+    return f32 ? (const void*)k_collide<false, true, BS, K, false> : (const void*)k_collide<false, false, BS, K, false>;
+  return f32 ? (const void*)k_collide<true, true, BS, K, false> : (const void*)k_collide<true, false, BS, K, false>;
 }
+
+// The collision kernel of the uniform-stream parity mode (kCollBS lanes, one
+// pick per lane).
+static const void* collide_stream_kernel(const dvm_ctx* c) {
+  const bool f32 = c->cfg.precision == DVM_PREC_FP32;
+  if (!c->lds_collide)
+    return f32 ? (const void*)k_collide<false, true, kCollBS, 1, true>
+               : (const void*)k_collide<false, false, kCollBS, 1, true>;
+  return f32 ? (const void*)k_collide<true, true, kCollBS, 1, true> : (const void*)k_collide<true, false, kCollBS, 1, true>;
+}
+
+// Uniforms per sample: the two partner uniforms, then (phi, theta) per
+// replenish location.
+static int stream_per_sample(const dvm_ctx* c) { return 2 + 2 * c->cfg.num_repl; }
 
 template <int BS>
 static const void* collide_kernel_bs(int k, bool lds, bool f32) {
@@ -391,6 +412,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   {
     const void* kern = collide_kernel(c);
     HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
+    HIP_TRY(c, hipFuncSetAttribute(collide_stream_kernel(c), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)c->coll_lds));
   }
   // ---- convection constants
   ConvParams& p = c->convp;
@@ -606,10 +629,31 @@ static bool separate_conserve(const dvm_ctx* c) {
 
 static int collide_async(dvm_ctx* c, int ntime) {
   if (!collides(c)) return DVM_OK;
-  void* args[] = {&c->g,    &c->collp, &ntime,   &c->split, &c->wd,     &c->props,   &c->f,  &c->fd,
-                  &c->dacc, &c->eqv,   &c->cdf,  &c->tabs,  &c->counts, &c->factors, &c->err};
-  HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(c->coll_bs), args,
-                             c->coll_lds, c->stream));
+  const bool stream = c->d_su != nullptr;
+  int per = stream_per_sample(c);
+  const double* su = c->d_su;
+  const long long* sbase = c->d_sbase;
+  if (stream) {
+    // offsets of every cell's samples in the list; the step must fit in what
+    // is left of it (a host sync: this is the parity mode)
+    hipLaunchKernelGGL(k_stream_bases, dim3(1), dim3(64), 0, c->stream, c->counts, c->ncell, c->su_cursor, per,
+                       c->d_sbase, c->d_stotal);
+    long long need = 0;
+    HIP_TRY(c, hipMemcpyAsync(&need, c->d_stotal, sizeof need, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (need > c->su_n - c->su_cursor)
+      return fail(c, DVM_E_ARG, "uniform stream: the step reads %lld values, %lld are left of %lld", need,
+                  c->su_n - c->su_cursor, c->su_n);
+    c->su_cursor += need;
+  }
+  void* args[] = {&c->g,    &c->collp, &ntime,  &c->split,  &c->wd,      &c->props, &c->f, &c->fd, &c->dacc,
+                  &c->eqv,  &c->cdf,   &c->tabs, &c->counts, &c->factors, &c->err,   &su,   &sbase, &per};
+  if (stream)
+    HIP_TRY(c, hipLaunchKernel(collide_stream_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(kCollBS), args,
+                               c->coll_lds, c->stream));
+  else
+    HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(c->coll_bs), args,
+                               c->coll_lds, c->stream));
   if (!separate_conserve(c)) {
     double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
   }
@@ -859,9 +903,29 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
       return rc;
     }
     const unsigned blocks = (unsigned)((nrec + kCollBS - 1) / kCollBS);
-    hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kCollBS), 0, c->stream, c->g, c->collp, ntime, cell, c->f, c->eqv,
-                       c->cdf, c->tabs, c->counts, c->factors, (long long)nrec, d_i, d_d);
-    hipError_t e = hipGetLastError();
+    hipError_t e = hipSuccess;
+    if (c->d_su) {
+      // the traced cell reads the list from the cursor, which does not move
+      const int per = stream_per_sample(c);
+      const long long base = c->su_cursor;
+      e = hipMemcpyAsync(c->d_sbase + cell, &base, sizeof base, hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess && total * per > c->su_n - c->su_cursor) {
+        (void)hipFree(d_i);
+        (void)hipFree(d_d);
+        return fail(c, DVM_E_ARG, "uniform stream: the trace reads %lld values, %lld are left", total * per,
+                    c->su_n - c->su_cursor);
+      }
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(kCollBS), 0, c->stream, c->g, c->collp, ntime, cell, c->f,
+                           c->eqv, c->cdf, c->tabs, c->counts, c->factors, (long long)nrec, d_i, d_d, c->d_su,
+                           c->d_sbase, per);
+    } else {
+      hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(kCollBS), 0, c->stream, c->g, c->collp, ntime, cell, c->f,
+                         c->eqv, c->cdf, c->tabs, c->counts, c->factors, (long long)nrec, d_i, d_d,
+                         (const double*)nullptr, (const long long*)nullptr, 0);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(irec, d_i, (size_t)32 * nrec * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(drec, d_d, (size_t)16 * nrec * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -873,5 +937,33 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
   return DVM_OK;
 }
 
+
+extern "C" int dvm_set_uniform_stream(dvm_ctx* c, const double* u, int64_t n) {
+  if (!c) return fail(c, DVM_E_ARG, "null context");
+  if (n < 0 || (n > 0 && !u)) return fail(c, DVM_E_ARG, "uniform stream: bad length or null pointer");
+  for (int64_t q = 0; q < n; ++q)
+    if (!(u[q] >= 0.0 && u[q] < 1.0)) return fail(c, DVM_E_ARG, "uniform stream: u[%lld] = %g is not in [0, 1)",
+                                                 (long long)q, u[q]);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));  // queued collisions may still read the old list
+  if (c->d_su) (void)hipFree(c->d_su);
+  c->d_su = nullptr;
+  c->su_n = 0;
+  c->su_cursor = 0;
+  if (n == 0) return DVM_OK;  // back to Philox
+  if (!c->d_sbase) {
+    if (int rc = dalloc(c, &c->d_sbase, (size_t)c->ncell)) return rc;
+    if (int rc = dalloc(c, &c->d_stotal, 1)) return rc;
+  }
+  if (int rc = dalloc(c, &c->d_su, (size_t)n)) return rc;
+  HIP_TRY(c, hipMemcpy(c->d_su, u, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  c->su_n = n;
+  return DVM_OK;
+}
+
+extern "C" int dvm_uniform_stream_position(dvm_ctx* c, int64_t* consumed) {
+  if (!c || !consumed) return fail(c, DVM_E_ARG, "null argument");
+  *consumed = c->su_cursor;
+  return DVM_OK;
+}
 
 #include "dvm_batch.hip"

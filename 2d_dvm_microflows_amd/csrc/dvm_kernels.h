@@ -77,11 +77,15 @@ __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const 
                       double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
                       double* __restrict__ eqpar,
                       int* __restrict__ counts, double* __restrict__ factors);
+template <bool kStream>
 __global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
                         const double* __restrict__ eqv, const double* __restrict__ cdf,
                         const int2* __restrict__ tabs, const int* __restrict__ counts,
                         const double* __restrict__ factors, long long max_records, int* __restrict__ irec,
-                        double* __restrict__ drec);
+                        double* __restrict__ drec, const double* __restrict__ su,
+                        const long long* __restrict__ sbase, int sper);
+__global__ void k_stream_bases(const int* __restrict__ counts, int ncell, long long cursor, int per,
+                               long long* __restrict__ base, long long* __restrict__ total);
 __global__ void k_remap_batch(Grid g, long long n, const double* __restrict__ uvw, double f_off,
                               int* __restrict__ st, double* __restrict__ m);
 __global__ void k_wall_ratio(Grid g, ConvParams p, const double* __restrict__ f, double* __restrict__ ratio);

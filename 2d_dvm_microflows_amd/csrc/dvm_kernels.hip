@@ -521,16 +521,54 @@ __device__ __forceinline__ R rpow(R x, R y) {
   if constexpr (sizeof(R) == 4) return powf(x, y); else return pow(x, y);
 }
 
+// Uniforms of one sample, in the reference's draw order: the two partner
+// uniforms (pick_collision_partner), then (phi, theta) of each replenish
+// location r = 1..NRepl (pick_random_velocity_on_sphere). q is the sample's
+// index in its cell over the three integrals, (t, s) its integral and index
+// in it.
+// PhiloxSrc: counter-based Philox4x32-10, one block (s, t + 1 | r << 2, cell,
+// ntime) per location r, the first one also carrying the partner uniforms
+// (words 0-1) before (phi, theta) (words 2-3), each uniform word * 2^-32.
+// pick_batch and vr_sample spell these steps out for it, with the seed of
+// CollParams.
+struct PhiloxSrc {
+  uint64_t seed;
+};
+
+// StreamSrc: an explicit list of uniforms (the parity mode of
+// dvm_set_uniform_stream): sample q of a cell reads the `per` = 2 + 2 NRepl
+// values u[base[cell] + q * per ...] in the order above.
+struct StreamSrc {
+  const double* u;
+  const long long* base;
+  int per;
+  __device__ __forceinline__ void pair(int cell, int ntime, int q, int t, int s, double& uA, double& uB,
+                                       uint32_t& wa, uint32_t& wb) const {
+    const double* p = u + base[cell] + (long long)q * per;
+    uA = p[0];
+    uB = p[1];
+    wa = 0u;
+    wb = 0u;
+  }
+  __device__ __forceinline__ void angles(int cell, int ntime, int q, int t, int s, int r, uint32_t wa, uint32_t wb,
+                                         double& uphi, double& uth) const {
+    const double* p = u + base[cell] + (long long)q * per + 2 * r;
+    uphi = p[0];
+    uth = p[1];
+  }
+};
+
 // Partner picks of K samples (pick_collision_partner, collision_utilities.f90:
 // 142-186, with referenced_search, sort_and_search.f90:40-86). The 2K
 // inverse-CDF searches advance in lockstep so that their dependent loads from
 // the cell's CDFs (L2/MALL resident, not LDS) overlap; each search follows the
 // reference's bucket bracket and bisection exactly. Also returns the sign of
 // the deviation at each deviation partner (+1 for an equilibrium partner).
-template <int K>
+template <int K, class Src>
 __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView& cv, int N, int cell, int ntime,
-                                           const int* t, const int* s, const bool* valid, int* lA, int* lB,
-                                           double* sgA, double* sgB, uint32_t* wphi, uint32_t* wth) {
+                                           const int* q, const int* t, const int* s, const bool* valid, int* lA,
+                                           int* lB, double* sgA, double* sgB, uint32_t* wphi, uint32_t* wth,
+                                           const Src& src) {
   constexpr int M = 2 * K;
   // the cell's two CDFs are adjacent: eq at cv.eqc, deviation at cv.eqc + N + 1
   int off[M], left[M], right[M], idx[M];
@@ -539,11 +577,21 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
   int2 lr[M];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    uint32_t w4[4] = {0u, 0u, 0u, 0u};
-    if (valid[k]) philox_block(cp.seed, (uint32_t)s[k], (uint32_t)(t[k] + 1), (uint32_t)cell, (uint32_t)ntime, w4);
-    const double uA = u32_to_uniform(w4[0]), uB = u32_to_uniform(w4[1]);
-    wphi[k] = w4[2];
-    wth[k] = w4[3];
+    double uA, uB;
+    if constexpr (std::is_same<Src, PhiloxSrc>::value) {
+      uint32_t w4[4] = {0u, 0u, 0u, 0u};
+      if (valid[k]) philox_block(cp.seed, (uint32_t)s[k], (uint32_t)(t[k] + 1), (uint32_t)cell, (uint32_t)ntime, w4);
+      uA = u32_to_uniform(w4[0]);
+      uB = u32_to_uniform(w4[1]);
+      wphi[k] = w4[2];
+      wth[k] = w4[3];
+    } else {
+      uA = 0.0;
+      uB = 0.0;
+      wphi[k] = 0u;
+      wth[k] = 0u;
+      if (valid[k]) src.pair(cell, ntime, q[k], t[k], s[k], uA, uB, wphi[k], wth[k]);
+    }
     const bool eqA = t[k] == 0, eqB = t[k] == 1;
     off[2 * k] = eqA ? 0 : N + 1;
     off[2 * k + 1] = eqB ? 0 : N + 1;
@@ -607,10 +655,11 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
 // R is the collision arithmetic (double, or float for the FP32-collision path);
 // the partner picks, the deviation signs and the remap weights are FP64 in both.
 // scatter(node, value) receives the two depletions and every stencil point.
-template <bool kRecord, class R, class Scatter>
-__device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, int cell, int ntime, int t, int s,
-                                         double factor, int lA, int lB, double signA_raw, double signB_raw,
-                                         uint32_t wphi, uint32_t wth, Scatter&& scatter, SampleRec* rec) {
+template <bool kRecord, class R, class Src, class Scatter>
+__device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, int cell, int ntime, int q, int t,
+                                         int s, double factor, int lA, int lB, double signA_raw, double signB_raw,
+                                         uint32_t wphi, uint32_t wth, const Src& src, Scatter&& scatter,
+                                         SampleRec* rec) {
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
   if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) return 1;  // unreachable for finite CDFs
   int iA, jA, kA, iB, jB, kB;
@@ -651,12 +700,18 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
   const R gr1 = rdiv<R>(gp1, gg), gr2 = rdiv<R>(gp2, gg);
   int bad = 0;
   for (int r = 1; r <= cp.num_repl; ++r) {
-    double uphi = u32_to_uniform(wphi), uth = u32_to_uniform(wth);
-    if (r > 1) {
-      uint32_t w4[4];
-      philox_block(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, w4);
-      uphi = u32_to_uniform(w4[0]);
-      uth = u32_to_uniform(w4[1]);
+    double uphi, uth;
+    if constexpr (std::is_same<Src, PhiloxSrc>::value) {
+      uphi = u32_to_uniform(wphi);
+      uth = u32_to_uniform(wth);
+      if (r > 1) {
+        uint32_t w4[4];
+        philox_block(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, w4);
+        uphi = u32_to_uniform(w4[0]);
+        uth = u32_to_uniform(w4[1]);
+      }
+    } else {
+      src.angles(cell, ntime, q, t, s, r, wphi, wth, uphi, uth);
     }
     // pick_random_velocity_on_sphere (collision_utilities.f90:284-352)
     const R ph = (R)2 * (R)kPiRef * (R)uphi;
@@ -761,31 +816,32 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
 //         the repeated addresses of a cell's distribution peak.)
 // Samples q in [q0, q1) of one cell (q indexes integral 1, then 2, then 3);
 // every depletion and stencil point goes to scatter(node, value).
-template <bool kF32, int BS, int K, class Scatter>
+template <bool kF32, int BS, int K, class Src, class Scatter>
 __device__ __forceinline__ int collide_samples(const Grid& g, const CollParams& cp, const CellView& cv,
                                                int rng_cell, int ntime, int n0, int n1, int q0, int q1,
-                                               const double* __restrict__ fac3, Scatter&& scatter) {
+                                               const double* __restrict__ fac3, const Src& src, Scatter&& scatter) {
   using R = typename std::conditional<kF32, float, double>::type;
   const int N = g.N;
   int bad = 0;
   for (int base = q0; base < q1; base += BS * K) {
-    int tt[K], ss[K], lA[K], lB[K];
+    int qq[K], tt[K], ss[K], lA[K], lB[K];
     bool valid[K];
     double sgA[K], sgB[K];
     uint32_t wphi[K], wth[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int q = base + k * BS + (int)threadIdx.x;
+      qq[k] = q;
       valid[k] = q < q1;
       tt[k] = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
       ss[k] = (tt[k] == 0) ? q : (tt[k] == 1 ? q - n0 : q - n0 - n1);
     }
-    pick_batch<K>(cp, cv, N, rng_cell, ntime, tt, ss, valid, lA, lB, sgA, sgB, wphi, wth);
+    pick_batch<K>(cp, cv, N, rng_cell, ntime, qq, tt, ss, valid, lA, lB, sgA, sgB, wphi, wth, src);
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
       if (!valid[k]) continue;
-      bad |= vr_sample<false, R>(g, cp, rng_cell, ntime, tt[k], ss[k], fac3[tt[k]], lA[k], lB[k], sgA[k], sgB[k],
-                                 wphi[k], wth[k], scatter, nullptr);
+      bad |= vr_sample<false, R>(g, cp, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k], sgA[k],
+                                 sgB[k], wphi[k], wth[k], src, scatter, nullptr);
     }
   }
   return bad;
@@ -934,12 +990,12 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
 //   FP32, full and sole: the whole change is in LDS, so the FP64
 //         conservation correction runs here (csh its reduction space) and
 //         fd = f + corrected change is written directly.
-template <bool kF32, int BS, int K>
+template <bool kF32, int BS, int K, class Src>
 __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp, const CellView& cv, int rng_cell,
                                             int ntime, int q0, int q1, const int* __restrict__ cnt,
                                             const double* __restrict__ fac, const Window& win, bool full, bool sole,
                                             double* acc, double* __restrict__ fd, double* __restrict__ da,
-                                            ConserveShared<BS>* csh) {
+                                            ConserveShared<BS>* csh, const Src& src) {
   const int N = g.N;
   const int total = cnt[0] + cnt[1] + cnt[2];
   q1 = q1 < total ? q1 : total;
@@ -949,14 +1005,14 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
   double* tgt = kF32 ? da : fd;
   int bad;
   if (full) {
-    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac,
+    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int, int, int, int node, double v) {
                                          if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], v);
                                        });
   } else {
     const int o0 = win.o[0], o1 = win.o[1], o2 = win.o[2], wx = win.w[0], wxy = win.wxy;
     const unsigned w0 = win.w[0], w1 = win.w[1], w2 = win.w[2];
-    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac,
+    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int i, int j, int k, int node, double v) {
                                          const unsigned a = i - o0, b = j - o1, c = k - o2;
                                          if (a < w0 && b < w1 && c < w2) atomicAdd(&acc[a + wx * b + wxy * c], v);
@@ -999,14 +1055,18 @@ struct WinDims {
   int w[3];
 };
 
-template <bool kLds, bool kF32, int BS, int K>
+// kStream: the uniforms come from the explicit list su (per-cell offsets sbase,
+// sper values per sample) instead of Philox.
+template <bool kLds, bool kF32, int BS, int K, bool kStream>
 __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, int split, WinDims wd,
                                                 const double* __restrict__ props,
                                                 const double* __restrict__ f, double* __restrict__ fd,
                                                 double* __restrict__ dacc,
                                                 const double* __restrict__ eqv, const double* __restrict__ cdf,
                                                 const int2* __restrict__ tabs, const int* __restrict__ counts,
-                                                const double* __restrict__ factors, int* __restrict__ err) {
+                                                const double* __restrict__ factors, int* __restrict__ err,
+                                                const double* __restrict__ su, const long long* __restrict__ sbase,
+                                                int sper) {
   extern __shared__ double dyn[];
   __shared__ ConserveShared<BS> csh;
   const int cell = blockIdx.x / split, part = blockIdx.x - cell * split;
@@ -1019,9 +1079,12 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   const int* cnt = counts + cell * 4;
   const int chunk = (cnt[0] + cnt[1] + cnt[2] + split - 1) / split;
   const int q0 = part * chunk;
+  using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
+  Src src;
+  if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
   const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
-                                            fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh);
+                                            fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src);
   if (bad) atomicOr(err, 1);
 }
 
@@ -1033,14 +1096,34 @@ __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restric
   conserve_cell<BS>(g, f + off, dacc + off, fd + off, sh);
 }
 
+// Offsets of each cell's samples in an explicit uniform list: cells in order,
+// each taking per * (its sample count) values from `cursor` on; total[0] is
+// the number of values the step reads. One lane (a parity mode, not a hot
+// path).
+__global__ void k_stream_bases(const int* __restrict__ counts, int ncell, long long cursor, int per,
+                               long long* __restrict__ base, long long* __restrict__ total) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  long long acc = 0;
+  for (int c = 0; c < ncell; ++c) {
+    base[c] = cursor + acc * per;
+    acc += counts[c * 4 + 3];
+  }
+  total[0] = acc * per;
+}
+
 // Parity trace of one cell: the same samples as the FP64 k_collide, recorded
 // instead of scattered.
+template <bool kStream>
 __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int ntime, int cell,
                                                    const double* __restrict__ f, const double* __restrict__ eqv,
                                                    const double* __restrict__ cdf, const int2* __restrict__ tabs,
                                                    const int* __restrict__ counts, const double* __restrict__ factors,
                                                    long long max_records, int* __restrict__ irec,
-                                                   double* __restrict__ drec) {
+                                                   double* __restrict__ drec, const double* __restrict__ su,
+                                                   const long long* __restrict__ sbase, int sper) {
+  using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
+  Src src;
+  if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
   const int N = g.N;
   const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
                                 tabs + (size_t)cell * 2 * kNBucket, N);
@@ -1056,9 +1139,10 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     int lA, lB;
     double sgA, sgB;
     uint32_t wphi, wth;
-    pick_batch<1>(cp, cv, N, cell, ntime, &t, &s, &valid, &lA, &lB, &sgA, &sgB, &wphi, &wth);
+    const int qi = q;
+    pick_batch<1>(cp, cv, N, cell, ntime, &qi, &t, &s, &valid, &lA, &lB, &sgA, &sgB, &wphi, &wth, src);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
-    vr_sample<true, double>(g, cp, cell, ntime, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth,
+    vr_sample<true, double>(g, cp, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth, src,
                             [](int, int, int, int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;

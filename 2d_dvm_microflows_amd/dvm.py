@@ -195,6 +195,21 @@ class Solver:
         k = min(n.value, max_records)
         return irec[:k], drec[:k], n.value
 
+    def set_uniform_stream(self, u):
+        """Parity mode: later collisions read their uniforms from the list u
+        (cells in order, samples in order, 2 + 2 NRepl values per sample: the
+        two partner uniforms, then phi and theta per replenish location)
+        instead of the counter-based generator; an empty list returns to it.
+        The library keeps its own copy of the list."""
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        self._check(self._fn("set_uniform_stream")(self.h, _dptr(u) if u.size else None, int(u.size)))
+
+    def uniform_stream_position(self):
+        """Values of the uniform list read so far."""
+        n = C.c_int64(0)
+        self._check(self._fn("uniform_stream_position")(self.h, C.byref(n)))
+        return n.value
+
     PHASES = ("cdf", "prep", "collide", "conserve", "walls", "sweep", "moments")
 
     def timing_reset(self):

@@ -117,6 +117,19 @@ module DVMHip
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: ntime
      end function dvm_step
+     ! parity mode: uniforms from an explicit list (replaces get_rand,
+     ! utils/random_number_generation.f90:140-162, for parity runs)
+     integer(c_int) function dvm_set_uniform_stream(ctx, u, n) bind(c, name="dvm_set_uniform_stream")
+       import :: c_int, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: u(*)
+       integer(c_int64_t), value :: n
+     end function dvm_set_uniform_stream
+     integer(c_int) function dvm_uniform_stream_position(ctx, consumed) bind(c, name="dvm_uniform_stream_position")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: consumed
+     end function dvm_uniform_stream_position
      integer(c_int) function dvm_timing_reset(ctx) bind(c, name="dvm_timing_reset")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

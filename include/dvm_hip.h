@@ -228,6 +228,25 @@ int dvm_remap(dvm_ctx* ctx, int64_t n, const double* uvw, double f_off,
 int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_records,
                       int32_t* irec, double* drec, int64_t* n_out);
 
+/* Parity mode: the collision phase takes its uniforms from the explicit list
+   u[0..n) instead of the counter-based generator, so that one list can be fed
+   to this library and to a CPU implementation. Each collision reads the list
+   from a cursor on: cells in order, in each cell its samples in order
+   (integral 1, 2, 3), 2 + 2*NRepl values per sample: uA, uB (the two partner
+   picks), then (phi, theta) of each replenish location, the order in which the
+   reference draws them (variance_reduction.f90:686-816,
+   replenishing_collisions.f90:134-354). The cursor starts at 0 and advances by
+   what each collision read; a collision that needs more values than are left
+   fails with DVM_E_ARG before it changes anything. dvm_collide_trace reads the
+   traced cell's samples from the cursor without moving it. Values must lie in
+   [0, 1); n = 0 returns to the counter-based generator. In this mode each
+   collision synchronises the host with the device once.
+   Replaces: get_rand (utils/random_number_generation.f90:140-162), the
+   reference's sequential stream, for parity runs. */
+int dvm_set_uniform_stream(dvm_ctx* ctx, const double* u, int64_t n);
+/* Values of the list read so far (the cursor). */
+int dvm_uniform_stream_position(dvm_ctx* ctx, int64_t* consumed);
+
 /* Device time per phase, measured with HIP events on the context's stream
    around every phase that dvm_step / dvm_collide / dvm_convect / dvm_moments
    enqueue. Phases: 0 CDF build, 1 collision prefill (only when a cell's slice

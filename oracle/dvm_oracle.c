@@ -73,6 +73,10 @@ struct orc_ctx {
   /* wall heat fluxes (set_wall_heat_flux_dump_freq, boundary_conditions.f90:108) */
   int heat_every, heat_on, heat_ntime;
   double* heat;       /* [4][max(nx,ny)] */
+  /* explicit uniform list (dvm_set_uniform_stream): su_n values, the next
+     collision reads from su_cursor on; su_base is the current cell's offset */
+  double* su;
+  int64_t su_n, su_cursor, su_base;
   char err[256];
 };
 
@@ -396,7 +400,7 @@ void orc_ctx_destroy(orc_ctx* c) {
   if (!c) return;
   for (int d = 0; d < 3; ++d) { free(c->g.ax[d]); free(c->g.beta[d]); }
   free(c->f); free(c->fd); free(c->props); free(c->eq_df); free(c->eq_par);
-  free(c->eqc); free(c->neqc); free(c->heat);
+  free(c->eqc); free(c->neqc); free(c->heat); free(c->su);
   free(c);
 }
 
@@ -704,7 +708,7 @@ static void fmc_counts(orc_ctx* c, int cell, cell_coll* cc) {
    vr_depletion_routine :1127-1237, replenish_collision
    replenishing_collisions.f90:134-354). Scatters into fd; when rec is
    non-NULL records the sample for the parity trace. Returns 1 if skipped. */
-static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_coll* cc,
+static int vr_sample(orc_ctx* c, int cell, int ntime, int64_t q, int t, int s, const cell_coll* cc,
                      double* fd, int32_t* irec, double* drec) {
   const dvm_config* cfg = &c->cfg;
   const ogrid* g = &c->g;
@@ -712,8 +716,14 @@ static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_c
   int nx = g->n[0], nxy = g->n[0] * g->n[1];
   const double* f = c->f + (size_t)cell * N;
   const double* eq_df = c->eq_df + (size_t)cell * (N + 1);
+  /* the sample's uniforms: uA, uB, then (phi, theta) per replenish location,
+     from one Philox block per location or from the explicit list (sample q of
+     the cell at su_base + q * (2 + 2 NRepl)) */
+  const int per = 2 + 2 * cfg->num_repl;
+  const double* us = c->su ? c->su + c->su_base + q * per : NULL;
   double u0[4];
-  uniforms4(cfg->seed, (uint32_t)s, (uint32_t)(t + 1), (uint32_t)cell, (uint32_t)ntime, u0);
+  if (us) { u0[0] = us[0]; u0[1] = us[1]; u0[2] = us[2]; u0[3] = us[3]; }
+  else uniforms4(cfg->seed, (uint32_t)s, (uint32_t)(t + 1), (uint32_t)cell, (uint32_t)ntime, u0);
   const double uA = u0[0], uB = u0[1];
   /* pick_collision_partner (collision_utilities.f90:142-186) */
   int lA, lB;
@@ -762,7 +772,10 @@ static int vr_sample(orc_ctx* c, int cell, int ntime, int t, int s, const cell_c
   if (irec) { drec[0] = kd1; drec[1] = kd2; }
   for (int r = 1; r <= cfg->num_repl; ++r) {
     double uphi = u0[2], uth = u0[3];
-    if (r > 1) {
+    if (r > 1 && us) {
+      uphi = us[2 * r];
+      uth = us[2 * r + 1];
+    } else if (r > 1) {
       double ur[4];
       uniforms4(cfg->seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, ur);
       uphi = ur[0];
@@ -841,13 +854,22 @@ static int collide_cell(orc_ctx* c, int cell, int ntime, int64_t* nsamp) {
   int N = c->g.num_points;
   cell_coll cc;
   cell_cdf_counts(c, cell, 1, NULL, &cc);
+  if (c->su) {
+    /* the cell's samples read the list from the cursor on */
+    const int64_t need = ((int64_t)cc.num[0] + cc.num[1] + cc.num[2]) * (2 + 2 * c->cfg.num_repl);
+    if (need > c->su_n - c->su_cursor)
+      return set_err(c, DVM_E_ARG, "uniform stream: cell %d reads %lld values, %lld are left", cell,
+                     (long long)need, (long long)(c->su_n - c->su_cursor));
+    c->su_base = c->su_cursor;
+    c->su_cursor += need;
+  }
   double* fd = c->fd + (size_t)cell * N;
   memcpy(fd, c->f + (size_t)cell * N, sizeof(double) * N);
-  int64_t tot = 0;
+  int64_t tot = 0, q = 0;
   for (int t = 0; t < 3; ++t) {
     tot += cc.num[t];
-    for (int s = 0; s < cc.num[t]; ++s) {
-      if (vr_sample(c, cell, ntime, t, s, &cc, fd, NULL, NULL) < 0)
+    for (int s = 0; s < cc.num[t]; ++s, ++q) {
+      if (vr_sample(c, cell, ntime, q, t, s, &cc, fd, NULL, NULL) < 0)
         return set_err(c, DVM_E_REMAP, "invalid remap exterior flag (cell %d)", cell);
     }
   }
@@ -889,13 +911,46 @@ int orc_collide_trace(orc_ctx* c, int32_t ntime, int32_t cell, int64_t max_recor
   cell_coll cc;
   cell_cdf_counts(c, cell, 1, NULL, &cc);
   int64_t q = 0;
+  if (c->su) {
+    /* the traced cell reads the list from the cursor, which does not move */
+    const int64_t need = ((int64_t)cc.num[0] + cc.num[1] + cc.num[2]) * (2 + 2 * c->cfg.num_repl);
+    if (need > c->su_n - c->su_cursor) {
+      memcpy(c->eq_df + (size_t)cell * (N + 1), save, sizeof(double) * (N + 1));
+      memcpy(c->eq_par + (size_t)cell * 5, par, sizeof par);
+      free(save);
+      return set_err(c, DVM_E_ARG, "uniform stream: the trace reads %lld values, %lld are left", (long long)need,
+                     (long long)(c->su_n - c->su_cursor));
+    }
+    c->su_base = c->su_cursor;
+  }
   for (int t = 0; t < 3; ++t)
     for (int s = 0; s < cc.num[t]; ++s, ++q)
-      if (q < max_records) vr_sample(c, cell, ntime, t, s, &cc, NULL, irec + 32 * q, drec + 16 * q);
+      if (q < max_records) vr_sample(c, cell, ntime, q, t, s, &cc, NULL, irec + 32 * q, drec + 16 * q);
   memcpy(c->eq_df + (size_t)cell * (N + 1), save, sizeof(double) * (N + 1));
   memcpy(c->eq_par + (size_t)cell * 5, par, sizeof par);
   free(save);
   *n_out = q;
+  return DVM_OK;
+}
+
+int orc_set_uniform_stream(orc_ctx* c, const double* u, int64_t n) {
+  if (n < 0 || (n > 0 && !u)) return set_err(c, DVM_E_ARG, "uniform stream: bad length or null pointer");
+  for (int64_t q = 0; q < n; ++q)
+    if (!(u[q] >= 0.0 && u[q] < 1.0))
+      return set_err(c, DVM_E_ARG, "uniform stream: u[%lld] = %g is not in [0, 1)", (long long)q, u[q]);
+  free(c->su);
+  c->su = NULL;
+  c->su_n = c->su_cursor = c->su_base = 0;
+  if (n == 0) return DVM_OK;
+  c->su = (double*)malloc(sizeof(double) * (size_t)n);
+  if (!c->su) return set_err(c, DVM_E_NOMEM, "uniform stream: out of memory");
+  memcpy(c->su, u, sizeof(double) * (size_t)n);
+  c->su_n = n;
+  return DVM_OK;
+}
+
+int orc_uniform_stream_position(orc_ctx* c, int64_t* consumed) {
+  *consumed = c->su_cursor;
   return DVM_OK;
 }
 
