@@ -69,6 +69,7 @@ def declare(lib, prefix):
         "step": (C.c_int, [P, I32]),
         "remap": (C.c_int, [P, I64, pd, D, pi32, pd]),
         "collide_trace": (C.c_int, [P, I32, I32, I64, pi32, pd, pi64]),
+        "l1": (C.c_int, [P, pd]),
         "set_uniform_stream": (C.c_int, [P, pd, I64]),
         "uniform_stream_position": (C.c_int, [P, pi64]),
         "set_wall_heat_flux_freq": (C.c_int, [P, I32]),

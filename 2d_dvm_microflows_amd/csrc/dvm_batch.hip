@@ -235,9 +235,8 @@ __global__ void __launch_bounds__(kMomBS) k_moments_cells(const SlotDesc* __rest
   moments_cell(d.g, P.f[cur] + cell_node0(d, r.cell), P.props + ((size_t)d.cell0 + r.cell) * DVM_NPROP, mass, red);
 }
 
-// Distance of a space-homogeneous slot's f from equilibrium: sum |f - M| / n,
-// where M is the discrete Maxwellian (times beta^3) at the slot's own density,
-// velocity and temperature, scaled so that sum M = n. One workgroup per slot.
+// Distance of each space-homogeneous slot's f from equilibrium (l1_cell).
+// One workgroup per slot.
 __global__ void __launch_bounds__(kMomBS) k_l1_slots(const SlotDesc* __restrict__ slots,
                                                      const SlotState* __restrict__ state,
                                                      const int* __restrict__ active, double mass, BatchPools P,
@@ -245,34 +244,8 @@ __global__ void __launch_bounds__(kMomBS) k_l1_slots(const SlotDesc* __restrict_
   __shared__ double red[kMomBS / 64];
   const int s = active[blockIdx.x];
   const SlotDesc& d = slots[s];
-  const Grid& g = d.g;
-  const double* f = P.f[state[s].cur] + d.off;
-  const double* p = P.props + (size_t)d.cell0 * DVM_NPROP;
-  const double n = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W], T = p[DVM_P_TRTEMP];
-  if (!(n > 0.0) || !(T > 0.0)) {
-    if (threadIdx.x == 0) out[s] = -1.0;
-    return;
-  }
-  const double a = mass / T;
-  double sm = 0.0;
-  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    const double dx = g.ax[0][i] - u, dy = g.ax[1][j] - v, dz = g.ax[2][k] - w;
-    sm += exp(-(dx * dx + dy * dy + dz * dz) * a) * g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
-  }
-  const double msum = block_sum<kMomBS>(sm, red);
-  const double scale = n / msum;
-  double e = 0.0;
-  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    const double dx = g.ax[0][i] - u, dy = g.ax[1][j] - v, dz = g.ax[2][k] - w;
-    const double m = scale * exp(-(dx * dx + dy * dy + dz * dz) * a) * g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
-    e += fabs(f[l] - m);
-  }
-  e = block_sum<kMomBS>(e, red);
-  if (threadIdx.x == 0) out[s] = e / n;
+  const double e = l1_cell(d.g, P.f[state[s].cur] + d.off, P.props + (size_t)d.cell0 * DVM_NPROP, mass, red);
+  if (threadIdx.x == 0) out[s] = e;
 }
 
 }  // namespace dvm

@@ -774,6 +774,21 @@ extern "C" int dvm_moments_ext(dvm_ctx* c, double* ext) {
   return DVM_OK;
 }
 
+extern "C" int dvm_l1(dvm_ctx* c, double* l1) {
+  if (!c || !l1) return fail(c, DVM_E_ARG, "null argument");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "L1 before upload");
+  if (int rc = run_phases(c, 0, 6, 6)) return rc;
+  double* d_l1 = nullptr;
+  if (int rc = dalloc(c, &d_l1, (size_t)c->ncell)) return rc;
+  hipLaunchKernelGGL(k_l1, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass, d_l1);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(l1, d_l1, (size_t)c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_l1);
+  if (e != hipSuccess) return fail(c, DVM_E_HIP, "dvm_l1: %s", hipGetErrorString(e));
+  return DVM_OK;
+}
+
 extern "C" int dvm_collide(dvm_ctx* c, int32_t ntime, int64_t* samples_out) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "collide needs f and its moments");

@@ -163,6 +163,46 @@ __device__ __forceinline__ void moments_cell(const Grid& g, const double* __rest
   }
 }
 
+// Distance of one cell's f from equilibrium: sum |f - M| / n, where M is the
+// discrete Maxwellian (times beta^3) at the cell's own density, velocity and
+// temperature p, scaled so that sum M = n; -1 for an empty or cold cell. The
+// reference has no such measure (it is the stopping rule of relaxation runs).
+// One workgroup of kMomBS lanes.
+__device__ __forceinline__ double l1_cell(const Grid& g, const double* __restrict__ f, const double* __restrict__ p,
+                                          double mass, double* red) {
+  const double n = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W], T = p[DVM_P_TRTEMP];
+  if (!(n > 0.0) || !(T > 0.0)) return -1.0;
+  const double a = mass / T;
+  double sm = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(g, l, i, j, k);
+    const double dx = g.ax[0][i] - u, dy = g.ax[1][j] - v, dz = g.ax[2][k] - w;
+    sm += exp(-(dx * dx + dy * dy + dz * dz) * a) * g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
+  }
+  const double msum = block_sum<kMomBS>(sm, red);
+  const double scale = n / msum;
+  double e = 0.0;
+  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+    int i, j, k;
+    node_ijk(g, l, i, j, k);
+    const double dx = g.ax[0][i] - u, dy = g.ax[1][j] - v, dz = g.ax[2][k] - w;
+    const double m = scale * exp(-(dx * dx + dy * dy + dz * dz) * a) * g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
+    e += fabs(f[l] - m);
+  }
+  e = block_sum<kMomBS>(e, red);
+  return e / n;
+}
+
+__global__ void __launch_bounds__(kMomBS) k_l1(Grid g, const double* __restrict__ f,
+                                               const double* __restrict__ props, double mass,
+                                               double* __restrict__ out) {
+  __shared__ double red[kMomBS / 64];
+  const int cell = blockIdx.x;
+  const double e = l1_cell(g, f + (size_t)cell * g.N, props + (size_t)cell * DVM_NPROP, mass, red);
+  if (threadIdx.x == 0) out[cell] = e;
+}
+
 // Output moments of one cell, from f and its moments p (density, velocity,
 // pressure): heat flux q_i = m/2 sum (c_i - u_i)|c - u|^2 f
 // (physical_props.f90:1914-1986), viscous stress

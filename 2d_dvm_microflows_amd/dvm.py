@@ -195,6 +195,13 @@ class Solver:
         k = min(n.value, max_records)
         return irec[:k], drec[:k], n.value
 
+    def l1_to_equilibrium(self):
+        """Per-cell sum |f - M| / n against the discrete Maxwellian at the cell's
+        own moments (dvm_l1)."""
+        out = np.zeros(self.ncell, dtype=np.float64)
+        self._check(self._fn("l1")(self.h, _dptr(out)))
+        return out
+
     def set_uniform_stream(self, u):
         """Parity mode: later collisions read their uniforms from the list u
         (cells in order, samples in order, 2 + 2 NRepl values per sample: the

@@ -117,6 +117,12 @@ module DVMHip
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: ntime
      end function dvm_step
+     ! distance of each cell's f from its equilibrium Maxwellian (no reference counterpart)
+     integer(c_int) function dvm_l1(ctx, l1) bind(c, name="dvm_l1")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: l1(*)
+     end function dvm_l1
      ! parity mode: uniforms from an explicit list (replaces get_rand,
      ! utils/random_number_generation.f90:140-162, for parity runs)
      integer(c_int) function dvm_set_uniform_stream(ctx, u, n) bind(c, name="dvm_set_uniform_stream")

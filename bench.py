@@ -7,7 +7,10 @@ cell per step. One step = CDF build + VR collision sampling + wall ghosts +
 upwind sweep + moments, with the distributions resident in HBM before timing
 starts. It runs with FP32 collision arithmetic (the value) and in plain FP64.
 
-Also measured (unless --no-batched / --no-serving):
+Also measured (unless --no-relaxation / --no-batched / --no-serving):
+  relaxation_c1  the space-homogeneous relaxation (1 cell, 21^3, 100 000 samples
+            per step, 200 steps, FP64) with the L1 distance from equilibrium
+            every step, beside the oracle on one core
   batched   4096 space-homogeneous instances, grid edge 15..41 odd, mixtures of
             1-3 Maxwellians, 50 samples per node, stepped by one replayed HIP
             graph per step with autotuned collision tiles
@@ -268,6 +271,47 @@ def run_shock(pkg, steps, warmup):
             "phase_ms": {k: round(v, 4) for k, v in phase.items()}}
 
 
+def run_relaxation(pkg, steps=200, cpu=True, cpu_steps=20):
+    """The space-homogeneous relaxation (the reference's CPU-runnable case): 21^3
+    nodes on +-5, two Maxwellians of equal density at T and 2T moving at +-1
+    along x, hard spheres, 100 000 samples per step, dt = 0.05, FP64. One pass
+    timed without host reads, then the same steps (same seed) with the moments
+    and the L1 distance from equilibrium read every step, as the workload asks;
+    the oracle on one core times a bounded number of the same steps."""
+    cfg, f = pkg.homogeneous_relaxation()
+    wall, phase, _ = run_gpu(pkg, cfg, f, steps, 0)
+    props, l1 = [], []
+    with pkg.Solver(cfg) as s:
+        props.append(s.initialize(f)[0])
+        l1.append(s.l1_to_equilibrium()[0])
+        for n in range(1, steps + 1):
+            s.step(n)
+            props.append(s.calculate_physical_properties()[0])
+            l1.append(s.l1_to_equilibrium()[0])
+    props = np.array(props)
+    out = {"workload": f"space-homogeneous relaxation, 1 cell x {cfg.nvx}^3 nodes on +-5, bi-Maxwellian T and 2T "
+                       f"at +-1 along x, hard sphere, {cfg.samples_per_cell} samples/step, dt {cfg.dt}, FP64",
+           "steps": steps, "ms_per_step": wall / steps * 1e3,
+           "collision_samples_per_s": cfg.samples_per_cell * steps / wall,
+           "phase_ms": {k: round(v, 4) for k, v in phase.items()},
+           "l1_to_equilibrium": {str(n): float(l1[n]) for n in (0, 1, 10, 50, 100, steps) if n <= steps},
+           "max_rel_drift": {"density": float(np.max(np.abs(props[:, 0] / props[0, 0] - 1.0))),
+                             "velocity": float(np.max(np.abs(props[:, 1:4] - props[0, 1:4]))),
+                             "temperature": float(np.max(np.abs(props[:, 4] / props[0, 4] - 1.0)))}}
+    if cpu:
+        from tests._oracle import oracle_solver
+        o = oracle_solver(cfg)
+        o.initialize(f)
+        t0 = time.perf_counter()
+        for n in range(1, cpu_steps + 1):
+            o.step(n)
+        dt = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"ms_per_step": dt / cpu_steps * 1e3, "cores": 1, "kind": "port",
+                               "sample": f"CPU oracle (serial C, -O2), the same workload, {cpu_steps} steps, {dt:.2f} s"}
+    return out
+
+
 def run_serving(pkg, count, rate, precision):
     reqs = pkg.synthetic_requests(count=count, rate=rate, seed=5, precision=precision)
     pools = dict(node_capacity=160_000_000, cell_capacity=65536, max_slots=512)
@@ -335,6 +379,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-shock", action="store_true")
     ap.add_argument("--no-fmc", action="store_true")
+    ap.add_argument("--no-relaxation", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default tiles (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=3)
@@ -409,6 +454,9 @@ def main():
     if not args.no_shock:
         out["shock_1d"] = run_shock(pkg, args.steps, args.warmup)
         print(f"[bench] shock 1-D: {out['shock_1d']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    if not args.no_relaxation:
+        out["relaxation_c1"] = run_relaxation(pkg, cpu=not args.no_cpu)
+        print(f"[bench] relaxation: {out['relaxation_c1']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_batched:
         out["batched"] = {}
         for prec in ("fp32", "fp64"):

@@ -228,6 +228,14 @@ int dvm_remap(dvm_ctx* ctx, int64_t n, const double* uvw, double f_off,
 int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_records,
                       int32_t* irec, double* drec, int64_t* n_out);
 
+/* Distance of each cell's f from equilibrium, l1[ncell] = sum |f - M| / n,
+   with M the discrete Maxwellian (times beta^3) at the cell's own density,
+   velocity and translational temperature, scaled so that sum M = n (-1 for an
+   empty or cold cell); the same measure as dvm_batch_l1. No counterpart in
+   the reference, which has no convergence measure: it is the per-step output
+   of space-homogeneous relaxation runs and their stopping rule. */
+int dvm_l1(dvm_ctx* ctx, double* l1);
+
 /* Parity mode: the collision phase takes its uniforms from the explicit list
    u[0..n) instead of the counter-based generator, so that one list can be fed
    to this library and to a CPU implementation. Each collision reads the list

@@ -933,6 +933,38 @@ int orc_collide_trace(orc_ctx* c, int32_t ntime, int32_t cell, int64_t max_recor
   return DVM_OK;
 }
 
+/* Distance from equilibrium of each cell (the measure of dvm_l1; no
+   counterpart in the reference): M_l = exp(-m |c_l - u|^2 / T) beta3_l
+   scaled so that sum M = n, then sum |f - M| / n, in node order. */
+int orc_l1(orc_ctx* c, double* l1) {
+  if (!c->f_valid) return set_err(c, DVM_E_STATE, "L1 before upload");
+  int rc = orc_moments(c, NULL);
+  if (rc) return rc;
+  const ogrid* g = &c->g;
+  int N = g->num_points;
+  for (int cell = 0; cell < c->ncell; ++cell) {
+    const double* p = c->props + (size_t)cell * DVM_NPROP;
+    const double* f = c->f + (size_t)cell * N;
+    double n = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W], T = p[DVM_P_TRTEMP];
+    if (!(n > 0.0) || !(T > 0.0)) { l1[cell] = -1.0; continue; }
+    double a = c->cfg.mass / T, msum = 0.0, e = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      double scale = (pass == 1) ? n / msum : 0.0;
+      int l = 0;
+      for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+          for (int i = 0; i < g->n[0]; ++i, ++l) {
+            double dx = g->ax[0][i] - u, dy = g->ax[1][j] - v, dz = g->ax[2][k] - w;
+            double m = exp(-(dx * dx + dy * dy + dz * dz) * a) * g->beta[0][i] * g->beta[1][j] * g->beta[2][k];
+            if (pass == 0) msum += m;
+            else e += fabs(f[l] - scale * m);
+          }
+    }
+    l1[cell] = e / n;
+  }
+  return DVM_OK;
+}
+
 int orc_set_uniform_stream(orc_ctx* c, const double* u, int64_t n) {
   if (n < 0 || (n > 0 && !u)) return set_err(c, DVM_E_ARG, "uniform stream: bad length or null pointer");
   for (int64_t q = 0; q < n; ++q)

@@ -35,6 +35,9 @@ int  orc_remap(orc_ctx* ctx, int64_t n, const double* uvw, double f_off,
                int32_t* stencil, double* mass);
 int  orc_collide_trace(orc_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_records,
                        int32_t* irec, double* drec, int64_t* n_out);
+/* dvm_l1: per-cell sum |f - M| / n against the discrete Maxwellian at the
+   cell's moments, scaled to sum M = n (-1 for an empty or cold cell). */
+int  orc_l1(orc_ctx* ctx, double* l1);
 /* Parity mode of dvm_set_uniform_stream: uniforms from an explicit list. */
 int  orc_set_uniform_stream(orc_ctx* ctx, const double* u, int64_t n);
 int  orc_uniform_stream_position(orc_ctx* ctx, int64_t* consumed);
