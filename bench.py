@@ -312,6 +312,48 @@ def run_relaxation(pkg, steps=200, cpu=True, cpu_steps=20):
     return out
 
 
+def run_shock_steady(pkg, steps=2000):
+    """The 1-D normal shock run to steady state, untimed: moments every 10
+    steps over the last 200. A stationary shock carries the same mass flux
+    n u through every cell; reported are its spread over the cells (in units
+    of the inflow's) and the largest relative density change per cell over
+    the last 100 steps."""
+    cfg, f = pkg.normal_shock_1d(nx=200, n_v=31, vmax=6.0, mach=2.0, samples=50_000)
+    cfg = cfg.with_(conv_order=pkg.capi.DVM_CONV_MINMOD2, precision=pkg.capi.DVM_PREC_FP64)
+    hist = {}
+    with pkg.Solver(cfg) as s:
+        s.initialize(f)
+        for n in range(1, steps + 1):
+            s.step(n)
+            if n > steps - 200 and n % 10 == 0:
+                hist[n] = s.calculate_physical_properties()
+    avg = np.mean([hist[n] for n in hist], axis=0)
+    flux = avg[:, 0] * avg[:, 1]
+    inflow = cfg.wall[0][0] * cfg.wall[0][1]
+    return {"steps": steps, "mass_flux_spread": float((flux.max() - flux.min()) / inflow),
+            "density_ratio_out_in": float(avg[-1, 0] / avg[0, 0]),
+            "max_density_change_last_100_steps": float(np.max(np.abs(hist[steps][:, 0] / hist[steps - 100][:, 0] - 1.0)))}
+
+
+def run_trajectory(pkg, cfg, f, steps):
+    """FP32 against FP64 along one trajectory of the cavity (same seed), beside
+    two FP64 runs with different seeds: after a few steps a pick that lands
+    within rounding of a CDF boundary differs, and the runs are then
+    independent Monte Carlo trajectories, so their difference is the method's
+    noise."""
+    def final(c):
+        with pkg.Solver(c) as s:
+            s.initialize(f)
+            for n in range(1, steps + 1):
+                s.step(n)
+            return s.calculate_physical_properties()
+    p64 = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP64))
+    p32 = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP32))
+    p64b = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP64, seed=cfg.seed + 1000003))
+    return {"steps": steps, "fp32_vs_fp64_same_seed": moment_error(p32, p64),
+            "fp64_seed_spread": moment_error(p64b, p64)}
+
+
 def run_serving(pkg, count, rate, precision):
     reqs = pkg.synthetic_requests(count=count, rate=rate, seed=5, precision=precision)
     pools = dict(node_capacity=160_000_000, cell_capacity=65536, max_slots=512)
@@ -380,6 +422,8 @@ def main():
     ap.add_argument("--no-shock", action="store_true")
     ap.add_argument("--no-fmc", action="store_true")
     ap.add_argument("--no-relaxation", action="store_true")
+    ap.add_argument("--no-trajectory", action="store_true")
+    ap.add_argument("--trajectory-steps", type=int, default=500)
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default tiles (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=3)
@@ -451,8 +495,12 @@ def main():
                            "collision_samples_per_s": samples_per_step * args.steps / wf,
                            "phase_ms": {k: round(v, 4) for k, v in pf.items()}}
         print(f"[bench] fmc {head}: {wf / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    if not args.no_trajectory and "fp64" in runs and "fp32" in runs:
+        out["fp32_trajectory"] = run_trajectory(pkg, cfg, f, args.trajectory_steps)
+        print(f"[bench] trajectory: {out['fp32_trajectory']}", file=sys.stderr, flush=True)
     if not args.no_shock:
         out["shock_1d"] = run_shock(pkg, args.steps, args.warmup)
+        out["shock_1d"]["steady_state"] = run_shock_steady(pkg)
         print(f"[bench] shock 1-D: {out['shock_1d']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_relaxation:
         out["relaxation_c1"] = run_relaxation(pkg, cpu=not args.no_cpu)
