@@ -634,8 +634,8 @@ static int batch_lists(dvm_batch* b) {
   int lds_n = 0;
   // Samples per work item: the tile when set; else about four items per CU
   // over the resident samples (never below 2^15: an item that splits a cell
-  // flushes its window with one atomic per node), so that a few large solves
-  // still spread over the whole chip.
+  // flushes its window with one atomic per node; never above 2^18, below),
+  // so that a few large solves still spread over the whole chip.
   long long step = b->item_samples;
   if (step <= 0) {
     long long total = 0;
@@ -643,6 +643,10 @@ static int batch_lists(dvm_batch* b) {
       if (b->hs[s].active) total += (explicit_mode ? b->hs[s].samples : 0) * b->hs[s].ncell;
     step = total / (4LL * (b->cus > 0 ? b->cus : 256));
     if (step < (1LL << 15)) step = 1LL << 15;
+    // and at most 2^18, so that a large cell's items are taken by several
+    // workgroups of its XCD, which share its CDFs in L2 (the autotuner's
+    // choice on the 4096-instance batch)
+    if (step > (1LL << 18)) step = 1LL << 18;
   }
   b->item_used = step;
   for (int s = 0; s < b->max_slots; ++s) {
