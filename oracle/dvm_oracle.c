@@ -28,7 +28,14 @@
  * like the HIP library, draws them from counter-based Philox4x32-10 keyed by
  * the seed with counter (sample, integral | slot<<2, cell, ntime): block slot 0
  * gives (u_A, u_B, u_phi, u_theta) of replenish location 1, block slot r gives
- * (u_phi, u_theta) of location r >= 2.
+ * (u_phi, u_theta) of location r >= 2. orc_set_uniform_stream replaces them by
+ * an explicit list read in the reference's draw order (cells in order, samples
+ * in order, u_A, u_B, then u_phi, u_theta per location), the parity mode that
+ * feeds one list to both codes.
+ *
+ * orc_l1 (the distance of f from its equilibrium Maxwellian) and the
+ * limited second-order 1-D sweep have no counterpart in the reference; they
+ * are pinned by invariants and closed forms in tests/.
  */
 #include "dvm_oracle.h"
 
