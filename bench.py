@@ -423,6 +423,7 @@ def main():
     ap.add_argument("--no-fmc", action="store_true")
     ap.add_argument("--no-relaxation", action="store_true")
     ap.add_argument("--no-trajectory", action="store_true")
+    ap.add_argument("--no-shard-tile", action="store_true")
     ap.add_argument("--trajectory-steps", type=int, default=500)
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default tiles (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=4096)
@@ -495,6 +496,21 @@ def main():
                            "collision_samples_per_s": samples_per_step * args.steps / wf,
                            "phase_ms": {k: round(v, 4) for k, v in pf.items()}}
         print(f"[bench] fmc {head}: {wf / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    if not args.no_shard_tile:
+        # one GPU's share of the cavity on 8 GPUs (a 2 x 4 grid of 32 x 16-cell
+        # tiles), at the same per-cell work; the halo each tile would exchange
+        # per step is the upwind half of its interior faces' nodes
+        ct, ft = pkg.lid_driven_cavity(nx=32, ny=16, n_v=args.nv, samples=args.samples)
+        ct = ct.with_(precision=pkg.capi.DVM_PREC_FP32 if head == "fp32" else pkg.capi.DVM_PREC_FP64)
+        wt, pt, _ = run_gpu(pkg, ct, ft, args.steps, args.warmup)
+        face_cells = 2 * 16 + 2 * 32  # an interior tile's four faces
+        out["shard_tile"] = {"workload": f"cavity tile 32x16 cells (one GPU's share of 64x64 on 8), {ct.nvx}^3 nodes, "
+                                         f"{ct.samples_per_cell} samples/cell, {head}",
+                             "ms_per_step": wt / args.steps * 1e3,
+                             "cell_node_updates_per_s": 32 * 16 * N * args.steps / wt,
+                             "phase_ms": {k: round(v, 4) for k, v in pt.items()},
+                             "halo_bytes_per_step_interior_tile": face_cells * N * 8 // 2}
+        print(f"[bench] shard tile: {out['shard_tile']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_trajectory and "fp64" in runs and "fp32" in runs:
         out["fp32_trajectory"] = run_trajectory(pkg, cfg, f, args.trajectory_steps)
         print(f"[bench] trajectory: {out['fp32_trajectory']}", file=sys.stderr, flush=True)
