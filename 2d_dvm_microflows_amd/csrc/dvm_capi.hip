@@ -46,6 +46,7 @@ struct dvm_ctx {
   int coll_bs = 1024;            // lanes per collision workgroup (256, 512 or 1024)
   int pick_k = 1;                // samples per lane picked in lockstep (1, 2 or 4)
   int split = 1;                 // workgroups per cell in the collision kernel
+  int cus = 256;                 // compute units of the device
   int heat_every = 0;            // wall heat fluxes every heat_every steps (0: off)
   int heat_ntime = -1;           // step of the values in d_heat
   double* d_heat = nullptr;      // [4][max(nx,ny)] wall heat fluxes
@@ -389,31 +390,53 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     if (v == 1 || v == 2 || v == 4) c->pick_k = v;
   }
   c->lds_collide = window_dims(n, c->wd.w);
-  // Few cells: split each cell's samples over several workgroups (about two
-  // per CU, at least 4096 samples each when the count is known up front).
-  {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    long long want = (2LL * cus + c->ncell - 1) / c->ncell;
-    if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT) {
-      const long long by_samples = cfg->samples_per_cell / 4096;
-      want = want < by_samples ? want : by_samples;
-    }
-    c->split = (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
-    if (const char* e = getenv("DVM_COLL_SPLIT")) {
-      const int v = atoi(e);
-      if (v >= 1 && v <= 1024) c->split = v;
-    }
-  }
   c->coll_lds = (size_t)(kAxisSlots + c->wd.w[0] * c->wd.w[1] * c->wd.w[2]) * sizeof(double);
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
   }
+  const void* kern = collide_kernel(c);
+  HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
+  HIP_TRY(c, hipFuncSetAttribute(collide_stream_kernel(c), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)c->coll_lds));
   {
-    const void* kern = collide_kernel(c);
-    HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
-    HIP_TRY(c, hipFuncSetAttribute(collide_stream_kernel(c), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)c->coll_lds));
+    int dev = 0;
+    c->cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  }
+  // Few cells: split each cell's samples over several workgroups, at least
+  // two rounds of workgroups over the chip (and at least 4096 samples per part
+  // when the count is known up front). Among the splits up to four times that,
+  // the one whose last round fills the chip best (fewest parts among equals):
+  // 200 cells on 256 CUs take 5 parts (1000 workgroups in 1024 slots), not 3
+  // (600 in 768).
+  {
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, c->coll_bs, c->coll_lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    const long long slots = (long long)per_cu * c->cus;
+    const long long want = (2LL * c->cus + c->ncell - 1) / c->ncell;
+    long long lo = want, hi = want > 1 ? 4 * want : 1;
+    if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT) {
+      const long long by_samples = cfg->samples_per_cell / 4096;
+      lo = lo < by_samples ? lo : by_samples;
+      hi = hi < by_samples ? hi : by_samples;
+    }
+    lo = lo < 1 ? 1 : (lo > 1024 ? 1024 : lo);
+    hi = hi < lo ? lo : (hi > 1024 ? 1024 : hi);
+    long long best = lo;
+    double best_fill = -1.0;
+    for (long long sp = lo; sp <= hi; ++sp) {
+      const long long wg = (long long)c->ncell * sp, rounds = (wg + slots - 1) / slots;
+      const double fill = (double)wg / (double)(rounds * slots);
+      if (fill > best_fill + 1e-9) { best = sp; best_fill = fill; }
+    }
+    c->split = (int)best;
+    if (const char* e = getenv("DVM_COLL_SPLIT")) {
+      const int v = atoi(e);
+      if (v >= 1 && v <= 1024) c->split = v;
+    }
   }
   // ---- convection constants
   ConvParams& p = c->convp;
@@ -593,8 +616,13 @@ static bool collides(const dvm_ctx* c) {
 }
 
 static int moments_async(dvm_ctx* c) {
-  hipLaunchKernelGGL(k_moments, dim3((c->ncell + kMomCells - 1) / kMomCells), dim3(kMomBS), 0, c->stream, c->g,
-                     c->f, c->props, c->cfg.mass, c->ncell);
+  // kMomCells cells per workgroup when that still gives two workgroups per CU
+  if (c->ncell >= 2 * kMomCells * c->cus)
+    hipLaunchKernelGGL(k_moments<kMomCells>, dim3((c->ncell + kMomCells - 1) / kMomCells), dim3(kMomBS), 0,
+                       c->stream, c->g, c->f, c->props, c->cfg.mass, c->ncell);
+  else
+    hipLaunchKernelGGL(k_moments<1>, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass,
+                       c->ncell);
   HIP_TRY(c, hipGetLastError());
   c->props_valid = true;
   return DVM_OK;

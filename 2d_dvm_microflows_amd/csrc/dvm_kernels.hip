@@ -10,7 +10,7 @@
 //   counts       [C][4]      int32 samples of the 3 integrals + their sum
 //   factors      [C][3]      depletion factor of each integral
 // Kernels:
-//   k_moments    one workgroup per cell; two passes, FP64 wave reductions
+//   k_moments    one or four cells per workgroup; one pass, FP64 wave reductions
 //   k_cdf        one workgroup per cell; cache test, equilibrium rebuild on a
 //                miss, |deviation| prefix scan, bucket tables, pair counts
 //   k_collide    one workgroup per cell; samples spread over the lanes, the
@@ -249,13 +249,14 @@ __global__ void __launch_bounds__(kMomBS) k_moments_ext(Grid g, const double* __
   }
 }
 
-// kMomCells cells per workgroup (the last workgroup may hold fewer).
+// C cells per workgroup (the last workgroup may hold fewer).
+template <int C>
 __global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
                                                     double* __restrict__ props, double mass, int ncell) {
-  __shared__ double red[kMomCells][6][kMomBS / 64];
-  const int c0 = blockIdx.x * kMomCells;
-  const int nc = (ncell - c0 < kMomCells) ? ncell - c0 : kMomCells;
-  moments_sums<kMomCells>(g, f + (size_t)c0 * g.N, (size_t)g.N, nc, red);
+  __shared__ double red[C][6][kMomBS / 64];
+  const int c0 = blockIdx.x * C;
+  const int nc = (ncell - c0 < C) ? ncell - c0 : C;
+  moments_sums<C>(g, f + (size_t)c0 * g.N, (size_t)g.N, nc, red);
   if ((int)threadIdx.x < nc) {
     double tot[6];
     moments_total(red[threadIdx.x], tot);
