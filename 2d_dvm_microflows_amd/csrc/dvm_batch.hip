@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf_cells(const SlotDesc* __restrict
                                                       const SlotState* __restrict__ state,
                                                       const CellRef* __restrict__ cells, CdfParams base,
                                                       BatchPools P) {
-  __shared__ CdfShared sh;
+  __shared__ CdfShared<kCdfBS, kCdfV> sh;
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
   const int cur = state[r.slot].cur;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf_cells(const SlotDesc* __restrict
   cp.beta3 = d.beta3;
   cp.samples_per_cell = d.samples;
   const size_t n0 = cell_node0(d, r.cell);
-  cdf_cell(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
+  cdf_cell<kCdfBS, kCdfV>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
            P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh);
 }
 

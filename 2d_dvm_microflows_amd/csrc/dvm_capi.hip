@@ -630,8 +630,13 @@ static int moments_async(dvm_ctx* c) {
 
 static int cdf_async(dvm_ctx* c) {
   if (!collides(c)) return DVM_OK;
-  hipLaunchKernelGGL(k_cdf, dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f, c->props, c->eqv,
-                     c->cdf, c->tabs, c->eqpar, c->counts, c->factors);
+  // few cells (fewer than four workgroups per CU): 1024 lanes per cell
+  if (c->ncell >= 4 * c->cus)
+    hipLaunchKernelGGL((k_cdf<kCdfBS, kCdfV>), dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f,
+                       c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors);
+  else
+    hipLaunchKernelGGL((k_cdf<kCdfWideBS, kCdfWideV>), dim3(c->ncell), dim3(kCdfWideBS), 0, c->stream, c->g, c->cdfp,
+                       c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors);
   HIP_TRY(c, hipGetLastError());
   return DVM_OK;
 }

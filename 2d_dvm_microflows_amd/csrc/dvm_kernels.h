@@ -11,6 +11,8 @@ constexpr int kMomBS = 256;      // moments / wall-flux workgroup
 constexpr int kMomCells = 4;     // cells per k_moments workgroup
 constexpr int kCdfBS = 256;      // CDF workgroup
 constexpr int kCdfV = 8;         // consecutive nodes per lane in the CDF scan
+constexpr int kCdfWideBS = 1024; // CDF workgroup when the cells are too few to fill the chip
+constexpr int kCdfWideV = 4;     //   (its chunk tile stays within the 64 KB of static LDS)
 constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell's LDS slice
 constexpr int kConvBS = 256;     // convection lanes per workgroup
 #ifndef DVM_CONV_STRIP
@@ -74,6 +76,7 @@ __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restri
                           int ncell);
 __global__ void k_moments_ext(Grid g, const double* __restrict__ f, const double* __restrict__ props, double mass,
                               double* __restrict__ ext);
+template <int BS, int V>
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
                       double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
                       double* __restrict__ eqpar,

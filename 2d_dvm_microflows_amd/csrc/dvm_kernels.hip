@@ -319,18 +319,20 @@ __device__ __forceinline__ double chunk_scan(double (&v)[V], double* red, double
 
 // Position in the LDS chunk tile of node m of a chunk: lane m / V owns V
 // consecutive nodes, rows padded to V + 1 doubles against bank conflicts.
-__device__ __forceinline__ int tile_pos(int m) { return (m / kCdfV) * (kCdfV + 1) + (m % kCdfV); }
+template <int V>
+__device__ __forceinline__ int tile_pos(int m) { return (m / V) * (V + 1) + (m % V); }
 
 // compute_cdf_vr (variance_reduction.f90:234-496) for the like-species pair,
 // plus compute_num_coll_pairs (collision_utilities.f90:106-140) and the
 // depletion factors (variance_reduction.f90:598-604,679,831,976).
-// One workgroup per cell. Each chunk of kCdfBS*kCdfV nodes is loaded and
-// stored coalesced through an LDS tile in which lane t owns kCdfV consecutive
+// One workgroup of BS lanes per cell. Each chunk of BS*V nodes is loaded and
+// stored coalesced through an LDS tile in which lane t owns V consecutive
 // nodes for the scan.
+template <int BS, int V>
 struct CdfShared {
-  double red[kCdfBS / 64];
-  double lastp[kCdfBS];
-  double tile[kCdfBS * (kCdfV + 1)];
+  double red[BS / 64];
+  double lastp[BS];
+  double tile[BS * (V + 1)];
   int rights[kNBucket];
   int s_new;
 };
@@ -338,12 +340,12 @@ struct CdfShared {
 // One cell: fc its distribution, p its moments, ev / ec / tab2 / par / cnt4 /
 // fac3 its slices of eqv, cdf ([2][N+1]), tabs ([2][1000]), eqpar, counts and
 // factors.
+template <int BS, int V>
 __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, const double* __restrict__ fc,
                                          const double* __restrict__ p, double* __restrict__ ev,
                                          double* __restrict__ ec, int2* __restrict__ tab2, double* __restrict__ par,
-                                         int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared& sh) {
-  constexpr int V = kCdfV;
-  constexpr int CH = kCdfBS * V;
+                                         int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared<BS, V>& sh) {
+  constexpr int CH = BS * V;
   double* red = sh.red;
   double* lastp = sh.lastp;
   double* tile = sh.tile;
@@ -398,11 +400,11 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
         pv[q] = val;
         if (++i == nx) { i = 0; if (++j == ny) { j = 0; ++k; } }
       }
-      carry = carry + chunk_scan<kCdfBS, V>(pv, red, carry);
+      carry = carry + chunk_scan<BS, V>(pv, red, carry);
       lastp[threadIdx.x] = pv[V - 1];
       __syncthreads();
       const double before = (threadIdx.x == 0) ? prev_last : lastp[threadIdx.x - 1];
-      prev_last = lastp[kCdfBS - 1];
+      prev_last = lastp[BS - 1];
       double e[V];
 #pragma unroll
       for (int q = 0; q < V; ++q) {
@@ -412,21 +414,21 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        const int m = q * kCdfBS + threadIdx.x, l = base + m;
-        if (l < N) ev[l] = tile[tile_pos(m)];
+        const int m = q * BS + threadIdx.x, l = base + m;
+        if (l < N) ev[l] = tile[tile_pos<V>(m)];
       }
-      carry_c = carry_c + chunk_scan<kCdfBS, V>(e, red, carry_c);
+      carry_c = carry_c + chunk_scan<BS, V>(e, red, carry_c);
 #pragma unroll
       for (int q = 0; q < V; ++q) tile[own + q] = e[q];
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < V; ++q) {
-        const int m = q * kCdfBS + threadIdx.x, l = base + m;
-        if (l < N) ec[l + 1] = tile[tile_pos(m)];
+        const int m = q * BS + threadIdx.x, l = base + m;
+        if (l < N) ec[l + 1] = tile[tile_pos<V>(m)];
       }
       // the next chunk writes the tile only after its first scan's barriers
     }
-    build_buckets<kCdfBS>(ec, N, tab2, rights);
+    build_buckets<BS>(ec, N, tab2, rights);
     if (threadIdx.x == 0) {
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
     }
@@ -440,37 +442,37 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   double nf[V], ne[V];
 #pragma unroll
   for (int q = 0; q < V; ++q) {
-    const int l = q * kCdfBS + threadIdx.x;
+    const int l = q * BS + threadIdx.x;
     nf[q] = (l < N) ? fc[l] : 0.0;
     ne[q] = (l < N && !fmc) ? ev[l] : 0.0;
   }
   for (int base = 0; base < N; base += CH) {
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-      const int m = q * kCdfBS + threadIdx.x, l = base + m;
-      tile[tile_pos(m)] = (l < N) ? fabs(nf[q] - ne[q]) : 0.0;
+      const int m = q * BS + threadIdx.x, l = base + m;
+      tile[tile_pos<V>(m)] = (l < N) ? fabs(nf[q] - ne[q]) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-      const int l = base + CH + q * kCdfBS + threadIdx.x;
+      const int l = base + CH + q * BS + threadIdx.x;
       if (l < N) { nf[q] = fc[l]; ne[q] = fmc ? 0.0 : ev[l]; }
     }
     __syncthreads();
     double d[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) d[q] = tile[own + q];
-    carry = carry + chunk_scan<kCdfBS, V>(d, red, carry);
+    carry = carry + chunk_scan<BS, V>(d, red, carry);
 #pragma unroll
     for (int q = 0; q < V; ++q) tile[own + q] = d[q];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-      const int m = q * kCdfBS + threadIdx.x, l = base + m;
-      if (l < N) nc[l + 1] = tile[tile_pos(m)];
+      const int m = q * BS + threadIdx.x, l = base + m;
+      if (l < N) nc[l + 1] = tile[tile_pos<V>(m)];
     }
     __syncthreads();
   }
-  build_buckets<kCdfBS>(nc, N, tab2 + kNBucket, rights);
+  build_buckets<BS>(nc, N, tab2 + kNBucket, rights);
   if (fmc) {
     // full_monte_carlo_kernel (full_mc.f90:240-317): abs_dens = dens - 2 neg_dens
     // with neg_dens = (dens - sum |f|) / 2; compute_num_coll_pairs with both
@@ -523,15 +525,16 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   }
 }
 
-__global__ void __launch_bounds__(kCdfBS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
+template <int BS, int V>
+__global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
                                                 const double* __restrict__ props, double* __restrict__ eqv,
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
                                                 double* __restrict__ eqpar,
                                                 int* __restrict__ counts, double* __restrict__ factors) {
-  __shared__ CdfShared sh;
+  __shared__ CdfShared<BS, V> sh;
   const int cell = blockIdx.x;
   const int N = g.N;
-  cdf_cell(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
+  cdf_cell<BS, V>(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
            cdf + (size_t)cell * 2 * (N + 1), tabs + (size_t)cell * 2 * kNBucket, eqpar + (size_t)cell * 5,
            counts + cell * 4, factors + cell * 3, sh);
 }
