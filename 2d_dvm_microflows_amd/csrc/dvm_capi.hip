@@ -632,7 +632,7 @@ static int cdf_async(dvm_ctx* c) {
   if (!collides(c)) return DVM_OK;
   // few cells (fewer than four workgroups per CU): 1024 lanes per cell
   if (c->ncell >= 4 * c->cus)
-    hipLaunchKernelGGL((k_cdf<kCdfBS, kCdfV>), dim3(c->ncell), dim3(kCdfBS), 0, c->stream, c->g, c->cdfp, c->f,
+    hipLaunchKernelGGL((k_cdf<kCdfCtxBS, kCdfV>), dim3(c->ncell), dim3(kCdfCtxBS), 0, c->stream, c->g, c->cdfp, c->f,
                        c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors);
   else
     hipLaunchKernelGGL((k_cdf<kCdfWideBS, kCdfWideV>), dim3(c->ncell), dim3(kCdfWideBS), 0, c->stream, c->g, c->cdfp,

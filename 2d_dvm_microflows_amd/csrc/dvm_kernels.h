@@ -9,7 +9,8 @@ namespace dvm {
 
 constexpr int kMomBS = 256;      // moments / wall-flux workgroup
 constexpr int kMomCells = 4;     // cells per k_moments workgroup
-constexpr int kCdfBS = 256;      // CDF workgroup
+constexpr int kCdfBS = 256;      // CDF workgroup of the batch
+constexpr int kCdfCtxBS = 512;   // CDF workgroup of a context with many cells
 constexpr int kCdfV = 8;         // consecutive nodes per lane in the CDF scan
 constexpr int kCdfWideBS = 1024; // CDF workgroup when the cells are too few to fill the chip
 constexpr int kCdfWideV = 4;     //   (its chunk tile stays within the 64 KB of static LDS)
