@@ -11,7 +11,8 @@
 //
 // One step of every active slot:
 //   k_zero_int         the work-queue heads (a fill kernel, graph-safe)
-//   k_cdf_cells        one workgroup per resident cell
+//   k_cdf_cells        one workgroup per resident cell (1024 lanes when fewer
+//                      than four cells per CU are resident, else 256)
 //   k_prep_cells       fd = f (FP64) or dacc = 0 (FP32) for cells that take
 //                      global atomics (window smaller than the grid, or
 //                      samples split over several work items)
@@ -84,11 +85,12 @@ __global__ void k_zero_int(int* __restrict__ p, int n) {
   for (int q = threadIdx.x; q < n; q += blockDim.x) p[q] = 0;
 }
 
-__global__ void __launch_bounds__(kCdfBS) k_cdf_cells(const SlotDesc* __restrict__ slots,
-                                                      const SlotState* __restrict__ state,
-                                                      const CellRef* __restrict__ cells, CdfParams base,
-                                                      BatchPools P) {
-  __shared__ CdfShared<kCdfBS, kCdfV> sh;
+template <int BS, int V>
+__global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ slots,
+                                                  const SlotState* __restrict__ state,
+                                                  const CellRef* __restrict__ cells, CdfParams base,
+                                                  BatchPools P) {
+  __shared__ CdfShared<BS, V> sh;
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
   const int cur = state[r.slot].cur;
@@ -100,7 +102,7 @@ __global__ void __launch_bounds__(kCdfBS) k_cdf_cells(const SlotDesc* __restrict
   cp.beta3 = d.beta3;
   cp.samples_per_cell = d.samples;
   const size_t n0 = cell_node0(d, r.cell);
-  cdf_cell<kCdfBS, kCdfV>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
+  cdf_cell<BS, V>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
            P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh);
 }
 
@@ -739,8 +741,12 @@ static int batch_lists(dvm_batch* b) {
 static int batch_enqueue_step(dvm_batch* b) {
   const bool f32 = batch_f32(b);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, b->stream, b->P.queue, kXcds);
-  hipLaunchKernelGGL(k_cdf_cells, dim3(b->n_cells), dim3(kCdfBS), 0, b->stream, b->d_slots, b->d_state, b->d_cells,
-                     b->cdfp, b->P);
+  if (b->n_cells >= 4 * b->cus)
+    hipLaunchKernelGGL((k_cdf_cells<kCdfBS, kCdfV>), dim3(b->n_cells), dim3(kCdfBS), 0, b->stream, b->d_slots,
+                       b->d_state, b->d_cells, b->cdfp, b->P);
+  else
+    hipLaunchKernelGGL((k_cdf_cells<kCdfWideBS, kCdfWideV>), dim3(b->n_cells), dim3(kCdfWideBS), 0, b->stream,
+                       b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P);
   if (b->any_global) {
     const unsigned chunks = (unsigned)((b->max_n + 255) / 256);
     const unsigned ys = (unsigned)(b->n_cells < 65535 ? b->n_cells : 65535);

@@ -15,8 +15,12 @@ number of steps, solver/dvm_solver.f90:299):
                the best L1 for `plateau_steps` steps (the discrete collision
                operator's fixed point sits slightly off the discrete
                Maxwellian on coarse grids), or max_steps
-  shock        steady state: max |n - n_prev| / max n over one check interval
-               <= steady_tol, or max_steps
+  shock        steady state: the density profile averaged over the checks of
+               one window (`steady_window` checks) differs from the previous
+               window's average by at most steady_tol (max |n - n_prev| / max n),
+               or max_steps. The averaging reduces the Monte Carlo noise of
+               single checks, which at 20 000 samples per cell is of the order of
+               the tolerance itself
 An instance's trajectory depends only on its own config (seeded counter-based
 sampling), and its checks fall every check_every steps from its own
 admission, so a request ends in the same state and step whether it is served
@@ -83,10 +87,12 @@ class Server:
                     instance's bytes)
     batch           an existing Batch (or an object with its admit / step /
                     l1 / moments / retire interface) instead of a new one
+    steady_window   checks averaged per window of the shock steady-state rule
     """
 
     def __init__(self, species, node_capacity, cell_capacity, max_slots, check_every=10, plateau_steps=50,
-                 keep_f=False, keep_moments=False, clock=time.perf_counter, sleep=time.sleep, batch=None):
+                 keep_f=False, keep_moments=False, clock=time.perf_counter, sleep=time.sleep, batch=None,
+                 steady_window=5):
         self.batch = batch if batch is not None else Batch(species, max_slots=max_slots,
                                                            node_capacity=node_capacity,
                                                            cell_capacity=cell_capacity)
@@ -94,6 +100,7 @@ class Server:
         self.cell_capacity = int(cell_capacity)
         self.check_every = int(check_every)
         self.plateau_steps = int(plateau_steps)
+        self.steady_window = max(1, int(steady_window))
         self.keep_f = keep_f
         self.keep_moments = keep_moments
         self.clock = clock
@@ -136,7 +143,7 @@ class Server:
                     continue
                 waiting.remove(req)
                 out[req.rid].admitted = self.clock() - t0
-                resident[slot] = dict(req=req, steps=0, best=math.inf, since=0, prev=None)
+                resident[slot] = dict(req=req, steps=0, best=math.inf, since=0, prev=None, acc=None, cnt=0)
             max_res = max(max_res, len(resident))
             if not resident:
                 if i < len(reqs):
@@ -171,11 +178,15 @@ class Server:
                         reason = "plateau"
                 else:
                     dens = b.moments(slot)[:, capi.P_DENS]
-                    if st["prev"] is not None:
-                        metric = float(np.max(np.abs(dens - st["prev"])) / max(np.max(np.abs(dens)), 1e-300))
-                        if metric <= req.tol:
-                            reason = "steady"
-                    st["prev"] = dens
+                    st["acc"] = dens.copy() if st["acc"] is None else st["acc"] + dens
+                    st["cnt"] += 1
+                    if st["cnt"] == self.steady_window:
+                        mean = st["acc"] / self.steady_window
+                        if st["prev"] is not None:
+                            metric = float(np.max(np.abs(mean - st["prev"])) / max(np.max(np.abs(mean)), 1e-300))
+                            if metric <= req.tol:
+                                reason = "steady"
+                        st["prev"], st["acc"], st["cnt"] = mean, None, 0
                 if not reason and st["steps"] >= req.max_steps:
                     reason = "max_steps"
                 if reason:
