@@ -103,9 +103,11 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
-def run_gpu(pkg, cfg, f, steps, warmup):
+def run_gpu(pkg, cfg, f, steps, warmup, final=None):
     """Times `steps` fused steps after `warmup` ones; returns wall seconds, mean
-    per-phase device ms (HIP events on the library's stream) and the moments."""
+    per-phase device ms (HIP events on the library's stream) and the moments.
+    With a dict `final`, also stores the final f and output moments in it
+    (after the timed region)."""
     s = pkg.Solver(cfg)
     s.initialize(f)
     for n in range(1, warmup + 1):
@@ -119,8 +121,24 @@ def run_gpu(pkg, cfg, f, steps, warmup):
     t1 = time.perf_counter()
     phase, calls = s.timing_read()
     props = s.calculate_physical_properties()
+    if final is not None:
+        final["f"] = s.download_f()
+        final["ext"] = s.output_moments()
     s.close()
     return (t1 - t0), {k: v / max(calls, 1) for k, v in phase.items()}, props
+
+
+def output_moment_error(e32, e64, f32, f64):
+    """Max over cells of the FP32 error of the heat flux, the viscous stress
+    (each relative to the largest magnitude of that field over the cavity, as
+    both vanish at equilibrium) and the entropy (relative per cell), and the
+    relative L1 difference of the distributions, sum |f32 - f64| / sum |f64|."""
+    q32, q64 = e32[:, 0:3], e64[:, 0:3]
+    t32, t64 = e32[:, 3:9], e64[:, 3:9]
+    return {"heat_flux": float(np.max(np.abs(q32 - q64)) / max(np.max(np.abs(q64)), 1e-300)),
+            "viscous_stress": float(np.max(np.abs(t32 - t64)) / max(np.max(np.abs(t64)), 1e-300)),
+            "entropy": float(np.max(np.abs(e32[:, 9] - e64[:, 9]) / np.maximum(np.abs(e64[:, 9]), 1e-300))),
+            "f_l1": float(np.sum(np.abs(f32 - f64)) / np.sum(np.abs(f64)))}
 
 
 def cpu_baseline(pkg, cfg_full, steps=12):
@@ -346,12 +364,16 @@ def run_trajectory(pkg, cfg, f, steps):
             s.initialize(f)
             for n in range(1, steps + 1):
                 s.step(n)
-            return s.calculate_physical_properties()
-    p64 = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP64))
-    p32 = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP32))
-    p64b = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP64, seed=cfg.seed + 1000003))
-    return {"steps": steps, "fp32_vs_fp64_same_seed": moment_error(p32, p64),
-            "fp64_seed_spread": moment_error(p64b, p64)}
+            return s.calculate_physical_properties(), s.output_moments(), s.download_f()
+    p64, e64, f64 = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP64))
+    p32, e32, f32 = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP32))
+    same = moment_error(p32, p64)
+    same.update(output_moment_error(e32, e64, f32, f64))
+    del p32, e32, f32
+    p64b, e64b, f64b = final(cfg.with_(precision=pkg.capi.DVM_PREC_FP64, seed=cfg.seed + 1000003))
+    spread = moment_error(p64b, p64)
+    spread.update(output_moment_error(e64b, e64, f64b, f64))
+    return {"steps": steps, "fp32_vs_fp64_same_seed": same, "fp64_seed_spread": spread}
 
 
 def run_serving(pkg, count, rate, precision):
@@ -440,10 +462,11 @@ def main():
     cfg, f = pkg.lid_driven_cavity(nx=args.nx, ny=args.nx, n_v=args.nv, samples=args.samples)
     ncell, N = cfg.nx * cfg.ny, cfg.num_points
     updates = ncell * N * args.steps
-    runs = {}
+    runs, finals = {}, {}
     for prec in (("fp64", "fp32") if args.precision == "both" else (args.precision,)):
         c = cfg.with_(precision=pkg.capi.DVM_PREC_FP32 if prec == "fp32" else pkg.capi.DVM_PREC_FP64)
-        runs[prec] = run_gpu(pkg, c, f, args.steps, args.warmup)
+        finals[prec] = {}
+        runs[prec] = run_gpu(pkg, c, f, args.steps, args.warmup, finals[prec])
         print(f"[bench] {prec}: {runs[prec][0] / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
     head = "fp32" if "fp32" in runs else "fp64"
     wall, phase, props = runs[head]
@@ -485,7 +508,10 @@ def main():
                             "phase_ms": {k: round(v, 4) for k, v in runs["fp64"][1].items()}}
         out["speedup_fp32_vs_fp64"] = w64 / wall
         out["fp32_moment_rel_error_vs_fp64"] = moment_error(runs["fp32"][2], runs["fp64"][2])
+        out["fp32_moment_rel_error_vs_fp64"].update(
+            output_moment_error(finals["fp32"]["ext"], finals["fp64"]["ext"], finals["fp32"]["f"], finals["fp64"]["f"]))
         out["fp32_moment_error_after_steps"] = args.warmup + args.steps
+    finals.clear()
     if not args.no_fmc:
         # the same cavity with the full Monte Carlo kernel (iColl = 2): one
         # integral of samples drawn from the |f| CDF
