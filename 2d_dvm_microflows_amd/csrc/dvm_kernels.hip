@@ -928,39 +928,33 @@ __device__ __forceinline__ void conserve_cell(const Grid& g, const double* __res
   auto& red = sh.red;
   double* lam = sh.lam;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  // The 20 sums (sum d phi_a, then sum |f| phi_a phi_b for a <= b) in two
-  // rounds of 10 accumulators, so that the reduction fits next to a
-  // register-heavy caller.
-#pragma unroll 1
-  for (int round = 0; round < 2; ++round) {
-    double sums[10];
+  // The 20 sums (sum d phi_a, then sum |f| phi_a phi_b for a <= b) in one
+  // pass over the nodes.
+  {
+    double sums[20];
 #pragma unroll
-    for (int z = 0; z < 10; ++z) sums[z] = 0.0;
+    for (int z = 0; z < 20; ++z) sums[z] = 0.0;
     for (int l = threadIdx.x; l < N; l += BS) {
       int i, j, k;
       node_ijk(g, l, i, j, k);
       const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
       const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
-      if (round == 0) {
-        const double d = da[l];
-        const double w = fabs(fc[l]);
+      const double d = da[l];
+      const double w = fabs(fc[l]);
 #pragma unroll
-        for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
+      for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
 #pragma unroll
-        for (int b = 0; b < 5; ++b) sums[5 + b] += w * phi[0] * phi[b];
-      } else {
-        const double w = fabs(fc[l]);
-        int o = 0;
+      for (int b = 0; b < 5; ++b) sums[5 + b] += w * phi[0] * phi[b];
+      int o = 10;
 #pragma unroll
-        for (int a = 1; a < 5; ++a)
+      for (int a = 1; a < 5; ++a)
 #pragma unroll
-          for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
-      }
+        for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
     }
 #pragma unroll
-    for (int z = 0; z < 10; ++z) {
+    for (int z = 0; z < 20; ++z) {
       const double v = wave_sum(sums[z]);
-      if (lane == 0) red[wid][round * 10 + z] = v;
+      if (lane == 0) red[wid][z] = v;
     }
   }
   __syncthreads();
