@@ -43,7 +43,7 @@ struct dvm_ctx {
   bool lds_collide = false;      // the FP64 slice fits LDS (window = grid)
   WinDims wd{};                  // LDS window extents
   size_t coll_lds = 0;
-  int coll_bs = 1024;            // lanes per collision workgroup (256, 512 or 1024)
+  int coll_bs = 1024;            // lanes per collision workgroup (256, 512, 768 or 1024)
   int pick_k = 1;                // samples per lane picked in lockstep (1, 2 or 4)
   int split = 1;                 // workgroups per cell in the collision kernel
   int cus = 256;                 // compute units of the device
@@ -381,6 +381,12 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   HIP_TRY(c, hipMemset(c->eqv, 0, CN * sizeof(double)));
   // collision accumulator placement: the cell's slice in LDS when it fits
   const bool f32 = cfg->precision == DVM_PREC_FP32;
+  c->lds_collide = window_dims(n, c->wd.w);
+  // FP64 with an LDS window (the grid exceeds LDS): 768 lanes, where the FP64
+  // sampling code's registers spill less than at the 128-register cap of 1024
+  // lanes (1-D shock, 31^3: collide 0.933 ms against 0.966; the cavity's
+  // whole-grid FP64 slice and the FP32 path run faster with 1024)
+  if (!f32 && !c->lds_collide) c->coll_bs = 768;
   if (const char* e = getenv("DVM_COLL_BS")) {
     const int v = atoi(e);
     if (v == 256 || v == 512 || v == 768 || v == 1024) c->coll_bs = v;
@@ -389,7 +395,6 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     const int v = atoi(e);
     if (v == 1 || v == 2 || v == 4) c->pick_k = v;
   }
-  c->lds_collide = window_dims(n, c->wd.w);
   c->coll_lds = (size_t)(kAxisSlots + c->wd.w[0] * c->wd.w[1] * c->wd.w[2]) * sizeof(double);
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
