@@ -621,13 +621,18 @@ static bool collides(const dvm_ctx* c) {
 }
 
 static int moments_async(dvm_ctx* c) {
-  // kMomCells cells per workgroup when that still gives two workgroups per CU
+  // kMomCells cells per workgroup when that still gives two workgroups per CU;
+  // one cell per workgroup otherwise, on 1024 lanes when even that leaves the
+  // chip short of two workgroups per CU (the 1-D shock, a homogeneous cell)
   if (c->ncell >= 2 * kMomCells * c->cus)
-    hipLaunchKernelGGL(k_moments<kMomCells>, dim3((c->ncell + kMomCells - 1) / kMomCells), dim3(kMomBS), 0,
-                       c->stream, c->g, c->f, c->props, c->cfg.mass, c->ncell);
+    hipLaunchKernelGGL((k_moments<kMomCells, kMomBS>), dim3((c->ncell + kMomCells - 1) / kMomCells), dim3(kMomBS),
+                       0, c->stream, c->g, c->f, c->props, c->cfg.mass, c->ncell);
+  else if (c->ncell >= 2 * c->cus)
+    hipLaunchKernelGGL((k_moments<1, kMomBS>), dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props,
+                       c->cfg.mass, c->ncell);
   else
-    hipLaunchKernelGGL(k_moments<1>, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass,
-                       c->ncell);
+    hipLaunchKernelGGL((k_moments<1, 1024>), dim3(c->ncell), dim3(1024), 0, c->stream, c->g, c->f, c->props,
+                       c->cfg.mass, c->ncell);
   HIP_TRY(c, hipGetLastError());
   c->props_valid = true;
   return DVM_OK;

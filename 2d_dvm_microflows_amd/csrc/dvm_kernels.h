@@ -72,7 +72,7 @@ struct ConvParams {
   int order;                     // 1-D sweep order (1 or 2)
 };
 
-template <int C>
+template <int C, int BS>
 __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restrict__ props, double mass,
                           int ncell);
 __global__ void k_moments_ext(Grid g, const double* __restrict__ f, const double* __restrict__ props, double mass,

@@ -98,20 +98,20 @@ __device__ __forceinline__ void moments_finish(const double* tot, double* __rest
   p[DVM_P_AVGSPEED] = tot[5] / dens;
 }
 
-// Sums of C cells f[c * stride + l] over the nodes, one workgroup of kMomBS
-// lanes: lane t takes nodes t, t + kMomBS, ..., so the node's coordinates and
+// Sums of C cells f[c * stride + l] over the nodes, one workgroup of BS
+// lanes: lane t takes nodes t, t + BS, ..., so the node's coordinates and
 // speed are computed once for the C cells and the C loads of an iteration are
 // independent. Each cell's sums are added in the same order whatever C is.
-template <int C>
+template <int C, int BS = kMomBS>
 __device__ __forceinline__ void moments_sums(const Grid& g, const double* __restrict__ f, size_t stride, int nc,
-                                             double (*red)[6][kMomBS / 64]) {
+                                             double (*red)[6][BS / 64]) {
   const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
   double acc[C][6];
 #pragma unroll
   for (int c = 0; c < C; ++c)
 #pragma unroll
     for (int q = 0; q < 6; ++q) acc[c][q] = 0.0;
-  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
+  for (int l = threadIdx.x; l < g.N; l += BS) {
     int i, j, k;
     node_ijk(g, l, i, j, k);
     const double x = X[i], y = Y[j], z = Z[k];
@@ -141,10 +141,11 @@ __device__ __forceinline__ void moments_sums(const Grid& g, const double* __rest
   __syncthreads();
 }
 
-__device__ __forceinline__ void moments_total(const double (*red)[kMomBS / 64], double* tot) {
+template <int BS = kMomBS>
+__device__ __forceinline__ void moments_total(const double (*red)[BS / 64], double* tot) {
   for (int q = 0; q < 6; ++q) {
     double t = 0.0;
-    for (int w = 0; w < kMomBS / 64; ++w) t += red[q][w];
+    for (int w = 0; w < BS / 64; ++w) t += red[q][w];
     tot[q] = t;
   }
 }
@@ -249,17 +250,17 @@ __global__ void __launch_bounds__(kMomBS) k_moments_ext(Grid g, const double* __
   }
 }
 
-// C cells per workgroup (the last workgroup may hold fewer).
-template <int C>
-__global__ void __launch_bounds__(kMomBS) k_moments(Grid g, const double* __restrict__ f,
-                                                    double* __restrict__ props, double mass, int ncell) {
-  __shared__ double red[C][6][kMomBS / 64];
+// C cells per workgroup of BS lanes (the last workgroup may hold fewer).
+template <int C, int BS>
+__global__ void __launch_bounds__(BS) k_moments(Grid g, const double* __restrict__ f,
+                                                double* __restrict__ props, double mass, int ncell) {
+  __shared__ double red[C][6][BS / 64];
   const int c0 = blockIdx.x * C;
   const int nc = (ncell - c0 < C) ? ncell - c0 : C;
-  moments_sums<C>(g, f + (size_t)c0 * g.N, (size_t)g.N, nc, red);
+  moments_sums<C, BS>(g, f + (size_t)c0 * g.N, (size_t)g.N, nc, red);
   if ((int)threadIdx.x < nc) {
     double tot[6];
-    moments_total(red[threadIdx.x], tot);
+    moments_total<BS>(red[threadIdx.x], tot);
     moments_finish(tot, props + (size_t)(c0 + threadIdx.x) * DVM_NPROP, mass);
   }
 }
