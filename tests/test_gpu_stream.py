@@ -20,11 +20,12 @@ def _cases(pkg):
         "vss_ar": pkg.homogeneous_relaxation(n_v=13, samples=3000),
         "cavity_4x4": pkg.lid_driven_cavity(nx=4, ny=4, n_v=9, vmax=3.0, samples=400),
         "window_split_29": pkg.homogeneous_relaxation(n_v=29, samples=20000),
+        "fmc_nrepl2": pkg.homogeneous_relaxation(n_v=11, samples=3000),
     }
 
 
 OVER = {"homogeneous_nrepl2": dict(num_repl=2), "vss_ar": dict(omega=0.81, alpha=1.40), "cavity_4x4": {},
-        "window_split_29": {}}
+        "window_split_29": {}, "fmc_nrepl2": dict(coll_solver=2, num_repl=2)}
 
 
 @pytest.mark.parametrize("case", sorted(OVER))

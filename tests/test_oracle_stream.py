@@ -14,10 +14,12 @@ def _cases(pkg):
     cfg_h, f_h = pkg.homogeneous_relaxation(n_v=9, samples=1500)
     cfg_h = cfg_h.with_(num_repl=2, omega=0.81, alpha=1.4)
     cfg_c, f_c = pkg.lid_driven_cavity(nx=3, ny=3, n_v=7, vmax=3.0, samples=200)
-    return {"homogeneous_vss_nrepl2": (cfg_h, f_h), "cavity_3x3": (cfg_c, f_c)}
+    cfg_m, f_m = pkg.homogeneous_relaxation(n_v=9, samples=1200)
+    cfg_m = cfg_m.with_(coll_solver=pkg.capi.DVM_COLL_FMC, num_repl=2)
+    return {"homogeneous_vss_nrepl2": (cfg_h, f_h), "cavity_3x3": (cfg_c, f_c), "fmc_nrepl2": (cfg_m, f_m)}
 
 
-@pytest.mark.parametrize("case", ["homogeneous_vss_nrepl2", "cavity_3x3"])
+@pytest.mark.parametrize("case", ["homogeneous_vss_nrepl2", "cavity_3x3", "fmc_nrepl2"])
 def test_philox_words_as_stream_reproduce_default_mode(pkg, case):
     cfg, f = _cases(pkg)[case]
     ncell = cfg.nx * cfg.ny
