@@ -1082,8 +1082,9 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
   return bad;
 }
 
-// Collision phase: `split` workgroups of BS lanes per cell (workgroup b takes
-// part b % split of cell b / split, a contiguous range of its samples).
+// Collision phase: `split` workgroups of BS lanes per cell (each takes one
+// part, a contiguous range of the cell's samples; the parts of a cell share an
+// XCD).
 //   kLds: the cell's FP64 slice fits LDS (window = grid); otherwise the
 //         window of extents wd around the cell's bulk velocity.
 //   split == 1 and kLds: fd (FP64) is written directly, or (FP32) the
@@ -1108,7 +1109,16 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                                 int sper) {
   extern __shared__ double dyn[];
   __shared__ ConserveShared<BS> csh;
-  const int cell = blockIdx.x / split, part = blockIdx.x - cell * split;
+  // Workgroup b is dispatched to XCD b % 8. With split > 1 the parts of one
+  // cell go to consecutive workgroups of the same XCD, so that they share
+  // that XCD's L2 copy of the cell's CDFs: XCD x takes the contiguous range
+  // [start_x, start_x + count_x) of the (cell, part) order.
+  int v = blockIdx.x;
+  if (split > 1) {
+    const int G = gridDim.x, q = G >> 3, r = G & 7, x = v & 7;
+    v = x * q + (x < r ? x : r) + (v >> 3);
+  }
+  const int cell = v / split, part = v - cell * split;
   const int N = g.N;
   Window win{};
   if (!kLds) win = make_window(g, wd.w, props + (size_t)cell * DVM_NPROP);
