@@ -168,7 +168,7 @@ def lid_driven_cavity(nx=64, ny=64, n_v=25, vmax=5.0, samples=20_000, kn=0.1, li
 
 
 def rankine_hugoniot(mach, gamma=5.0 / 3.0):
-    """Density, velocity and temperature ratios across a normal shock
+    """Density, velocity, temperature and pressure ratios across a normal shock
     (shock_conditions.f90:140-160)."""
     m2 = mach * mach
     dens = (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
@@ -177,15 +177,23 @@ def rankine_hugoniot(mach, gamma=5.0 / 3.0):
     return dens, 1.0 / dens, temp, press
 
 
+def sound_speed(gamma, temp, mass=1.0):
+    """sqrt(gamma T / (2 m)): the speed of sound in the reference's units, where
+    the Maxwellian is exp(-m |c - u|^2 / T) (shock_conditions.f90:54-171; the
+    shock calculator's u_up = M sqrt(gamma T / (2 m)) with the mixture's mass,
+    software/shock_calculator/normal_shock_parameters.txt)."""
+    return math.sqrt(gamma * temp / (2.0 * mass))
+
+
 def normal_shock_1d(nx=200, n_v=31, vmax=6.0, mach=2.0, samples=50_000, kn=1.0, dx=0.1, cfl=0.5, seed=3,
-                    omega=0.81, width=10.0):
+                    omega=0.81, width=10.0, gamma=5.0 / 3.0):
     """1-D normal shock: upstream Maxwellian at Mach `mach` (T = 1), downstream
     state from Rankine-Hugoniot, tanh profile between them, fixed in/out flow
-    at both ends, VHS omega with isotropic scattering."""
+    at both ends, VHS omega with isotropic scattering; gamma = 5/3 for the
+    monatomic gas."""
     ax = uniform_axis(n_v, -vmax, vmax)
     dt = cfl * dx / vmax
-    gamma = 5.0 / 3.0
-    u1 = mach * math.sqrt(gamma * 1.0 / (2.0 * 1.0))  # sound speed sqrt(gamma T / 2m) in these units
+    u1 = mach * sound_speed(gamma, 1.0, 1.0)
     rd, ru, rt, _ = rankine_hugoniot(mach, gamma)
     wall = np.array([[1.0, u1, 0.0, 0.0, 1.0], [rd, u1 * ru, 0.0, 0.0, rt],
                      [1.0, 0.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 0.0, 1.0]])

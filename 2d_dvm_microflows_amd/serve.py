@@ -35,7 +35,7 @@ import numpy as np
 
 from . import capi
 from .batch import Batch
-from .config import SolverConfig, maxwellian, mixture_instance, rankine_hugoniot, uniform_axis
+from .config import SolverConfig, maxwellian, mixture_instance, rankine_hugoniot, sound_speed, uniform_axis
 
 
 @dataclass
@@ -211,13 +211,13 @@ class Server:
 
 
 # ------------------------------------------------------------------- workload
-def shock_instance(rng, nx, edge, mach, samples_per_cell, seed, precision, dx=0.1, cfl=0.5, kn=1.0):
+def shock_instance(rng, nx, edge, mach, samples_per_cell, seed, precision, dx=0.1, cfl=0.5, kn=1.0,
+                   gamma=5.0 / 3.0):
     """A 1-D normal shock request: Mach `mach` upstream (T = 1), the
     Rankine-Hugoniot state downstream, a tanh profile between them, in/out
     flow at both ends; hard spheres so that it shares the homogeneous
-    requests' species."""
-    gamma = 5.0 / 3.0
-    u1 = mach * math.sqrt(gamma / 2.0)
+    requests' species (monatomic: gamma = 5/3)."""
+    u1 = mach * sound_speed(gamma, 1.0, 1.0)
     rd, ru, rt, _ = rankine_hugoniot(mach, gamma)
     vmax = max(6.0, u1 + 4.0 * math.sqrt(max(1.0, rt)))
     ax = uniform_axis(edge, -vmax, vmax)

@@ -436,6 +436,28 @@ int orc_upload_f(orc_ctx* c, int32_t species, const double* f) {
   return DVM_OK;
 }
 
+int orc_set_f_keep_cache(orc_ctx* c, const double* f) {
+  size_t n = (size_t)c->ncell * c->g.num_points;
+  memcpy(c->f, f, n * sizeof(double));
+  memcpy(c->fd, f, n * sizeof(double));
+  c->f_valid = 1;
+  c->props_valid = 0;
+  return DVM_OK;
+}
+
+int orc_grid_info(orc_ctx* c, int32_t* ints, double* beta3_min) {
+  const ogrid* g = &c->g;
+  ints[0] = g->num_points;
+  for (int d = 0; d < 3; ++d) {
+    ints[1 + 3 * d] = g->zero[d];
+    ints[2 + 3 * d] = g->neg[d];
+    ints[3 + 3 * d] = g->pos[d];
+  }
+  ints[10] = g->n[0];
+  *beta3_min = g->beta3_min;
+  return DVM_OK;
+}
+
 int orc_download_f(orc_ctx* c, int32_t species, double* f) {
   if (species != 0) return set_err(c, DVM_E_ARG, "only species 0 is supported");
   memcpy(f, c->f, (size_t)c->ncell * c->g.num_points * sizeof(double));

@@ -46,6 +46,15 @@ int  orc_wall_heat_flux(orc_ctx* ctx, double* q, int32_t* ntime_out);
 /* Collision phase restricted to cells [c0, c1) (CPU-baseline sampling). */
 int  orc_collide_cells(orc_ctx* ctx, int32_t ntime, int32_t c0, int32_t c1);
 
+/* Test hooks. orc_set_f_keep_cache replaces f (and invalidates the moments)
+   without resetting the equilibrium cache, so that a test can move the
+   cell state between collisions the way convection does; orc_grid_info
+   returns the derived integers of the velocity grid in the layout of the
+   reference's grid fields (num_points, i_zero, i_neg, i_pos, j_zero, j_neg,
+   j_pos, k_zero, k_neg, k_pos, num_points_x) and beta3_min. */
+int  orc_set_f_keep_cache(orc_ctx* ctx, const double* f);
+int  orc_grid_info(orc_ctx* ctx, int32_t* ints /* [11] */, double* beta3_min);
+
 /* Building blocks exposed for unit tests. */
 void orc_uniform_axis(int32_t n, double vmin, double vmax, double* out);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
