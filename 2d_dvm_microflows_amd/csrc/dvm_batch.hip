@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
     const int bad = collide_item<kF32, BS, K>(g, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
                                               P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed});
-    if (bad) atomicOr(P.err, 1);
+    if (bad) atomicOr(P.err, bad);
     __syncthreads();  // LDS axes, accumulator and s_item are reused by the next item
   }
 }
@@ -187,7 +187,8 @@ __global__ void __launch_bounds__(BS) k_conserve_cells(const SlotDesc* __restric
   if (d.full && d.split == 1) return;  // corrected inside the collision kernel
   const int cur = state[r.slot].cur;
   const size_t n0 = cell_node0(d, r.cell);
-  conserve_cell<BS>(d.g, P.f[cur] + n0, P.dacc + n0, P.f[cur ^ 1] + n0, sh);
+  const int flag = conserve_cell<BS>(d.g, P.f[cur] + n0, P.dacc + n0, P.f[cur ^ 1] + n0, sh);
+  if (flag && threadIdx.x == 0) atomicOr(P.err, flag);
 }
 
 // 1-D sweep of the convecting slots: from the post-collision pool back into
@@ -512,8 +513,8 @@ static int check_instance(dvm_batch* b, const dvm_config* inst) {
   if (inst->dim == DVM_DIM_1D && inst->enable_convection &&
       (inst->bc_left == DVM_BC_WALL || inst->bc_right == DVM_BC_WALL))
     return bfail(b, DVM_E_ARG, "batched 1-D solves take in/out flow or zero-gradient ends (1-D walls: dvm_ctx)");
-  if (inst->sample_mode == DVM_SAMPLES_EXPLICIT && inst->samples_per_cell > 2147483647LL)
-    return bfail(b, DVM_E_ARG, "samples_per_cell must fit in 31 bits");
+  if (inst->sample_mode == DVM_SAMPLES_EXPLICIT && inst->samples_per_cell > kMaxCellSamples)
+    return bfail(b, DVM_E_ARG, "samples_per_cell must be <= 2^30");
   return DVM_OK;
 }
 
@@ -811,7 +812,9 @@ static int batch_check_err(dvm_batch* b) {
   BHIP(b, hipStreamSynchronize(b->stream));
   if (h[0]) {
     BHIP(b, hipMemset(b->P.err, 0, sizeof h));
-    return bfail(b, DVM_E_REMAP, "remap found no exterior point (error flags 0x%x)", h[0]);
+    char text[400];
+    const int rc = device_error(h[0], text, sizeof text);
+    return bfail(b, rc, "%s", text);
   }
   return DVM_OK;
 }

@@ -58,6 +58,11 @@ struct dvm_ctx {
   long long *d_sbase = nullptr, *d_stotal = nullptr;
   bool f_valid = false, props_valid = false;
   int trace_ready = -1;
+  // device error flags of earlier dvm_step calls, copied to pinned host
+  // memory behind each step and read at a later step once that copy is done
+  int* err_host = nullptr;
+  hipEvent_t err_ev = nullptr;
+  bool err_pending = false;
   char msg[512] = {0};
 };
 
@@ -148,8 +153,8 @@ static int validate(const dvm_config* cfg) {
     return fail(nullptr, DVM_E_ARG, "invalid sample_mode");
   if (cfg->sample_mode == DVM_SAMPLES_REFERENCE && !(cfg->coln_rms > 0.0))
     return fail(nullptr, DVM_E_ARG, "coln_rms must be > 0 for the reference sample-count rule");
-  if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT && cfg->samples_per_cell < 0)
-    return fail(nullptr, DVM_E_ARG, "samples_per_cell must be >= 0");
+  if (cfg->sample_mode == DVM_SAMPLES_EXPLICIT && (cfg->samples_per_cell < 0 || cfg->samples_per_cell > kMaxCellSamples))
+    return fail(nullptr, DVM_E_ARG, "samples_per_cell must be in [0, 2^30]");
   if (!(cfg->dt > 0.0) || !(cfg->kn > 0.0) || !(cfg->mass > 0.0)) return fail(nullptr, DVM_E_ARG, "dt, kn, mass must be > 0");
   if (cfg->enable_convection && cfg->dim != DVM_DIM_0D && (!(cfg->dx > 0.0) || (cfg->dim == DVM_DIM_2D && !(cfg->dy > 0.0))))
     return fail(nullptr, DVM_E_ARG, "dx (and dy in 2-D) must be > 0");
@@ -172,6 +177,8 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   for (auto& row : c->ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
+  if (c->err_ev) (void)hipEventDestroy(c->err_ev);
+  if (c->err_host) (void)hipHostFree(c->err_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -375,6 +382,9 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
       (rc = dalloc(c, &c->counts, (size_t)c->ncell * 4)) || (rc = dalloc(c, &c->err, 4)))
     return rc;
   HIP_TRY(c, hipMemset(c->err, 0, 4 * sizeof(int)));
+  HIP_TRY(c, hipHostMalloc((void**)&c->err_host, 4 * sizeof(int), hipHostMallocDefault));
+  c->err_host[0] = 0;
+  HIP_TRY(c, hipEventCreateWithFlags(&c->err_ev, hipEventDisableTiming));
   HIP_TRY(c, hipMemset(c->counts, 0, (size_t)c->ncell * 4 * sizeof(int)));
   // the full Monte Carlo kernel samples with f_eq = 0: its partner signs are
   // dsgn(f - 0) = dsgn(f)
@@ -555,13 +565,29 @@ extern "C" int dvm_ctx_create(const dvm_config* cfg, dvm_ctx** out) {
   return DVM_OK;
 }
 
+// Message and status of a device error word (the kErr* bits); 0 if clear.
+static int device_error(int flags, char* text, size_t n) {
+  if (!flags) return DVM_OK;
+  snprintf(text, n, "device error flags 0x%x:%s%s%s%s", flags,
+           (flags & kErrRemap) ? " a post-collision remap found no exterior point (the reference stops, "
+                                 "remapping.f90:348-354);" : "",
+           (flags & kErrSpecRemap) ? " a specular ghost remap found no exterior point;" : "",
+           (flags & kErrSpecNode) ? " a specular ghost stencil point left the grid;" : "",
+           (flags & kErrConserve) ? " the FP32 conservation system of a cell was not positive definite "
+                                    "(the change was kept uncorrected);" : "");
+  return (flags & (kErrRemap | kErrSpecRemap | kErrSpecNode)) ? DVM_E_REMAP : DVM_E_STATE;
+}
+
 static int check_err(dvm_ctx* c) {
   int h[4] = {0, 0, 0, 0};
   HIP_TRY(c, hipMemcpyAsync(h, c->err, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->err_pending = false;  // this read covers every step queued so far
   if (h[0]) {
     HIP_TRY(c, hipMemset(c->err, 0, sizeof h));
-    return fail(c, DVM_E_REMAP, "remap found no exterior point (error flags 0x%x)", h[0]);
+    char text[400];
+    const int rc = device_error(h[0], text, sizeof text);
+    return fail(c, rc, "%s", text);
   }
   return DVM_OK;
 }
@@ -707,7 +733,7 @@ static int collide_async(dvm_ctx* c, int ntime) {
 // then phi := delta_phi.
 static int conserve_async(dvm_ctx* c) {
   if (!collides(c) || !separate_conserve(c)) return DVM_OK;
-  hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd);
+  hipLaunchKernelGGL(k_conserve<256>, dim3(c->ncell), dim3(256), 0, c->stream, c->g, c->f, c->dacc, c->fd, c->err);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;
   return DVM_OK;
@@ -855,11 +881,26 @@ extern "C" int dvm_convect(dvm_ctx* c, int32_t species, int32_t ntime) {
   return check_err(c);
 }
 
+// dvm_step does not wait for the device: the error flags that a step raises
+// are copied to pinned host memory behind it and reported by the first later
+// dvm_step that finds the copy complete (or by any call that synchronises,
+// which checks the flags itself), so a failing step surfaces within a step or
+// two, as the reference's stop would end the run.
 extern "C" int dvm_step(dvm_ctx* c, int32_t ntime) {
   if (!c) return fail(c, DVM_E_ARG, "null context");
   if (!c->f_valid || !c->props_valid) return fail(c, DVM_E_STATE, "step needs f and its moments");
   if (ntime < 0) return fail(c, DVM_E_ARG, "ntime must be >= 0");
-  return run_phases(c, ntime, 0, 6);
+  if (c->err_pending && hipEventQuery(c->err_ev) == hipSuccess) {
+    c->err_pending = false;
+    if (c->err_host[0]) return check_err(c);
+  }
+  if (int rc = run_phases(c, ntime, 0, 6)) return rc;
+  if (!c->err_pending) {
+    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->err, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(c->err_ev, c->stream));
+    c->err_pending = true;
+  }
+  return DVM_OK;
 }
 
 extern "C" int dvm_timing_reset(dvm_ctx* c) {

@@ -18,6 +18,16 @@ constexpr double kPiRef = 3.1459265358979323;
 constexpr double kDoubleTol = 1.0e-13;  // constants.f90:52
 constexpr int kNBucket = 1000;          // search table size, variance_reduction.f90:471-493
 constexpr int kSearchCap = 96;          // bound on bisection steps (>> log2 of any grid)
+// Samples of one cell per collision phase: at most 2^30, each integral's
+// count at most a third of that, so that sample indices, per-cell totals and
+// the lanes' running indices stay within 32-bit integers.
+constexpr int kMaxCellSamples = 1 << 30;
+constexpr int kMaxIntegralSamples = kMaxCellSamples / 3;
+// Device error flags (bit values of the context's / batch's error word):
+constexpr int kErrRemap = 1;       // a post-collision remap found no exterior point (remapping.f90:348-354)
+constexpr int kErrSpecRemap = 2;   // a specular ghost remap found no exterior point
+constexpr int kErrSpecNode = 4;    // a specular ghost stencil point left the grid
+constexpr int kErrConserve = 8;    // the FP32 path's conservation system was not positive definite
 
 // Velocity grid as seen by a kernel. Axes and quadrature weights may live in
 // global memory or be staged into LDS by the caller.

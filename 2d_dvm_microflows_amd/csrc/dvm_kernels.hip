@@ -491,7 +491,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
         num = round((0.5 * cp.dt * ct * ratio * ratio) / (cp.kn * cp.rms_sq * cp.beta3));
       }
       if (!(num > 0.0) || !(abs_dens > 0.0)) num = 0.0;
-      if (num > 2147483647.0) num = 2147483647.0;
+      if (num > (double)kMaxCellSamples) num = (double)kMaxCellSamples;
       const int n = (int)num;
       cnt4[0] = 0; cnt4[1] = 0; cnt4[2] = n; cnt4[3] = n;
       fac3[0] = 0.0; fac3[1] = 0.0;
@@ -516,7 +516,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
         num = round((0.5 * cp.dt * ct * dA[t] * dB[t]) / (cp.kn * cp.rms_sq * cp.beta3));
       }
       if (!(num > 0.0)) num = 0.0;
-      if (num > 2147483647.0) num = 2147483647.0;
+      if (num > (double)kMaxIntegralSamples) num = (double)kMaxIntegralSamples;
       const int n = (int)num;
       cnt4[t] = n;
       sum += n;
@@ -851,7 +851,8 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
   return true;
 }
 
-// Collision phase of one cell per workgroup of BS lanes.
+// Collision phase of one cell per workgroup of BS lanes. Returns device error
+// bits (kErrRemap for a remap without an exterior point).
 //   kLds: the cell's increments accumulate in LDS (dynamic shared memory after
 //         the axes); otherwise fd (a copy of f) takes global FP64 atomics.
 //   kF32: FP32 collision arithmetic; the FP64-accumulated change is written to
@@ -919,10 +920,11 @@ struct ConserveShared {
   double tot[20];
   double M[25];
   double lam[5];
+  int flag;
 };
 
 template <int BS>
-__device__ __forceinline__ void conserve_cell(const Grid& g, const double* __restrict__ fc,
+__device__ __forceinline__ int conserve_cell(const Grid& g, const double* __restrict__ fc,
                                               const double* __restrict__ da, double* __restrict__ out,
                                               ConserveShared<BS>& sh) {
   const int N = g.N;
@@ -974,10 +976,14 @@ __device__ __forceinline__ void conserve_cell(const Grid& g, const double* __res
     for (int a = 0; a < 5; ++a)
       for (int c = a; c < 5; ++c) { M[a * 5 + c] = tot[o]; M[c * 5 + a] = tot[o]; ++o; }
     for (int a = 0; a < 5; ++a) lam[a] = tot[a];
-    if (!solve5(M, lam))
+    sh.flag = 0;
+    if (!solve5(M, lam)) {
       for (int a = 0; a < 5; ++a) lam[a] = 0.0;
+      sh.flag = kErrConserve;
+    }
   }
   __syncthreads();
+  const int flag = sh.flag;
   const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
   for (int l = threadIdx.x; l < N; l += BS) {
     int i, j, k;
@@ -987,6 +993,7 @@ __device__ __forceinline__ void conserve_cell(const Grid& g, const double* __res
     const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
     out[l] = fl + (da[l] - corr);
   }
+  return flag;
 }
 
 // LDS accumulation window of a cell: a box of w[0] x w[1] x w[2] nodes, the
@@ -1059,10 +1066,7 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
                                        });
   }
   __syncthreads();
-  if (kF32 && full && sole) {
-    conserve_cell<BS>(g, cv.f, acc, fd, *csh);
-    return bad;
-  }
+  if (kF32 && full && sole) return bad | conserve_cell<BS>(g, cv.f, acc, fd, *csh);
   const int nx = g.n[0], nxy = g.nxy;
   for (int q = threadIdx.x; q < wn; q += BS) {
     int l = q;
@@ -1134,15 +1138,17 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src);
-  if (bad) atomicOr(err, 1);
+  if (bad) atomicOr(err, bad);
 }
 
 template <int BS>
 __global__ void __launch_bounds__(BS) k_conserve(Grid g, const double* __restrict__ f,
-                                                 const double* __restrict__ dacc, double* __restrict__ fd) {
+                                                 const double* __restrict__ dacc, double* __restrict__ fd,
+                                                 int* __restrict__ err) {
   __shared__ ConserveShared<BS> sh;
   const size_t off = (size_t)blockIdx.x * g.N;
-  conserve_cell<BS>(g, f + off, dacc + off, fd + off, sh);
+  const int flag = conserve_cell<BS>(g, f + off, dacc + off, fd + off, sh);
+  if (flag && threadIdx.x == 0) atomicOr(err, flag);
 }
 
 // Offsets of each cell's samples in an explicit uniform list: cells in order,
@@ -1333,12 +1339,12 @@ __global__ void k_spec_ghost(Grid g, ConvParams p, const double* __restrict__ f,
   if (side < 2) x = -x; else y = -y;
   Stencil s;
   remap(g, x, y, z, 1.0, s);
-  if (s.flag == 0) { atomicOr(err, 2); return; }
+  if (s.flag == 0) { atomicOr(err, kErrSpecRemap); return; }
   double* gh = spec + ((size_t)side * p.max_along + pos) * g.N;
   for (int q = 0; q < 7; ++q) {
     if (q < 4 || (s.flag & (1 << (q - 4)))) {
       const int node = stencil_node(s, q, g.n[0], g.nxy);
-      if ((unsigned)node < (unsigned)g.N) atomicAdd(&gh[node], dens * s.m[q]); else atomicOr(err, 4);
+      if ((unsigned)node < (unsigned)g.N) atomicAdd(&gh[node], dens * s.m[q]); else atomicOr(err, kErrSpecNode);
     }
   }
 }
@@ -1631,7 +1637,7 @@ __global__ void k_spec1d(Grid g, ConvParams p, const double* __restrict__ f, dou
   const double ds = (1.0 - p.acc) * f0;
   Stencil st;
   remap(g, -g.ax[0][i], g.ax[1][j], g.ax[2][k], 1.0, st);
-  if (st.flag == 0) { atomicOr(err, 2); return; }
+  if (st.flag == 0) { atomicOr(err, kErrSpecRemap); return; }
   double* sp = spec + (size_t)side * N;
 #pragma unroll
   for (int q = 0; q < 7; ++q) {

@@ -670,7 +670,8 @@ static void pair_counts(orc_ctx* c, int cell, cell_coll* cc) {
       num = fnint((0.5 * cfg->dt * ct * dA[t] * dB[t]) / (cfg->kn * rms_sq * beta3));
     }
     if (num < 0) num = 0;
-    if (num > 0x7fffffffLL) num = 0x7fffffffLL;
+    /* the library's bound on one integral's samples (a third of 2^30) */
+    if (num > (1LL << 30) / 3) num = (1LL << 30) / 3;
     cc->num[t] = (int)num;
     cc->factor[t] = (num > 0) ? factor_coeff * c->sigma * wA[t] * wB[t] / (double)num : 0.0;
   }
@@ -725,7 +726,7 @@ static void fmc_counts(orc_ctx* c, int cell, cell_coll* cc) {
     num = (x > 0.0) ? fnint(x) : 0; /* the reference's nint of NaN (empty cell) is left undefined */
   }
   if (!(abs_dens > 0.0) || num < 0) num = 0;
-  if (num > 0x7fffffffLL) num = 0x7fffffffLL;
+  if (num > (1LL << 30)) num = 1LL << 30; /* the library's bound on a cell's samples */
   cc->num[2] = (int)num;
   if (num > 0) {
     double factor = (cfg->dt / cfg->kn) * c->sigma * abs_dens * abs_dens / (double)num;

@@ -58,3 +58,91 @@ def golden_cases(name):
         case, field = k.split("__", 1)
         cases.setdefault(case, {})[field] = v
     return cases
+
+
+class CellParallelOracle:
+    """The oracle time step over many cells on several host threads (the
+    collision phase is per cell; ctypes releases the GIL during the call).
+    Context k collides the cells of its range and keeps their equilibrium
+    caches; context 0 convects the assembled f; every context then takes the
+    new f with its cache kept and recomputes the moments. The result equals
+    orc_step on one context: the collision of a cell reads only that cell's
+    f, moments and cache, and the counter-based samples are keyed by the
+    global cell index."""
+
+    def __init__(self, cfg, threads=8):
+        import ctypes as C
+        import numpy as np
+        self.np = np
+        self.C = C
+        lib = load_oracle()
+        lib.orc_set_f_keep_cache.restype = C.c_int
+        lib.orc_set_f_keep_cache.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        self.lib = lib
+        self.ncell = cfg.nx * cfg.ny
+        threads = max(1, min(int(threads), self.ncell))
+        self.ctx = [oracle_solver(cfg) for _ in range(threads)]
+        edges = np.linspace(0, self.ncell, threads + 1).astype(int)
+        self.ranges = list(zip(edges[:-1], edges[1:]))
+        self.f = None
+
+    def _parallel(self, fn):
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(len(self.ctx)) as ex:
+            return list(ex.map(fn, range(len(self.ctx))))
+
+    def _share(self, f):
+        f = self.np.ascontiguousarray(f, dtype=self.np.float64)
+        ptr = f.ctypes.data_as(self.C.POINTER(self.C.c_double))
+
+        def put(k):
+            s = self.ctx[k]
+            if self.lib.orc_set_f_keep_cache(s.h, ptr) != 0:
+                raise RuntimeError("orc_set_f_keep_cache failed")
+            return s.calculate_physical_properties()
+
+        props = self._parallel(put)
+        self.f = f
+        return props[0]
+
+    def initialize(self, f):
+        props = self._parallel(lambda k: self.ctx[k].initialize(f))
+        self.f = self.np.ascontiguousarray(f, dtype=self.np.float64).reshape(self.ncell, -1)
+        return props[0]
+
+    def collide(self, ntime):
+        """The collision phase of every cell; returns the post-collision f
+        (the step is finished by finish_step)."""
+        def collide(k):
+            c0, c1 = self.ranges[k]
+            rc = self.lib.orc_collide_cells(self.ctx[k].h, int(ntime), int(c0), int(c1))
+            if rc != 0:
+                raise RuntimeError(f"orc_collide_cells: {rc}")
+            return self.ctx[k].download_f()[c0:c1]
+
+        parts = self._parallel(collide)
+        self.fc = self.np.concatenate(parts, axis=0)
+        return self.fc.copy()
+
+    def finish_step(self, ntime):
+        """Convection of the post-collision f, then the moments everywhere."""
+        fc = self.fc
+        main = self.ctx[0]
+        if self.lib.orc_set_f_keep_cache(main.h, fc.ctypes.data_as(self.C.POINTER(self.C.c_double))) != 0:
+            raise RuntimeError("orc_set_f_keep_cache failed")
+        # as orc_step: the sweep runs on the post-collision f with the moments
+        # of the start of the step still in place
+        main.compute_convection(ntime)
+        self.props = self._share(main.download_f())
+        return self.props
+
+    def step(self, ntime):
+        self.collide(ntime)
+        return self.finish_step(ntime)
+
+    def download_f(self):
+        return self.f.copy()
+
+    def close(self):
+        for s in self.ctx:
+            s.close()

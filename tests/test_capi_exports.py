@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -44,3 +46,31 @@ def test_create_without_device_reports_error(pkg):
     cfg, _ = pkg.homogeneous_relaxation(n_v=5, samples=10)
     with pytest.raises(torch_free.DvmError):
         pkg.Solver(cfg)
+
+
+def test_sample_count_bound_is_checked_before_any_device_call(pkg):
+    """Per-cell sample counts are bounded by 2^30 so that sample indices and
+    per-cell totals stay within 32-bit integers on the device; a larger
+    explicit count is refused by the argument checks, which run before any
+    HIP call."""
+    import importlib
+    dvm = importlib.import_module("2d_dvm_microflows_amd.dvm")
+    cfg, _ = pkg.homogeneous_relaxation(n_v=5, samples=(1 << 30) + 1)
+    with pytest.raises(dvm.DvmError) as e:
+        pkg.Solver(cfg)
+    assert "2^30" in str(e.value)
+    b = pkg.Batch(cfg.with_(samples_per_cell=10), max_slots=1, node_capacity=125, cell_capacity=1) \
+        if _gpu_visible() else None
+    if b is not None:
+        with pytest.raises(dvm.DvmError):
+            b.admit(cfg, np.ones((1, 125)))
+        b.close()
+
+
+def _gpu_visible():
+    n = C.c_int(0)
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return False
+    return hip.hipGetDeviceCount(C.byref(n)) == 0 and n.value > 0
