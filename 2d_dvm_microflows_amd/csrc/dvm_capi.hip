@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -63,8 +64,41 @@ struct dvm_ctx {
   int* err_host = nullptr;
   hipEvent_t err_ev = nullptr;
   bool err_pending = false;
+  double* scratch = nullptr;     // this device's address of the file-scope scratch array
   char msg[512] = {0};
 };
+
+// Scratch of the allocation-free entry point (dvm_l1_noalloc): a file-scope
+// device array, one copy per GPU the library runs on, held until the process
+// ends. Host code never passes `scratch` itself (in host code the name is a
+// stand-in, not a device address): each device's address is looked up once,
+// with that device current, and kept in g_scratch_ptr.
+constexpr long long kScratchDoubles = 1LL << 20;  // 8 MiB: the L1 staging of up to 2^20 cells
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch` (looked up on the first call on
+// that device); nullptr with a message on failure.
+static double* scratch_address(char* msg, size_t n) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    snprintf(msg, n, "scratch: no current device below %d", kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      snprintf(msg, n, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
 
 static char g_create_msg[512] = "";
 
@@ -419,6 +453,9 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     if (hipGetDevice(&dev) == hipSuccess)
       (void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev);
   }
+  // the device's scratch address, for dvm_l1_noalloc (looked up here, at set-up)
+  c->scratch = scratch_address(c->msg, sizeof c->msg);
+  if (!c->scratch) return DVM_E_HIP;
   // Few cells: split each cell's samples over several workgroups, at least
   // two rounds of workgroups over the chip (and at least 4096 samples per part
   // when the count is known up front). Among the splits up to four times that,
@@ -855,6 +892,26 @@ extern "C" int dvm_l1(dvm_ctx* c, double* l1) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d_l1);
   if (e != hipSuccess) return fail(c, DVM_E_HIP, "dvm_l1: %s", hipGetErrorString(e));
+  return DVM_OK;
+}
+
+// dvm_l1 without any device allocation: the per-cell results are staged in
+// the device's file-scope scratch array (every entry the kernel reads it
+// writes first, so nothing is cleared). Calls that overlap on one GPU share
+// that array and race.
+extern "C" int dvm_l1_noalloc(dvm_ctx* c, double* l1) {
+  if (!c || !l1) return fail(c, DVM_E_ARG, "null argument");
+  if (!c->f_valid) return fail(c, DVM_E_STATE, "L1 before upload");
+  if ((long long)c->ncell > kScratchDoubles)
+    return fail(c, DVM_E_ARG, "dvm_l1_noalloc: %d cells exceed its scratch of %lld cells (use dvm_l1)", c->ncell,
+                kScratchDoubles);
+  if (!c->scratch) return fail(c, DVM_E_STATE, "dvm_l1_noalloc: no scratch address for this device");
+  if (int rc = run_phases(c, 0, 6, 6)) return rc;
+  hipLaunchKernelGGL(k_l1, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props, c->cfg.mass,
+                     c->scratch);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(l1, c->scratch, (size_t)c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   return DVM_OK;
 }
 

@@ -236,6 +236,15 @@ int dvm_collide_trace(dvm_ctx* ctx, int32_t ntime, int32_t cell, int64_t max_rec
    of space-homogeneous relaxation runs and their stopping rule. */
 int dvm_l1(dvm_ctx* ctx, double* l1);
 
+/* dvm_l1 for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving and relaxation loops, harnesses that forbid
+   hipMalloc): the same results, staged in a file-scope device scratch array
+   of 2^20 doubles that the library holds on each GPU it runs on until the
+   process ends, instead of a buffer allocated per call. Refuses contexts with
+   more than 2^20 cells (DVM_E_ARG). Calls that overlap on one GPU share the
+   array and race. No counterpart in the reference. */
+int dvm_l1_noalloc(dvm_ctx* ctx, double* l1);
+
 /* Parity mode: the collision phase takes its uniforms from the explicit list
    u[0..n) instead of the counter-based generator, so that one list can be fed
    to this library and to a CPU implementation. Each collision reads the list
