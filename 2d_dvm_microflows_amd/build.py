@@ -39,12 +39,14 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 
 
 def build_fortran(verbose=False):
-    """bin/dvm_gpu from fortran/dvm_hip.f90, dvm_deck.f90 and dvm_gpu.f90,
+    """bin/dvm_gpu from fortran/dvm_hip.f90, dvm_deck.f90, screen_output.f90,
+    dvm_solver.f90 (the DVMSolver and ScreenOutput modules) and dvm_gpu.f90,
     linked against lib/libdvmhip.so with an $ORIGIN-relative rpath. Returns
     None (and builds nothing) when amdflang is not installed."""
     if not os.path.exists(FLANG):
         return None
-    src = [os.path.join(HERE, "fortran", n) for n in ("dvm_hip.f90", "dvm_deck.f90", "dvm_gpu.f90")]
+    src = [os.path.join(HERE, "fortran", n) for n in ("dvm_hip.f90", "dvm_deck.f90", "screen_output.f90",
+                                                       "dvm_solver.f90", "dvm_gpu.f90")]
     bin_dir = os.path.join(HERE, "bin")
     mod_dir = os.path.join(bin_dir, "mod")
     os.makedirs(mod_dir, exist_ok=True)

@@ -10,10 +10,13 @@
 ! first entry: the library runs one monatomic species.
 !
 ! Supported: num_species = 1 of He, Ne, Ar, Xe (species_data.f90:245-524),
-! csection_method 1-4, csection_coeff 1-3, uniform velocity grids
-! (x/y/z_vel_grid_type = 1), iColl 0, 2 or 3, iConv 0 or 1, InitC 1, 2, 3, 5
-! with InitDF = 1 (Maxwellian), LWallBC / RWallBC 1-4. Anything else stops
-! with a message, as the reference stops on an invalid deck value.
+! csection_method 1-4, csection_coeff 1-3, uniform velocity axes
+! (x/y/z_vel_grid_type = 1) and axes read from the velocity-grid file of the
+! command line (type 0, read_input_2D.f90:779-829), iColl 0, 2 or 3, iConv 0
+! or 1, InitC 1-5 (4: the Rankine-Hugoniot shock profile) with InitDF 1-4
+! (Maxwellian, BKW, delta function, exact Gaussian), LWallBC / RWallBC 1-4.
+! Anything else stops with a message, as the reference stops on an invalid
+! deck value.
 !
 ! Extensions (keys the reference does not read):
 !   dvm_precision    64 (default) or 32: FP32 collision arithmetic
@@ -229,15 +232,20 @@ contains
   ! The deck's run: scaling (scaling.f90:96-130), species constants scaled by
   ! the reference species (species_data.f90:155-229), grids, boundary states
   ! and the initial distribution (initial_conditions.f90:212-650).
-  subroutine run_from_deck(d, r, ok, msg)
+  subroutine run_from_deck(d, r, ok, msg, grid_file)
     type(deck_t), intent(in) :: d
     type(run_t), intent(inout) :: r
     logical, intent(out) :: ok
     character(len=*), intent(out) :: msg
+    ! the velocity-grid file of the command line (command_line.f90:45-86), read
+    ! for the axes whose x/y/z_vel_grid_type is 0 (read_input_2D.f90:779-829)
+    character(len=*), intent(in), optional :: grid_file
     character(len=16) :: ref_species
     real(c_double) :: dref, mref, d_s, m_s, om, al, vis, tvis, dd, mm, oo, aa, vv, tt
     real(c_double) :: vmin(3), vmax(3), dens, u, v, w, temp, opt, coeff, s, x, y, z, b
-    integer :: npt(3), method, icoll, iconv, initc, mid, nx, ny, ncell, n, i, j, k, c, ix, iy, gt
+    integer :: npt(3), method, icoll, iconv, initc, initdf, mid, nx, ny, ncell, n, i, j, k, c, ix, iy, gt, gts(3)
+    real(c_double), allocatable :: ga(:), gb(:), gc(:), sdens(:), su(:), stemp(:)
+    real(c_double) :: xk, bnorm, c2, x1, x2, y1, y2, z1, z2, dl(3)
     logical :: sok
     type(dvm_config) :: cfg
     character(len=2), parameter :: side(4) = ['LW', 'RW', 'BW', 'TW']
@@ -312,26 +320,57 @@ contains
     npt = [deck_int(d, 'num_points_x', 11), deck_int(d, 'num_points_y', 11), deck_int(d, 'num_points_z', 11)]
     vmin = [deck_real(d, 'xv_min', -3.0d0), deck_real(d, 'yv_min', -3.0d0), deck_real(d, 'zv_min', -3.0d0)]
     vmax = [deck_real(d, 'xv_max', 3.0d0), deck_real(d, 'yv_max', 3.0d0), deck_real(d, 'zv_max', 3.0d0)]
+    gts = [deck_int(d, 'x_vel_grid_type', 1), deck_int(d, 'y_vel_grid_type', 1), deck_int(d, 'z_vel_grid_type', 1)]
     do k = 1, 3
-       gt = 1
-       if (k == 1) gt = deck_int(d, 'x_vel_grid_type', 1)
-       if (k == 2) gt = deck_int(d, 'y_vel_grid_type', 1)
-       if (k == 3) gt = deck_int(d, 'z_vel_grid_type', 1)
-       if (gt /= 1) then
-          msg = 'only uniform velocity grids (vel_grid_type = 1) are supported'
+       if (gts(k) /= 0 .and. gts(k) /= 1) then
+          msg = 'x/y/z_vel_grid_type must be 0 (read from the velocity-grid file) or 1 (uniform)'
           return
        end if
     end do
-    allocate(r%vx(npt(1)), r%vy(npt(2)), r%vz(npt(3)))
-    do i = 0, npt(1) - 1
-       r%vx(i + 1) = vmin(1) + dble(i) * (vmax(1) - vmin(1)) / dble(npt(1) - 1)
-    end do
-    do i = 0, npt(2) - 1
-       r%vy(i + 1) = vmin(2) + dble(i) * (vmax(2) - vmin(2)) / dble(npt(2) - 1)
-    end do
-    do i = 0, npt(3) - 1
-       r%vz(i + 1) = vmin(3) + dble(i) * (vmax(3) - vmin(3)) / dble(npt(3) - 1)
-    end do
+    if (any(gts == 0)) then
+       if (.not. present(grid_file)) then
+          msg = 'a velocity grid type 0 needs the velocity-grid file (second command-line argument)'
+          return
+       end if
+       if (len_trim(grid_file) == 0) then
+          msg = 'a velocity grid type 0 needs the velocity-grid file (second command-line argument)'
+          return
+       end if
+       call read_grid_file(grid_file, ga, gb, gc, sok, msg)
+       if (.not. sok) return
+    end if
+    ! generate_uniform_grid (velocity_grid.f90:636-648) or the file's axis
+    ! (generate_file_read_grid)
+    if (gts(1) == 0) then
+       npt(1) = size(ga)
+       allocate(r%vx(npt(1)))
+       r%vx = ga
+    else
+       allocate(r%vx(npt(1)))
+       do i = 0, npt(1) - 1
+          r%vx(i + 1) = vmin(1) + dble(i) * (vmax(1) - vmin(1)) / dble(npt(1) - 1)
+       end do
+    end if
+    if (gts(2) == 0) then
+       npt(2) = size(gb)
+       allocate(r%vy(npt(2)))
+       r%vy = gb
+    else
+       allocate(r%vy(npt(2)))
+       do i = 0, npt(2) - 1
+          r%vy(i + 1) = vmin(2) + dble(i) * (vmax(2) - vmin(2)) / dble(npt(2) - 1)
+       end do
+    end if
+    if (gts(3) == 0) then
+       npt(3) = size(gc)
+       allocate(r%vz(npt(3)))
+       r%vz = gc
+    else
+       allocate(r%vz(npt(3)))
+       do i = 0, npt(3) - 1
+          r%vz(i + 1) = vmin(3) + dble(i) * (vmax(3) - vmin(3)) / dble(npt(3) - 1)
+       end do
+    end if
     cfg%nvx = npt(1)
     cfg%nvy = npt(2)
     cfg%nvz = npt(3)
@@ -459,18 +498,27 @@ contains
     r%display_freq = max(1, deck_int(d, 'display_freq', 1))
     r%out_x = min(max(deck_int(d, 'screen_output_loc', 1, 1), 1), nx)
     r%out_y = min(max(deck_int(d, 'screen_output_loc', 1, 2), 1), ny)
-    ! initial distribution: Maxwellian x beta^3 per cell at the InitC state,
-    ! renormalised to the cell's density (initial_conditions.f90:338-650)
-    if (deck_int(d, 'InitDF', 1) /= 1) then
-       msg = 'only InitDF = 1 (Maxwellian) is supported'
+    ! initial distribution (initialize_dist_func, initial_conditions.f90:212-650):
+    ! per cell a state from InitC, the InitDF shape times beta^3 at every node,
+    ! renormalised to the cell's density
+    initdf = deck_int(d, 'InitDF', 1)
+    if (initdf < 1 .or. initdf > 4) then
+       msg = 'InitDF must be 1 (Maxwellian), 2 (BKW), 3 (delta function) or 4 (exact Gaussian)'
        return
     end if
     initc = deck_int(d, 'InitC', 5)
     mid = deck_int(d, 'spatial_middle', nx / 2)
-    if (initc < 1 .or. initc > 5 .or. initc == 4) then
-       msg = 'InitC must be 1, 2, 3 or 5 (the Rankine-Hugoniot profile, 4, is not supported)'
+    if (initc < 1 .or. initc > 5) then
+       msg = 'InitC must be 1 (left wall), 2 (right wall), 3 (left | right), 4 (Rankine-Hugoniot shock) or 5 (user input)'
        return
     end if
+    ! the state of each column ix (the profile depends on x only)
+    allocate(sdens(nx), su(nx), stemp(nx))
+    if (initc == 4) then
+       call shock_profile(cfg, deck_int(d, 'shock_thickness', 1), mid, sdens, su, stemp, sok, msg)
+       if (.not. sok) return
+    end if
+    dl = [deck_real(d, 'x_delta_loc', 0.0d0), deck_real(d, 'y_delta_loc', 0.0d0), deck_real(d, 'z_delta_loc', 0.0d0)]
     ncell = nx * ny
     n = npt(1) * npt(2) * npt(3)
     allocate(r%f0(n, ncell))
@@ -488,6 +536,9 @@ contains
              else
                 dens = cfg%wall(1, 1); u = cfg%wall(2, 1); v = cfg%wall(3, 1); w = cfg%wall(4, 1); temp = cfg%wall(5, 1)
              end if
+          case (4)
+             ! v and w stay the left wall's (initial_conditions.f90:338-347)
+             dens = sdens(ix); u = su(ix); v = cfg%wall(3, 1); w = cfg%wall(4, 1); temp = stemp(ix)
           case default
              dens = deck_real(d, 'dens_init', 1.0d0)
              u = deck_real(d, 'x_vel_init', 0.0d0)
@@ -501,21 +552,66 @@ contains
           end if
           opt = cfg%mass / (pi_ref * temp)
           coeff = dens * sqrt(opt * opt * opt)
+          ! compute_bkw at time 0 (initial_conditions.f90:483-485)
+          xk = 1.0d0 - 0.4d0 * exp(-0.0d0 / 6.0d0)
+          bnorm = 0.5d0 * dens * sqrt(opt * opt * opt) * xk**(-2.5d0)
+          c2 = cfg%mass / temp
           s = 0.0d0
           i = 0
           do k = 1, npt(3)
              z = r%vz(k)
+             z1 = z - beta(r%vz, k)
+             if (k > 1) z1 = r%vz(k - 1)
+             z2 = z + beta(r%vz, k)
+             if (k < npt(3)) z2 = r%vz(k + 1)
              do j = 1, npt(2)
                 y = r%vy(j)
+                y1 = y - beta(r%vy, j)
+                if (j > 1) y1 = r%vy(j - 1)
+                y2 = y + beta(r%vy, j)
+                if (j < npt(2)) y2 = r%vy(j + 1)
                 do gt = 1, npt(1)
                    x = r%vx(gt)
                    i = i + 1
                    b = beta(r%vx, gt) * beta(r%vy, j) * beta(r%vz, k)
-                   r%f0(i, c) = coeff * exp(-((x - u)**2 + (y - v)**2 + (z - w)**2) * cfg%mass / temp) * b
+                   select case (initdf)
+                   case (1)
+                      ! compute_maxwellian (dist_func.f90:317-344)
+                      opt = (x - u)**2 + (y - v)**2 + (z - w)**2
+                      r%f0(i, c) = coeff * exp(-opt * cfg%mass / temp)
+                   case (2)
+                      ! compute_bkw (dist_func.f90:374-401)
+                      opt = (x - u)**2 + (y - v)**2 + (z - w)**2
+                      r%f0(i, c) = bnorm * (5.0d0 * xk - 3.0d0 + 2.0d0 * (1.0d0 - xk) * opt * cfg%mass / (xk * temp)) * &
+                           exp(-opt * cfg%mass / (xk * temp))
+                   case (3)
+                      ! a delta function at the node (x/y/z_delta_loc)
+                      if (abs(dl(1) - x) < 1.0d-13 .and. abs(dl(2) - y) < 1.0d-13 .and. abs(dl(3) - z) < 1.0d-13) then
+                         r%f0(i, c) = dens / b
+                      else
+                         r%f0(i, c) = 0.0d0
+                      end if
+                   case default
+                      ! compute_exact_gaussian (dist_func.f90:346-372): the
+                      ! Maxwellian integrated over the node's cell
+                      x1 = x - beta(r%vx, gt)
+                      if (gt > 1) x1 = r%vx(gt - 1)
+                      x2 = x + beta(r%vx, gt)
+                      if (gt < npt(1)) x2 = r%vx(gt + 1)
+                      r%f0(i, c) = 0.125d0 * coeff * (pi_ref / c2)**(1.5d0) * &
+                           (erf(sqrt(c2) * (x + 0.5d0 * (x2 - x) - u)) - erf(sqrt(c2) * (x1 + 0.5d0 * (x - x1) - u))) * &
+                           (erf(sqrt(c2) * (y1 + 0.5d0 * (y - y1) - v)) - erf(sqrt(c2) * (y + 0.5d0 * (y2 - y) - v))) * &
+                           (erf(sqrt(c2) * (z1 + 0.5d0 * (z - z1) - w)) - erf(sqrt(c2) * (z + 0.5d0 * (z2 - z) - w)))
+                   end select
+                   r%f0(i, c) = r%f0(i, c) * b
                    s = s + r%f0(i, c)
                 end do
              end do
           end do
+          if (s == 0.0d0) then
+             msg = 'the initial distribution of a cell is zero everywhere (a delta location that is not a node?)'
+             return
+          end if
           r%f0(:, c) = (dens / s) * r%f0(:, c)
        end do
     end do
@@ -524,6 +620,155 @@ contains
   end subroutine run_from_deck
 
   ! First node with |a| < double_tol (1e-13), 0 if none.
+  ! read_vel_grid (read_input_2D.f90:779-829) for one species: the point
+  ! counts of x, y and z, then the x, y and z values, one per record
+  subroutine read_grid_file(fname, ax, ay, az, ok, msg)
+    character(len=*), intent(in) :: fname
+    real(c_double), allocatable, intent(out) :: ax(:), ay(:), az(:)
+    logical, intent(out) :: ok
+    character(len=*), intent(out) :: msg
+    integer :: u, ios, n(3), i
+    ok = .false.
+    open(newunit=u, file=trim(fname), status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       msg = 'cannot open the velocity-grid file ' // trim(fname)
+       return
+    end if
+    read(u, *, iostat=ios) n(1)
+    if (ios == 0) read(u, *, iostat=ios) n(2)
+    if (ios == 0) read(u, *, iostat=ios) n(3)
+    if (ios /= 0 .or. any(n < 2)) then
+       msg = 'the velocity-grid file must start with three point counts >= 2'
+       close(u)
+       return
+    end if
+    allocate(ax(n(1)), ay(n(2)), az(n(3)))
+    do i = 1, n(1)
+       if (ios == 0) read(u, *, iostat=ios) ax(i)
+    end do
+    do i = 1, n(2)
+       if (ios == 0) read(u, *, iostat=ios) ay(i)
+    end do
+    do i = 1, n(3)
+       if (ios == 0) read(u, *, iostat=ios) az(i)
+    end do
+    close(u)
+    if (ios /= 0) then
+       msg = 'the velocity-grid file ends before its point values'
+       return
+    end if
+    ok = .true.
+  end subroutine read_grid_file
+
+  ! InitC = 4: the shock jump of initialize_normal_shock_props
+  ! (shock_conditions.f90:54-171) for one monatomic species (3 d.o.f., so
+  ! gamma = cp / (cp - 1) = 5/3, shock_conditions.f90:255-314), applied to the
+  ! left wall (initial_conditions.f90:309-336), and the per-column profile
+  ! between the two walls (:394-454, initial_shock_profile :652-671).
+  subroutine shock_profile(cfg, thick, space_mid, dens_x, u_x, temp_x, ok, msg)
+    type(dvm_config), intent(inout) :: cfg
+    integer, intent(in) :: thick, space_mid
+    real(c_double), intent(out) :: dens_x(:), u_x(:), temp_x(:)
+    logical, intent(out) :: ok
+    character(len=*), intent(out) :: msg
+    real(c_double) :: dens_up, t_up, u_up, gamma, a_up, a, b, c, u_s, m_inf, t_ratio, dens_ratio, m_down
+    real(c_double) :: a_down, u_down, dens_lw, u_lw, temp_lw, dens_rw, u_rw, temp_rw, dl, ul, tl, ratio, cp
+    integer :: nx, half, dmid, tmid
+    logical :: reset
+    ok = .false.
+    nx = size(dens_x)
+    ! calculate_upstream_conditions: the right wall, or the left when the
+    ! right wall's flow is positive
+    dens_up = cfg%wall(1, 2); u_up = cfg%wall(2, 2); t_up = cfg%wall(5, 2)
+    if (u_up > 0.0d0) then
+       dens_up = cfg%wall(1, 1); u_up = cfg%wall(2, 1); t_up = cfg%wall(5, 1)
+    end if
+    cp = 3.0d0 * 0.5d0 + 1.0d0
+    gamma = cp / (cp - 1.0d0)
+    a_up = sqrt(gamma * t_up / (2.0d0 * cfg%mass))
+    if (cfg%bc_left == DVM_BC_WALL .or. cfg%bc_right == DVM_BC_WALL) then
+       ! a shock moving away from a reflecting wall
+       a = 1.0d0
+       b = 0.5d0 * (3.0d0 - gamma) * u_up
+       c = -0.5d0 * (gamma + (gamma - 1.0d0) * u_up * u_up)
+       u_s = abs((-b - sqrt(b * b - 4.0d0 * a * c)) / (2.0d0 * a))
+       m_inf = (abs(u_up) + u_s) / a_up
+    else
+       if (u_up > 0.0d0) then
+          msg = 'InitC 4: the flow of a stationary shock must be negative (towards -x)'
+          return
+       end if
+       u_s = 0.0d0
+       m_inf = abs(u_up) / a_up
+    end if
+    if (m_inf < 1.0d-13) then
+       t_ratio = 1.0d0; dens_ratio = 1.0d0; u_down = 0.0d0
+    else
+       t_ratio = (1.0d0 + 2.0d0 * gamma * (m_inf * m_inf - 1.0d0) / (gamma + 1.0d0)) * &
+            (2.0d0 + (gamma - 1.0d0) * m_inf * m_inf) / ((gamma + 1.0d0) * m_inf * m_inf)
+       dens_ratio = (gamma + 1.0d0) * m_inf * m_inf / (2.0d0 + (gamma - 1.0d0) * m_inf * m_inf)
+       m_down = sqrt((1.0d0 + 0.5d0 * (gamma - 1.0d0) * m_inf * m_inf) / &
+            (gamma * m_inf * m_inf - 0.5d0 * (gamma - 1.0d0)))
+       a_down = sqrt(gamma * t_ratio * t_up / (2.0d0 * cfg%mass))
+       u_down = sign(1.0d0, u_up) * a_down * m_down + u_s
+    end if
+    dens_rw = cfg%wall(1, 2); u_rw = cfg%wall(2, 2); temp_rw = cfg%wall(5, 2)
+    dens_lw = dens_ratio * dens_rw
+    u_lw = u_down
+    temp_lw = t_ratio * temp_rw
+    cfg%wall(1, 1) = dens_lw
+    cfg%wall(2, 1) = u_lw
+    cfg%wall(5, 1) = temp_lw
+    half = ceiling(0.5d0 * dble(thick))
+    if (half < 1) then
+       msg = 'InitC 4 needs shock_thickness >= 1'
+       return
+    end if
+    dmid = space_mid
+    tmid = ceiling(dble(space_mid) + 0.5d0 * dble(half))
+    dl = dens_lw; ul = u_lw; tl = temp_lw
+    reset = .false.
+    do nx = 1, size(dens_x)
+       if (.not. reset) then
+          if (nx > dmid + half) then
+             dl = dens_rw; ul = u_rw
+          else if (nx > dmid) then
+             dl = profile(dens_lw, dens_rw, half, dmid, nx)
+             ratio = (dl - dens_rw) / (dens_lw - dens_rw)
+             ul = u_rw + (u_lw - u_rw) * ratio
+          else if (nx >= dmid - half) then
+             dl = profile(dens_rw, dens_lw, half, nx, dmid)
+             ratio = (dl - dens_rw) / (dens_lw - dens_rw)
+             ul = u_rw + (u_lw - u_rw) * ratio
+          else
+             dl = dens_lw; ul = u_lw
+          end if
+          if (nx > tmid + half) then
+             tl = temp_rw
+             reset = .true.
+          else if (nx > tmid) then
+             tl = profile(temp_lw, temp_rw, half, tmid, nx)
+          else if (nx >= tmid - half) then
+             tl = profile(temp_rw, temp_lw, half, nx, tmid)
+          else
+             tl = temp_lw
+          end if
+       end if
+       dens_x(nx) = dl; u_x(nx) = ul; temp_x(nx) = tl
+    end do
+    ok = .true.
+  contains
+    ! initial_shock_profile (initial_conditions.f90:652-671)
+    real(c_double) function profile(lw, rw, th, midpoint, n)
+      real(c_double), intent(in) :: lw, rw
+      integer, intent(in) :: th, midpoint, n
+      real(c_double) :: expo, lx
+      expo = 0.5d0 * (lw - rw) / dble(th * th)
+      lx = dble(midpoint - n + th)
+      profile = rw + expo * lx * lx
+    end function profile
+  end subroutine shock_profile
+
   integer function zero_node(a)
     real(c_double), intent(in) :: a(:)
     integer :: i

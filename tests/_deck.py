@@ -24,12 +24,23 @@ def deck_text(name="cavity10_ar.inp", **changes):
     return "\n".join(lines) + "\n"
 
 
-def run_driver(workdir, deck, *args, check=True):
-    """Writes the deck into workdir and runs the driver there."""
+def run_driver(workdir, deck, *args, check=True, grid=None):
+    """Writes the deck into workdir and runs the driver there (with `grid`,
+    the path of a velocity-grid file, as the second argument)."""
     path = os.path.join(workdir, "run.inp")
     with open(path, "w") as fh:
         fh.write(deck)
-    return subprocess.run([DRIVER, path, *args], cwd=workdir, capture_output=True, text=True, check=check)
+    extra = [grid] if grid else []
+    return subprocess.run([DRIVER, path, *extra, *args], cwd=workdir, capture_output=True, text=True, check=check)
+
+
+def write_grid_file(path, axes):
+    """A velocity-grid file in the layout of read_vel_grid
+    (read_input_2D.f90:779-829): three point counts, then the values."""
+    with open(path, "w") as fh:
+        fh.write("".join(f"{len(a)}\n" for a in axes))
+        for a in axes:
+            fh.writelines(f"{x:.17g}\n" for x in a)
 
 
 def parse_check(stdout):

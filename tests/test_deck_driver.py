@@ -55,7 +55,8 @@ def test_extension_keys(tmp_path):
 
 
 @pytest.mark.parametrize("key, value, text", [("iColl", 1, "iColl"), ("csection_method", 7, "csection_method"),
-                                              ("species_codes", "'H2'", "monatomic"), ("InitC", "'4'", "InitC")])
+                                              ("species_codes", "'H2'", "monatomic"), ("InitDF", "'5'", "InitDF"),
+                                              ("x_vel_grid_type", "'0'", "velocity-grid file")])
 def test_unsupported_decks_stop_with_a_message(tmp_path, key, value, text):
     r = run_driver(str(tmp_path), deck_text(**{key: value}), "--check", check=False)
     assert r.returncode != 0 and text in r.stdout
@@ -189,3 +190,124 @@ def test_species_properties_file_header(tmp_path):
     assert lines == ['title = "Macroscopic Properties"',
                      'variables = "Time", "x-Location", "y-Location"' + "".join(", " + n for n in names)
                      + ', "Moment  3", "Moment  4"']
+
+
+def _f0(pkg, tmp_path, r, grid=None):
+    cfg = solver_config(pkg, parse_check(r.stdout))
+    if grid is not None:
+        cfg = cfg.with_(vx=grid[0], vy=grid[1], vz=grid[2])
+    return cfg, np.fromfile(tmp_path / "f0.bin", dtype=np.float64).reshape(cfg.nx * cfg.ny, cfg.num_points)
+
+
+def _moments(pkg, cfg, f):
+    v = pkg.node_velocities(cfg)
+    n = f.sum()
+    u = f @ v / n
+    return n, u, 2.0 * cfg.mass * (f * ((v - u) ** 2).sum(1)).sum() / (3.0 * n)
+
+
+SHOCK = dict(nx_space="'24'", ny_space="'1'", dvm_dim=1, LWallBC="'2'", RWallBC="'2'", InitC="'4'",
+             spatial_middle="'12'", shock_thickness="'6'", dens_RW="'1.0d0'", x_vel_RW="'-1.6d0'",
+             tr_temp_RW="'1.0d0'", dens_LW="'1.0d0'", x_vel_LW="'0.0d0'", tr_temp_LW="'1.0d0'",
+             xv_min="'-6.0d0'", xv_max="'6.0d0'", num_points_x="'17'", num_points_y="'9'", num_points_z="'9'",
+             yv_min="'-4.0d0'", yv_max="'4.0d0'", zv_min="'-4.0d0'", zv_max="'4.0d0'")
+
+
+def test_initc4_stationary_shock_jump_and_profile(pkg, tmp_path):
+    """InitC = 4 with in/out-flow sides: a stationary shock (shock_conditions.
+    f90:54-171) whose downstream state goes to the left wall. Pinned by the
+    Rankine-Hugoniot ratios (checked against the reference's shock-calculator
+    fixture) at Mach |u_up| / sqrt(gamma T / 2m), gamma = 5/3, and by mass-flux
+    conservation across the shock; the columns between follow
+    initial_shock_profile (initial_conditions.f90:652-671) around
+    spatial_middle, each cell the renormalised Maxwellian of its state."""
+    r = run_driver(str(tmp_path), deck_text(**SHOCK), "--check", "--dump-init", "f0.bin")
+    c = parse_check(r.stdout)
+    mach = 1.6 / pkg.sound_speed(5.0 / 3.0, 1.0, 1.0)
+    dens, vel, temp, _ = pkg.rankine_hugoniot(mach, 5.0 / 3.0)
+    lw = c["wall0"]
+    assert abs(lw[0] - dens) < 1e-13 and abs(lw[4] - temp) < 1e-13
+    assert abs(lw[0] * lw[1] - (-1.6)) < 1e-12          # n u is the same on both sides
+    assert abs(lw[1] - (-1.6) * vel) < 1e-12
+    cfg, f0 = _f0(pkg, tmp_path, r)
+    half, mid = 3, 12
+    tmid = int(np.ceil(mid + 0.5 * half))
+    for ix in range(1, 25):
+        n_, u_, t_ = _moments(pkg, cfg, f0[ix - 1])
+        if ix < mid - half:
+            want_n, want_t = lw[0], lw[4]
+        elif ix > mid + half and ix > tmid + half:
+            want_n, want_t = 1.0, 1.0
+        else:
+            continue
+        assert abs(n_ - want_n) < 1e-12 and abs(t_ - want_t) < 2e-3 * want_t, (ix, n_, t_)
+    # the density profile: LW + (RW - LW)/2 (x - mid + half)^2 / half^2 left of the
+    # middle, RW + (LW - RW)/2 (mid - x + half)^2 / half^2 right of it
+    for ix in range(mid - half, mid + half + 1):
+        x = (ix - mid + half) if ix <= mid else (mid - ix + half)
+        want = (lw[0] + 0.5 * (1.0 - lw[0]) * x * x / half ** 2) if ix <= mid else \
+            (1.0 + 0.5 * (lw[0] - 1.0) * x * x / half ** 2)
+        assert abs(f0[ix - 1].sum() - want) < 1e-12, ix
+
+
+def test_initc4_reflected_shock_leaves_the_gas_at_rest(tmp_path):
+    """With a reflecting wall the shock moves (the reference solves a quadratic
+    for its speed); behind a shock reflected from a wall the gas is at rest."""
+    over = dict(SHOCK, LWallBC="'1'")
+    c = parse_check(run_driver(str(tmp_path), deck_text(**over), "--check").stdout)
+    assert abs(c["wall0"][1]) < 1e-12 and c["wall0"][0] > 1.0 and c["wall0"][4] > 1.0
+
+
+@pytest.mark.parametrize("initdf", [2, 3, 4])
+def test_initdf_shapes(pkg, tmp_path, initdf):
+    """InitDF 2: BKW at t = 0 (dist_func.f90:374-401): every BKW profile has
+    the Maxwellian's temperature; 3: all of the density on the node of
+    x/y/z_delta_loc; 4: the Maxwellian integrated over each node's cell
+    (dist_func.f90:346-372), which approaches the nodal Maxwellian."""
+    over = dict(InitC="'5'", InitDF=f"'{initdf}'", nx_space="'2'", ny_space="'2'", dens_init="'1.3d0'",
+                x_vel_init="'0.25d0'", tr_temp_init="'1.2d0'", num_points_x="'29'", num_points_y="'29'",
+                num_points_z="'29'", xv_min="'-6.0d0'", xv_max="'6.0d0'", yv_min="'-6.0d0'", yv_max="'6.0d0'",
+                zv_min="'-6.0d0'", zv_max="'6.0d0'", x_delta_loc="'0.0d0'", y_delta_loc="'1.28571428571428570d0'",
+                z_delta_loc="'0.0d0'")
+    r = run_driver(str(tmp_path), deck_text(**over), "--check", "--dump-init", "f0.bin")
+    cfg, f0 = _f0(pkg, tmp_path, r)
+    n, u, t = _moments(pkg, cfg, f0[0])
+    assert abs(n - 1.3) < 1e-13
+    if initdf == 2:
+        assert abs(t - 1.2) < 1e-6 and abs(u[0] - 0.25) < 1e-9
+        m = pkg.maxwellian(cfg, 1.3, 0.25, 0.0, 0.0, 1.2)
+        assert np.abs(f0[0] - m).max() > 1e-2 * m.max()  # not the Maxwellian
+    elif initdf == 3:
+        v = pkg.node_velocities(cfg)
+        nz = np.nonzero(f0[0])[0]
+        assert len(nz) == 1 and abs(f0[0][nz[0]] - 1.3) < 1e-15
+        assert np.allclose(v[nz[0]], [0.0, 9.0 / 7.0, 0.0], atol=1e-12)
+    else:
+        m = pkg.maxwellian(cfg, 1.3, 0.25, 0.0, 0.0, 1.2)
+        assert np.abs(f0[0] - m).max() < 0.05 * m.max()
+        assert abs(t - 1.2) < 0.05
+
+
+def test_velocity_grid_file(pkg, tmp_path):
+    """x/y/z_vel_grid_type = 0 takes the axes from the velocity-grid file of the
+    command line (read_vel_grid: the three point counts, then the values)."""
+    gx = np.array([-4.0, -2.5, -1.5, -0.75, 0.0, 0.75, 1.5, 2.5, 4.0])
+    gy = np.array([-3.0, -1.0, 0.0, 1.0, 3.0])
+    gz = np.array([-3.5, -2.0, -0.5, 0.5, 2.0, 3.5, 4.0])
+    with open(tmp_path / "grid.dat", "w") as fh:
+        fh.write(f"{len(gx)}\n{len(gy)}\n{len(gz)}\n")
+        for a in (gx, gy, gz):
+            fh.writelines(f"{x:.17g}\n" for x in a)
+    over = dict(x_vel_grid_type="'0'", y_vel_grid_type="'0'", z_vel_grid_type="'0'", InitC="'5'",
+                x_vel_init="'0.3d0'", nx_space="'2'", ny_space="'1'")
+    path = tmp_path / "run.inp"
+    path.write_text(deck_text(**over))
+    import subprocess
+    from tests._deck import DRIVER
+    r = subprocess.run([DRIVER, str(path), str(tmp_path / "grid.dat"), "--check", "--dump-init", "f0.bin"],
+                       cwd=tmp_path, capture_output=True, text=True, check=True)
+    c = parse_check(r.stdout)
+    assert (c["nvx"], c["nvy"], c["nvz"]) == (9, 5, 7) and c["vx0"] == -4.0 and c["vz1"] == 4.0
+    cfg, f0 = _f0(pkg, tmp_path, r, grid=(gx, gy, gz))
+    want = pkg.maxwellian(cfg, 1.0, 0.3, 0.0, 0.0, 1.0, pi=3.1459265358979323)
+    assert np.allclose(f0[0], want, rtol=1e-13, atol=0)

@@ -5,7 +5,7 @@ reads the same moments the library computes."""
 import numpy as np
 import pytest
 
-from tests._deck import deck_text, have_driver, parse_check, run_driver, solver_config
+from tests._deck import deck_text, have_driver, parse_check, run_driver, solver_config, write_grid_file
 from tests._oracle import oracle_solver
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not have_driver(), reason="dvm_gpu is not built")]
@@ -38,6 +38,52 @@ def test_deck_run_matches_the_oracle(pkg, tmp_path, over):
     scale[1:4] = np.sqrt(np.max(pw[:, 4]))
     assert np.all(np.max(np.abs(props - pw), axis=0) <= 1e-12 * scale)
     assert "steps: 3" in r.stdout
+
+
+def _compare_run(pkg, tmp_path, deck, steps, grid=None, axes=None):
+    run_driver(str(tmp_path), deck, "--steps", str(steps), "--dump-init", "f0.bin", "--dump", "f.bin", grid=grid)
+    cfg = solver_config(pkg, parse_check(run_driver(str(tmp_path), deck, "--check", grid=grid).stdout))
+    if axes is not None:
+        cfg = cfg.with_(vx=axes[0], vy=axes[1], vz=axes[2])
+    ncell, N = cfg.nx * cfg.ny, cfg.num_points
+    f0 = np.fromfile(tmp_path / "f0.bin", dtype=np.float64).reshape(ncell, N)
+    f = np.fromfile(tmp_path / "f.bin", dtype=np.float64).reshape(ncell, N)
+    orc = oracle_solver(cfg)
+    orc.initialize(f0)
+    for n in range(1, steps + 1):
+        orc.step(n)
+    want = orc.download_f()
+    err = np.max(np.abs(f - want), axis=1) / np.max(np.abs(want), axis=1)
+    return cfg, err
+
+
+def test_shock_deck_initc4_matches_the_oracle(pkg, tmp_path):
+    """The deck's own shock set-up (InitC = 4: Rankine-Hugoniot jump, quadratic
+    profile) as a 1-D run with in/out flow, 3 steps."""
+    deck = deck_text(nx_space="'20'", ny_space="'1'", dvm_dim=1, LWallBC="'2'", RWallBC="'2'", InitC="'4'",
+                     spatial_middle="'10'", shock_thickness="'6'", dens_RW="'1.0d0'", x_vel_RW="'-1.5d0'",
+                     tr_temp_RW="'1.0d0'", xv_min="'-6.0d0'", xv_max="'6.0d0'", num_points_x="'15'",
+                     num_points_y="'9'", num_points_z="'9'", dvm_samples=500)
+    print("[deck] InitC 4 stationary shock, 20 cells, 3 steps", flush=True)
+    cfg, err = _compare_run(pkg, tmp_path, deck, 3)
+    assert cfg.dim == 1 and cfg.wall[0][0] > 1.5
+    assert np.all(err <= 1e-12), err.max()
+
+
+def test_velocity_grid_file_run_matches_the_oracle(pkg, tmp_path):
+    """A cavity on non-uniform velocity axes read from the velocity-grid file
+    (the remap's bracket walk and bisection, the boundary stencils and the
+    quadrature weights on uneven spacing), 3 steps."""
+    axes = (np.array([-4.0, -2.8, -1.9, -1.2, -0.6, 0.0, 0.55, 1.15, 1.85, 2.7, 4.0]),
+            np.array([-3.6, -2.0, -1.0, -0.35, 0.35, 1.0, 2.0, 3.6]),
+            np.array([-3.5, -2.25, -1.25, -0.5, 0.0, 0.5, 1.25, 2.25, 3.5]))
+    write_grid_file(tmp_path / "grid.dat", axes)
+    deck = deck_text(x_vel_grid_type="'0'", y_vel_grid_type="'0'", z_vel_grid_type="'0'", nx_space="'5'",
+                     ny_space="'4'", dvm_samples=600)
+    print("[deck] cavity 5x4 on a non-uniform 11x8x9 grid file, 3 steps", flush=True)
+    cfg, err = _compare_run(pkg, tmp_path, deck, 3, grid=str(tmp_path / "grid.dat"), axes=axes)
+    assert cfg.nvx == 11 and cfg.nvy == 8 and cfg.nvz == 9
+    assert np.all(err <= 1e-12), err.max()
 
 
 def test_properties_file_matches_the_oracle(pkg, tmp_path):
