@@ -1,6 +1,6 @@
 """Batched solves over the dvm_batch_* entry points of include/dvm_hip.h.
 
-A `Batch` keeps many independent space-homogeneous and 1-D solves resident on
+A `Batch` keeps many independent space-homogeneous, 1-D and 2-D solves resident on
 the device and advances all of them with one replayed HIP graph per step.
 Every instance keeps the reference's per-solve semantics (the loop body of
 solve(), software/trunk/src/solver/dvm_solver.f90:299-357): its trajectory is
@@ -28,7 +28,7 @@ class Batch:
                  admitted instance must share (precision, sample mode, ...)
     max_slots    instances resident at once
     node_capacity, cell_capacity
-                 pool sizes: sum of nx * nvx*nvy*nvz and of nx over residents
+                 pool sizes: sum of nx*ny * nvx*nvy*nvz and of nx*ny over residents
     """
 
     def __init__(self, species: SolverConfig, max_slots, node_capacity, cell_capacity, lib=None):
@@ -79,10 +79,10 @@ class Batch:
         return n.value, b.value
 
     def admit(self, cfg: SolverConfig, f0):
-        """Admit one instance with initial f0 [nx, N]; returns its slot.
+        """Admit one instance with initial f0 [nx*ny, N]; returns its slot.
         Raises DvmError with code DVM_E_NOMEM (-3) when it does not fit now."""
         c, keep = cfg.to_c()
-        ncell, N = cfg.nx, cfg.num_points
+        ncell, N = cfg.nx * cfg.ny, cfg.num_points
         f0 = np.ascontiguousarray(f0, dtype=np.float64).reshape(ncell, N)
         slot = C.c_int32(-1)
         self._check(self.lib.dvm_batch_admit(self.h, C.byref(c), _dptr(f0), C.byref(slot)))

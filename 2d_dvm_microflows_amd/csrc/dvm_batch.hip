@@ -1,6 +1,6 @@
-// dvm_batch.hip -- many independent solves (space-homogeneous and 1-D) with
-// uneven velocity grids and cell counts, resident together and advanced by
-// one replayed HIP graph per step. Included at the end of dvm_capi.hip (one
+// dvm_batch.hip -- many independent solves (space-homogeneous, 1-D and 2-D)
+// with uneven velocity grids and cell counts, resident together and advanced
+// by one replayed HIP graph per step. Included at the end of dvm_capi.hip (one
 // translation unit).
 //
 // Each solve ("slot") owns a velocity grid, a time step, a sample count, a
@@ -25,6 +25,10 @@
 //   k_conserve_cells   FP32 path: FP64 conservation correction of the cells
 //                      the collision kernel could not correct itself
 //   k_convect1d_cells  1-D slots: the zone-grid sweep with in/out flow
+//   per 2-D slot       the wall flux ratios, specular ghosts and 2-D sweep of
+//                      the slot's grid (k_wall_ratio, k_spec_ghost,
+//                      k_convect2d), from the post-collision pool back into
+//                      the pool the step started from
 //   k_tick_slots       step counter += 1; ping-pong index flips unless the
 //                      sweep brought f back into the original pool
 //   k_moments_cells    moments of the new f
@@ -311,7 +315,19 @@ struct HostSlot {
   long long nodes = 0, cdf = 0;  // pool lengths held
   long long samples = 0;
   int steps = 0;                 // steps launched since admission
+  // 2-D slots that convect: the sweep's constants and the slot's own wall
+  // buffers (wall Maxwellians [4][N], flux ratios [4][max(nx,ny)], specular
+  // ghosts [4][max(nx,ny)][N] when acc < 1)
+  bool conv2d = false;
+  ConvParams cp{};
+  double *wallM = nullptr, *ratio = nullptr, *spec = nullptr;
 };
+
+static void free_slot_walls(HostSlot& h) {
+  for (double* p : {h.wallM, h.ratio, h.spec})
+    if (p) (void)hipFree(p);
+  h.wallM = h.ratio = h.spec = nullptr;
+}
 
 }  // namespace
 
@@ -402,6 +418,7 @@ extern "C" void dvm_batch_destroy(dvm_batch* b) {
                   b->d_items, b->d_l1};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (HostSlot& h : b->hs) free_slot_walls(h);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -409,7 +426,7 @@ extern "C" void dvm_batch_destroy(dvm_batch* b) {
 // Node-pool, cell-pool and CDF-pool lengths an instance occupies.
 static void instance_extent(const dvm_config* c, long long& nodes, int& ncell, long long& cdf) {
   const long long N = (long long)c->nvx * c->nvy * c->nvz;
-  ncell = c->nx;
+  ncell = c->nx * c->ny;
   nodes = N * ncell;
   cdf = 2 * (N + 1) * ncell;
 }
@@ -494,15 +511,22 @@ extern "C" int dvm_batch_footprint(const dvm_config* inst, int32_t precision, in
   int nc;
   instance_extent(inst, nn, nc, cdf);
   if (nodes) *nodes = nn;
-  if (bytes)
+  if (bytes) {
     *bytes = nn * bytes_per_node(precision == DVM_PREC_FP32) + (long long)nc * (16 * kNBucket + 8 * 20 + 16);
+    if (inst->dim == DVM_DIM_2D && inst->enable_convection) {
+      // the slot's wall buffers
+      const long long N = (long long)inst->nvx * inst->nvy * inst->nvz;
+      const long long along = inst->nx > inst->ny ? inst->nx : inst->ny;
+      *bytes += 8 * (4 * N + 4 * along + (inst->acc_coeff < 1.0 ? 4 * along * N : 0));
+    }
+  }
   return DVM_OK;
 }
 
 static int check_instance(dvm_batch* b, const dvm_config* inst) {
   if (int rc = validate(inst)) return bfail(b, rc, "%s", g_create_msg);
-  if (inst->dim != DVM_DIM_0D && inst->dim != DVM_DIM_1D)
-    return bfail(b, DVM_E_ARG, "batched solves are space-homogeneous or 1-D");
+  if (inst->dim != DVM_DIM_0D && inst->dim != DVM_DIM_1D && inst->dim != DVM_DIM_2D)
+    return bfail(b, DVM_E_ARG, "batched solves are space-homogeneous, 1-D or 2-D");
   const dvm_config& s = b->species;
   if (inst->mass != s.mass || inst->diam != s.diam || inst->omega != s.omega || inst->alpha != s.alpha ||
       inst->csection_coeff != s.csection_coeff || inst->viscosity != s.viscosity ||
@@ -563,13 +587,36 @@ extern "C" int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const doubl
   d.ncell = nc;
   d.full = window_dims(hg.g.n, d.wd) ? 1 : 0;
   d.split = 1;
-  d.convect = (inst->dim == DVM_DIM_1D && inst->enable_convection) ? 1 : 0;
+  // a convecting slot's sweep brings f back into the pool the step started
+  // from, so its ping-pong index stays 0 (pool 0 holds f, pool 1 the
+  // post-collision f) for its whole residence
+  d.convect = (inst->dim != DVM_DIM_0D && inst->enable_convection) ? 1 : 0;
   d.fx = inst->dt / (inst->dx > 0.0 ? inst->dx : 1.0);
   d.bc[0] = inst->bc_left;
   d.bc[1] = inst->bc_right;
   d.order = inst->conv_order == 2 ? 2 : 1;
   const int N = hg.g.N;
-  if (d.convect) {
+  HostSlot fresh{};
+  if (inst->dim == DVM_DIM_2D && d.convect) {
+    fresh.conv2d = true;
+    conv_params(inst, hg, fresh.cp);
+    std::vector<double> wm;
+    conv2d_walls(inst, hg, fresh.cp, wm);
+    const size_t along = (size_t)fresh.cp.max_along;
+    int rc = balloc(b, &fresh.wallM, (size_t)4 * N);
+    if (!rc) rc = balloc(b, &fresh.ratio, 4 * along);
+    if (!rc && inst->acc_coeff < 1.0) rc = balloc(b, &fresh.spec, 4 * along * N);
+    if (rc) {
+      free_slot_walls(fresh);
+      b->nodes.release(off, nn);
+      b->cells.release(c0, nc);
+      b->cdfs.release(coff, ncdf);
+      return rc;
+    }
+    BHIP(b, hipMemcpy(fresh.wallM, wm.data(), wm.size() * sizeof(double), hipMemcpyHostToDevice));
+    BHIP(b, hipMemset(fresh.ratio, 0, 4 * along * sizeof(double)));
+  }
+  if (d.convect && inst->dim == DVM_DIM_1D) {
     std::vector<double> inf;
     inflow_maxwellians(inst, hg, inf);  // 2N <= nx*N: the slot's own range of the inflow pool
     BHIP(b, hipMemcpyAsync(b->P.inflow + off, inf.data(), inf.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -591,6 +638,7 @@ extern "C" int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const doubl
   BHIP(b, hipGetLastError());
   BHIP(b, hipStreamSynchronize(b->stream));  // host vectors above are freed on return
   HostSlot& h = b->hs[slot];
+  h = fresh;
   h.active = true;
   h.N = N;
   h.ncell = nc;
@@ -614,6 +662,7 @@ extern "C" int dvm_batch_retire(dvm_batch* b, int32_t slot) {
   b->nodes.release(d.off, h.nodes);
   b->cells.release(d.cell0, h.ncell);
   b->cdfs.release(d.coff, h.cdf);
+  free_slot_walls(h);
   h = HostSlot{};
   b->dirty = true;
   return DVM_OK;
@@ -668,7 +717,7 @@ static int batch_lists(dvm_batch* b) {
     if (!d.full || d.split > 1) b->any_global = true;
     for (int c = 0; c < h.ncell; ++c) {
       cl.push_back(CellRef{s, c});
-      if (d.convect) cl1.push_back(CellRef{s, c});
+      if (d.convect && h.dim == DVM_DIM_1D) cl1.push_back(CellRef{s, c});
       cw.push_back(CellWork{est, (int)items.size(), d.split});
       for (int q = 0; q < d.split; ++q) {
         const long long q0 = (long long)q * step;
@@ -676,7 +725,7 @@ static int batch_lists(dvm_batch* b) {
         items.push_back(WorkItem{s, c, (int)q0, last ? 0x7fffffff : (int)(q0 + step)});
       }
     }
-    if (d.convect) b->max_n1d = h.N > b->max_n1d ? h.N : b->max_n1d;
+    if (d.convect && h.dim == DVM_DIM_1D) b->max_n1d = h.N > b->max_n1d ? h.N : b->max_n1d;
   }
   // cells to the XCD queues, costliest first, each to the least loaded queue
   // (longest-processing-time balancing); a queue lists its cells' items in
@@ -772,6 +821,24 @@ static int batch_enqueue_step(dvm_batch* b) {
     const unsigned ys = (unsigned)(b->n_cells1d < 65535 ? b->n_cells1d : 65535);
     hipLaunchKernelGGL(k_convect1d_cells, dim3(chunks, ys), dim3(kConvBS), 0, b->stream, b->d_slots, b->d_state,
                        b->d_cells1d, b->n_cells1d, b->P);
+  }
+  // 2-D slots: walls and sweep of each slot's own grid, from the
+  // post-collision pool 1 into pool 0 (a convecting slot's index stays 0)
+  for (int s = 0; s < b->max_slots; ++s) {
+    const HostSlot& h = b->hs[s];
+    if (!h.active || !h.conv2d) continue;
+    const SlotDesc& d = b->slots[s];
+    const ConvParams& p = h.cp;
+    const double* fc = b->P.f[1] + d.off;
+    double* fo = b->P.f[0] + d.off;
+    if (p.acc > 0.0)
+      hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, b->stream, d.g, p, fc, h.ratio);
+    if (p.acc < 1.0) {
+      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, b->stream, h.spec, (long long)4 * p.max_along * h.N, 0.0);
+      hipLaunchKernelGGL(k_spec_ghost, dim3((h.N + 255) / 256, p.max_along, 4), dim3(256), 0, b->stream, d.g, p, fc,
+                         h.spec, b->P.err);
+    }
+    launch_convect2d(d.g, p, fc, fo, h.wallM, h.ratio, h.spec, b->stream);
   }
   hipLaunchKernelGGL(k_tick_slots, dim3(1), dim3(256), 0, b->stream, b->d_slots, b->d_state, b->d_active, b->n_active);
   hipLaunchKernelGGL(k_moments_cells, dim3(b->n_cells), dim3(kMomBS), 0, b->stream, b->d_slots, b->d_state,

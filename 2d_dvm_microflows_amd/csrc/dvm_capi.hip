@@ -388,6 +388,63 @@ static void inflow_maxwellians(const dvm_config* cfg, const HostGrid& h, std::ve
   }
 }
 
+// Convection constants of a run (the sweep's CFL factors, boundary types and
+// the zero-plane flags of boundary_conditions.f90:662-673, kept per grid: the
+// reference's SAVE makes them sticky across grids).
+static void conv_params(const dvm_config* cfg, const HostGrid& hg, ConvParams& p) {
+  const int* n = hg.g.n;
+  p.nx = cfg->nx;
+  p.ny = cfg->ny;
+  p.max_along = cfg->nx > cfg->ny ? cfg->nx : cfg->ny;
+  p.fx = cfg->dt / cfg->dx;
+  p.fy = (cfg->dim == DVM_DIM_2D) ? cfg->dt / cfg->dy : 0.0;
+  p.acc = cfg->acc_coeff;
+  p.bc[0] = cfg->bc_left;
+  p.bc[1] = cfg->bc_right;
+  p.order = cfg->conv_order == 2 ? 2 : 1;
+  for (int s = 0; s < 4; ++s) { p.wall_side[s] = 0; p.couette_side[s] = 0; p.flux_out[s] = 1.0; }
+  const int izero = hg.g.zero[0], jzero = hg.g.zero[1];
+  p.izflag = (izero != 0 && izero != hg.g.neg[0] && izero != hg.g.pos[0] && izero != n[0] - 1);
+  p.jzflag = (jzero != 0 && jzero != hg.g.neg[1] && jzero != hg.g.pos[1] && jzero != n[1] - 1);
+}
+
+// 2-D walls: which sides are diffuse / specular walls or Couette sides, the
+// wall Maxwellian * beta^3 on each wall's outgoing half-space (wm [4][N]) and
+// its flux (boundary_conditions.f90:785-828 and analogues), summed in the
+// reference's loop order.
+static void conv2d_walls(const dvm_config* cfg, const HostGrid& hg, ConvParams& p, std::vector<double>& wm) {
+  const int* n = hg.g.n;
+  const int N = hg.g.N;
+  const double m = cfg->mass;
+  p.wall_side[0] = cfg->bc_left == DVM_BC_WALL;
+  p.wall_side[1] = cfg->bc_right == DVM_BC_WALL;
+  p.wall_side[2] = 1;  // bottom and top are walls unconditionally in trunk 2-D
+  p.wall_side[3] = 1;
+  p.couette_side[0] = cfg->bc_left == DVM_BC_COUETTE;
+  p.couette_side[1] = cfg->bc_right == DVM_BC_COUETTE;
+  wm.assign((size_t)4 * N, 0.0);
+  for (int s = 0; s < 4; ++s) {
+    if (!p.wall_side[s]) continue;
+    const double* wl = cfg->wall[s];
+    const double kt = wl[DVM_W_TEMP], u = wl[DVM_W_U], v = wl[DVM_W_V], w = wl[DVM_W_W];
+    const double mc = sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * kt * kt * kt));
+    double flux_out = 0.0;
+    double* gm = wm.data() + (size_t)s * N;
+    auto put = [&](int i, int j, int k) {
+      const double beta3 = hg.beta[0][i] * hg.beta[1][j] * hg.beta[2][k];
+      const double mv = maxwell_host(hg.ax[0][i], hg.ax[1][j], hg.ax[2][k], mc, m, u, v, w, kt);
+      gm[i + n[0] * (j + n[1] * k)] = mv * beta3;
+      const double vn = (s < 2) ? hg.ax[0][i] : hg.ax[1][j];
+      flux_out = flux_out + fabs(vn) * mv * beta3;
+    };
+    if (s == 0) { for (int k = 0; k < n[2]; ++k) for (int j = 0; j < n[1]; ++j) for (int i = hg.g.pos[0]; i < n[0]; ++i) put(i, j, k); }
+    if (s == 1) { for (int k = 0; k < n[2]; ++k) for (int j = 0; j < n[1]; ++j) for (int i = 0; i <= hg.g.neg[0]; ++i) put(i, j, k); }
+    if (s == 2) { for (int k = 0; k < n[2]; ++k) for (int i = 0; i < n[0]; ++i) for (int j = hg.g.pos[1]; j < n[1]; ++j) put(i, j, k); }
+    if (s == 3) { for (int k = 0; k < n[2]; ++k) for (int i = 0; i < n[0]; ++i) for (int j = 0; j <= hg.g.neg[1]; ++j) put(i, j, k); }
+    p.flux_out[s] = flux_out;
+  }
+}
+
 static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   c->cfg = *cfg;
   c->ncell = cfg->nx * cfg->ny;
@@ -492,51 +549,10 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   }
   // ---- convection constants
   ConvParams& p = c->convp;
-  p.nx = cfg->nx;
-  p.ny = cfg->ny;
-  p.max_along = cfg->nx > cfg->ny ? cfg->nx : cfg->ny;
-  p.fx = cfg->dt / cfg->dx;
-  p.fy = (cfg->dim == DVM_DIM_2D) ? cfg->dt / cfg->dy : 0.0;
-  p.acc = cfg->acc_coeff;
-  p.bc[0] = cfg->bc_left;
-  p.bc[1] = cfg->bc_right;
-  p.order = cfg->conv_order == 2 ? 2 : 1;
-  for (int s = 0; s < 4; ++s) { p.wall_side[s] = 0; p.couette_side[s] = 0; p.flux_out[s] = 1.0; }
-  // boundary_conditions.f90:662-673 (flag kept per grid; the reference's SAVE makes it sticky across grids)
-  const int izero = c->g.zero[0], jzero = c->g.zero[1];
-  p.izflag = (izero != 0 && izero != c->g.neg[0] && izero != c->g.pos[0] && izero != n[0] - 1);
-  p.jzflag = (jzero != 0 && jzero != c->g.neg[1] && jzero != c->g.pos[1] && jzero != n[1] - 1);
+  conv_params(cfg, hg, p);
   if (cfg->dim == DVM_DIM_2D && cfg->enable_convection) {
-    p.wall_side[0] = cfg->bc_left == DVM_BC_WALL;
-    p.wall_side[1] = cfg->bc_right == DVM_BC_WALL;
-    p.wall_side[2] = 1;  // bottom and top are walls unconditionally in trunk 2-D
-    p.wall_side[3] = 1;
-    p.couette_side[0] = cfg->bc_left == DVM_BC_COUETTE;
-    p.couette_side[1] = cfg->bc_right == DVM_BC_COUETTE;
-    // wall Maxwellian * beta^3 on each wall's outgoing half-space and its flux
-    // (boundary_conditions.f90:785-828 and analogues), summed in the
-    // reference's loop order
-    std::vector<double> wm((size_t)4 * N, 0.0);
-    for (int s = 0; s < 4; ++s) {
-      if (!p.wall_side[s]) continue;
-      const double* wl = cfg->wall[s];
-      const double kt = wl[DVM_W_TEMP], u = wl[DVM_W_U], v = wl[DVM_W_V], w = wl[DVM_W_W];
-      const double mc = sqrt(m * m * m / (kPiRef * kPiRef * kPiRef * kt * kt * kt));
-      double flux_out = 0.0;
-      double* gm = wm.data() + (size_t)s * N;
-      auto put = [&](int i, int j, int k) {
-        const double beta3 = c->beta[0][i] * c->beta[1][j] * c->beta[2][k];
-        const double mv = maxwell_host(c->ax[0][i], c->ax[1][j], c->ax[2][k], mc, m, u, v, w, kt);
-        gm[i + n[0] * (j + n[1] * k)] = mv * beta3;
-        const double vn = (s < 2) ? c->ax[0][i] : c->ax[1][j];
-        flux_out = flux_out + fabs(vn) * mv * beta3;
-      };
-      if (s == 0) { for (int k = 0; k < n[2]; ++k) for (int j = 0; j < n[1]; ++j) for (int i = c->g.pos[0]; i < n[0]; ++i) put(i, j, k); }
-      if (s == 1) { for (int k = 0; k < n[2]; ++k) for (int j = 0; j < n[1]; ++j) for (int i = 0; i <= c->g.neg[0]; ++i) put(i, j, k); }
-      if (s == 2) { for (int k = 0; k < n[2]; ++k) for (int i = 0; i < n[0]; ++i) for (int j = c->g.pos[1]; j < n[1]; ++j) put(i, j, k); }
-      if (s == 3) { for (int k = 0; k < n[2]; ++k) for (int i = 0; i < n[0]; ++i) for (int j = 0; j <= c->g.neg[1]; ++j) put(i, j, k); }
-      p.flux_out[s] = flux_out;
-    }
+    std::vector<double> wm;
+    conv2d_walls(cfg, hg, p, wm);
     if ((rc = dalloc(c, &c->wallM, (size_t)4 * N)) || (rc = dalloc(c, &c->ratio, (size_t)4 * p.max_along))) return rc;
     HIP_TRY(c, hipMemcpy(c->wallM, wm.data(), wm.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemset(c->ratio, 0, (size_t)4 * p.max_along * sizeof(double)));

@@ -145,7 +145,8 @@ class Server:
         return ("steady" if st["ok"] >= 2 else ""), metric
 
     def _fits_at_all(self, req):
-        return req.cfg.nx * req.cfg.num_points <= self.node_capacity and req.cfg.nx <= self.cell_capacity
+        nc = req.cfg.nx * req.cfg.ny
+        return nc * req.cfg.num_points <= self.node_capacity and nc <= self.cell_capacity
 
     def run(self, requests):
         reqs = sorted(requests, key=lambda r: r.arrival)

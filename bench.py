@@ -311,6 +311,28 @@ def run_batched(pkg, count, steps, precision, tune=True):
             "l1_median_after": float(np.median(l1[l1 >= 0]))}
 
 
+def run_batched_cavity(pkg, cfg, f, steps, warmup, precision):
+    """The cavity as one slot of a batch: the same cells, collisions and sweep,
+    but every step is one replay of the batch's captured HIP graph (shared
+    persistent collision kernel fed from per-XCD work queues, per-slot 2-D
+    walls and sweep)."""
+    c = cfg.with_(precision=precision)
+    b = pkg.Batch(c, max_slots=2, node_capacity=c.nx * c.ny * c.num_points, cell_capacity=c.nx * c.ny)
+    try:
+        b.admit(c, f)
+        b.step(warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        wall = time.perf_counter() - t0
+        tiles = b.tiles()
+    finally:
+        b.close()
+    return {"ms_per_step": wall / steps * 1e3, "value": c.nx * c.ny * c.num_points * steps / wall,
+            "precision": "fp32" if precision == pkg.capi.DVM_PREC_FP32 else "fp64", "tiles": tiles}
+
+
 def run_shock(pkg, steps, warmup):
     """The 1-D normal shock: 200 cells x 31^3 nodes on +-6, VHS omega 0.81,
     Mach 2 with the Rankine-Hugoniot downstream state and a tanh profile,
@@ -555,6 +577,17 @@ def main():
             output_moment_error(finals["fp32"]["ext"], finals["fp64"]["ext"], finals["fp32"]["f"], finals["fp64"]["f"]))
         out["fp32_moment_error_after_steps"] = args.warmup + args.steps
     finals.clear()
+    if not args.no_batched:
+        # the batched form of the cavity (one graph replay per step), both
+        # precisions, against the plain FP64 context path above
+        bc = {}
+        for prec in ("fp32", "fp64"):
+            code = pkg.capi.DVM_PREC_FP32 if prec == "fp32" else pkg.capi.DVM_PREC_FP64
+            bc[prec] = run_batched_cavity(pkg, cfg, f, args.steps, args.warmup, code)
+            print(f"[bench] batched cavity {prec}: {bc[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        if "fp64" in runs:
+            bc["speedup_batched_fp32_vs_plain_fp64"] = (runs["fp64"][0] / args.steps * 1e3) / bc["fp32"]["ms_per_step"]
+        out["batched_cavity"] = bc
     if not args.no_fmc:
         # the same cavity with the full Monte Carlo kernel (iColl = 2): one
         # integral of samples drawn from the |f| CDF

@@ -279,7 +279,7 @@ int dvm_timing_reset(dvm_ctx* ctx);
 int dvm_timing_read(dvm_ctx* ctx, double* phase_ms /* [DVM_NPHASE] */, int64_t* calls);
 
 /* ------------------------------------------------------------------------
-   Batched solves. Many independent space-homogeneous (dim 0) and 1-D solves,
+   Batched solves. Many independent space-homogeneous (dim 0), 1-D and 2-D solves,
    each with its own velocity grid, cell count, dt, Kn, sample count, seed and
    step counter, resident in shared device pools and advanced together by one
    replayed HIP graph per step. Each instance behaves as the loop body of
@@ -306,7 +306,9 @@ const char* dvm_batch_last_error(const dvm_batch* b);
 /* Pool nodes and HBM bytes an instance would occupy (admission by footprint). */
 int dvm_batch_footprint(const dvm_config* inst, int32_t precision, int64_t* nodes, int64_t* bytes);
 
-/* Admits one instance (dim 0 or 1; its species fields must equal the batch's)
+/* Admits one instance (dim 0, 1 or 2; its species fields must equal the batch's;
+   a 2-D instance brings its own walls, swept by per-slot launches inside the
+   step's graph)
    with initial f0[nx][nnode], computes its moments and returns its slot.
    DVM_E_NOMEM with a message when no slot or pool range is free: the caller
    waits for a retirement and tries again.

@@ -167,3 +167,40 @@ def test_full_monte_carlo_instances_match_standalone(pkg):
             got = b.download_f(s)
             want = standalone(pkg, c, f, 3)
             assert cellwise(got, want) <= RTOL, (c.name, cellwise(got, want))
+
+
+def test_2d_slots_match_standalone_in_the_mix(pkg):
+    """2-D cavities as batch slots (their walls and 2-D sweep launched per slot
+    inside the step's graph) next to a homogeneous instance and a 1-D shock:
+    diffuse walls with a moving lid, partially specular walls, Couette sides;
+    every slot equals a standalone context to 1e-12 after three steps,
+    admitted and retired around each other."""
+    rng = np.random.default_rng(9)
+    cav, fc = pkg.lid_driven_cavity(nx=6, ny=5, n_v=9, vmax=4.0, samples=800, seed=301)
+    spec, fs = pkg.lid_driven_cavity(nx=4, ny=7, n_v=11, vmax=4.0, samples=600, seed=302)
+    spec = spec.with_(acc_coeff=0.6)
+    cou, fo = pkg.lid_driven_cavity(nx=5, ny=4, n_v=9, vmax=4.0, samples=500, seed=303)
+    cou = cou.with_(bc_left=pkg.capi.DVM_BC_COUETTE, bc_right=pkg.capi.DVM_BC_COUETTE)
+    hom = pkg.mixture_instance(rng, 11, samples_per_node=4, seed=304)
+    shk = pkg.normal_shock_1d(nx=8, n_v=9, vmax=6.0, samples=600, omega=0.5, seed=305)
+    hom = (hom[0].with_(omega=0.5), hom[1])
+    insts = [(cav, fc), (spec, fs), (cou, fo), hom, shk]
+    print("[batch] 2-D cavities (diffuse + lid, partially specular, Couette) with a homogeneous and a 1-D slot",
+          flush=True)
+    b = pkg.Batch(cav, max_slots=8, node_capacity=200_000, cell_capacity=128)
+    try:
+        slots = [b.admit(c, f) for c, f in insts]
+        b.step(2)
+        b.retire(slots[1])
+        again = b.admit(spec, fs)
+        b.step(1)
+        for (c, f), s in zip(insts, slots):
+            if s == slots[1]:
+                continue
+            err = cellwise(b.download_f(s), standalone(pkg, c, f, 3))
+            print(f"  {c.name}: {err:.2e}", flush=True)
+            assert err <= RTOL, (c.name, err)
+        err = cellwise(b.download_f(again), standalone(pkg, spec, fs, 1))
+        assert err <= RTOL, err
+    finally:
+        b.close()

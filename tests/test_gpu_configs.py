@@ -294,6 +294,33 @@ def test_batch_4096_instances_fp32_match_standalone(pkg):
     _batch_check(pkg, FP32, picks=(3, 1500, 4000))
 
 
+@pytest.mark.parametrize("precision", [FP32, FP64])
+def test_cavity_64x64_as_a_batch_slot_matches_standalone(pkg, precision):
+    """The 64x64 cavity admitted into a batch (its cells in the shared CDF,
+    collision and moment kernels, its walls and 2-D sweep launched for the
+    slot inside the replayed graph) equals the standalone context after two
+    steps (1e-12: FP32 runs receive identical FP32 inputs in the first step;
+    the atomics order can move the last bit of the second step's inputs)."""
+    cfg, f = pkg.lid_driven_cavity(precision=precision)
+    print(f"[config] {cfg.name} as a batch slot, precision {precision}, 2 steps", flush=True)
+    b = pkg.Batch(cfg, max_slots=2, node_capacity=cfg.nx * cfg.ny * cfg.num_points, cell_capacity=cfg.nx * cfg.ny)
+    try:
+        s = b.admit(cfg, f)
+        b.step(2)
+        b.synchronize()
+        got = b.download_f(s)
+    finally:
+        b.close()
+    with pkg.Solver(cfg) as ref:
+        ref.initialize(f)
+        ref.step(1)
+        ref.step(2)
+        want = ref.download_f()
+    err = cellwise_all(got, want)
+    print(f"  cells within 1e-12 {np.mean(err <= 1e-12):.4f}, max {err.max():.2e}", flush=True)
+    assert np.mean(err <= 1e-12) >= 0.99 and err.max() <= 1e-5
+
+
 class TickClock:
     def __init__(self, tick):
         self.t = 0.0
