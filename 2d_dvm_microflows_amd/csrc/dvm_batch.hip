@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
     const SlotState st = state[w.slot];
     const size_t gc = (size_t)d.cell0 + w.cell;
     const size_t n0 = cell_node0(d, w.cell);
-    stage_axes(g, dyn);
+    using R = typename std::conditional<kF32, float, double>::type;
+    const Axes<R> ax = stage_collision_axes<R>(g, dyn);
     CollParams cp = base;
     cp.seed = d.seed;
     const int ntime = st.ntime + 1;
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
                                   P.tabs + gc * 2 * kNBucket, N);
     Window win{};
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
-    const int bad = collide_item<kF32, BS, K>(g, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
+    const int bad = collide_item<kF32, BS, K>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
                                               P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed});
     if (bad) atomicOr(P.err, bad);

@@ -51,8 +51,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
                                               uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    // each 32 x 32 -> 64-bit product in one v_mad_u64_u32
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0;
     const uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -102,6 +104,39 @@ __device__ __forceinline__ int referenced_search(const double* __restrict__ cumu
 }
 
 // ---------------------------------------------------------------- remap
+// The velocity axes in the arithmetic R of a remap: FP64, whose stencils and
+// weights equal the reference's bit for bit, or FP32 (the FP32-collision
+// path, whose replenishment weights are FP32 like the rest of its collision
+// arithmetic).
+template <class R>
+struct Axes {
+  const R* a[3];
+  int n[3];
+  R vmin[3], vmax[3], factor[3];
+};
+
+__device__ __forceinline__ Axes<double> grid_axes(const Grid& g) {
+  Axes<double> x;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    x.a[d] = g.ax[d];
+    x.n[d] = g.n[d];
+    x.vmin[d] = g.vmin[d];
+    x.vmax[d] = g.vmax[d];
+    x.factor[d] = g.factor[d];
+  }
+  return x;
+}
+
+// a / b: correctly rounded in FP64; in FP32 a times the hardware reciprocal
+// (v_rcp_f32, 1 ulp), two instructions instead of the scaled Newton sequence.
+template <class R>
+__device__ __forceinline__ R quot(R a, R b) {
+  if constexpr (sizeof(R) == 4) return a * __builtin_amdgcn_rcpf(b); else return a / b;
+}
+template <class R>
+__device__ __forceinline__ int isgn_r(R x) { return signbit(x) ? -1 : 1; }
+
 // binary_search_with_first_guess (sort_and_search.f90:130-183): nearest axis
 // node to val, ties to the lower index, clamped to the axis ends. The
 // reference's bisection can only stop at the unique index with
@@ -110,19 +145,19 @@ __device__ __forceinline__ int referenced_search(const double* __restrict__ cumu
 // node of that bracket, so the bracket is settled by one comparison at the
 // guess; a walk and then the bisection run only when it is not (non-uniform
 // axes).
-__device__ __forceinline__ int nearest_node(const double* __restrict__ a, int mx, double val, int index, double lo,
-                                            double hi) {
+template <class R>
+__device__ __forceinline__ int nearest_node(const R* __restrict__ a, int mx, R val, int index, R lo, R hi) {
   // lo = a[0], hi = a[mx], held in registers
   if (lo > val) return 0;
   if (hi < val) return mx;
   index = index < 0 ? 0 : (index > mx ? mx : index);
-  const double ag = a[index];
-  if (fabs(val - ag) < kDoubleTol) return index;
+  const R ag = a[index];
+  if (fabs(val - ag) < (R)kDoubleTol) return index;
   // the bracket is [index, index+1] or [index-1, index] when the guess is
   // within one node (uniform axes): one compare and two independent reads
   int i = (val > ag) ? index : index - 1;
   i = i < 0 ? 0 : (i > mx - 1 ? mx - 1 : i);
-  double a0 = a[i], a1 = a[i + 1];
+  R a0 = a[i], a1 = a[i + 1];
   if (!(val > a0 && val <= a1)) {
     // farther guesses (non-uniform axes): walk, then bisect
     index = i;
@@ -145,67 +180,77 @@ __device__ __forceinline__ int nearest_node(const double* __restrict__ a, int mx
   return (fabs(val - a0) > fabs(val - a1)) ? i + 1 : i;
 }
 
-__device__ __forceinline__ int first_guess(double factor, double val, double vmin, int mx) {
-  double g = round(factor * (val - vmin));  // Fortran nint: halves away from zero
-  g = g < -1.0 ? -1.0 : (g > (double)(mx + 1) ? (double)(mx + 1) : g);
+// First guess of nearest_node, clamped to [-1, mx + 1]. The reference rounds
+// with nint (halves away from zero); this rounds halves to even, one
+// instruction. The two differ only when val lies exactly midway between two
+// nodes of a uniform axis, where either guess is one of the two bracketing
+// nodes and, val being a half-spacing from both, neither takes the guess's
+// early exit: the bracket and so the result are the same.
+template <class R>
+__device__ __forceinline__ int first_guess(R factor, R val, R vmin, int mx) {
+  R g = rint(factor * (val - vmin));
+  g = g < (R)-1 ? (R)-1 : (g > (R)(mx + 1) ? (R)(mx + 1) : g);
   return (int)g;
 }
 
 // find_boundary_stencil (remapping.f90:396-438)
-__device__ __forceinline__ void boundary_stencil(int& sa, int& sb, int& sc, double abc, int oi, int oj, int ok,
-                                                 const double* X, const double* Y, const double* Z,
-                                                 int imax, int jmax, int kmax) {
+template <class R>
+__device__ __forceinline__ void boundary_stencil(int& sa, int& sb, int& sc, R abc, int oi, int oj, int ok,
+                                                 const R* X, const R* Y, const R* Z, int imax, int jmax, int kmax) {
   while (oi + sa > 0 && oi + sa < imax) {
-    if (fabs(abc) > 0.5 * fabs(X[oi + sa] - X[oi])) sa = sa + isign1(sa); else break;
+    if (fabs(abc) > (R)0.5 * fabs(X[oi + sa] - X[oi])) sa = sa + isign1(sa); else break;
   }
   while (oj + sb > 0 && oj + sb < jmax && oj - sb > 0 && oj - sb < jmax) {
-    if (fabs(abc) > 0.5 * fabs(Y[oj + sb] - Y[oj])) sb = sb + isign1(sb); else break;
+    if (fabs(abc) > (R)0.5 * fabs(Y[oj + sb] - Y[oj])) sb = sb + isign1(sb); else break;
   }
   while (ok + sc > 0 && ok + sc < kmax && ok - sc > 0 && ok - sc < kmax) {
-    if (fabs(abc) > 0.5 * fabs(Z[ok + sc] - Z[ok])) sc = sc + isign1(sc); else break;
+    if (fabs(abc) > (R)0.5 * fabs(Z[ok + sc] - Z[ok])) sc = sc + isign1(sc); else break;
   }
 }
 
 // Result of a conservative remap: origin node, interior offsets, exterior
 // flag (bit 0 x, bit 1 y, bit 2 z) and the seven weights.
-struct Stencil {
+template <class R>
+struct StencilT {
   int oi, oj, ok, a, b, c, flag;
-  double m[7];  // f_o, f_ix, f_iy, f_iz, f_ex, f_ey, f_ez
+  R m[7];  // f_o, f_ix, f_iy, f_iz, f_ex, f_ey, f_ez
 };
+using Stencil = StencilT<double>;
 
 // compute_remapping_quantities (remapping.f90:92-394), the non-uniform branch
 // that trunk always takes. Moments 0, 1 and 2 of the 7-point stencil equal
 // f_off at (u,v,w) in exact arithmetic. flag == 0 is the reference's fatal case.
-__device__ __forceinline__ void remap(const Grid& g, double u, double v, double w, double f_off, Stencil& s) {
-  const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
+template <class R>
+__device__ __forceinline__ void remap_t(const Axes<R>& g, R u, R v, R w, R f_off, StencilT<R>& s) {
+  const R *X = g.a[0], *Y = g.a[1], *Z = g.a[2];
   const int imax = g.n[0] - 1, jmax = g.n[1] - 1, kmax = g.n[2] - 1;
   int corner = 0;
-  int oi = nearest_node(X, imax, u, first_guess(g.factor[0], u, g.vmin[0], imax), g.vmin[0], g.vmax[0]);
+  int oi = nearest_node<R>(X, imax, u, first_guess<R>(g.factor[0], u, g.vmin[0], imax), g.vmin[0], g.vmax[0]);
   if (oi >= imax) { oi = imax; ++corner; } else if (oi <= 0) { oi = 0; ++corner; }
-  int oj = nearest_node(Y, jmax, v, first_guess(g.factor[1], v, g.vmin[1], jmax), g.vmin[1], g.vmax[1]);
+  int oj = nearest_node<R>(Y, jmax, v, first_guess<R>(g.factor[1], v, g.vmin[1], jmax), g.vmin[1], g.vmax[1]);
   if (oj >= jmax) { oj = jmax; ++corner; } else if (oj <= 0) { oj = 0; ++corner; }
-  int ok = nearest_node(Z, kmax, w, first_guess(g.factor[2], w, g.vmin[2], kmax), g.vmin[2], g.vmax[2]);
+  int ok = nearest_node<R>(Z, kmax, w, first_guess<R>(g.factor[2], w, g.vmin[2], kmax), g.vmin[2], g.vmax[2]);
   if (ok >= kmax) { ok = kmax; ++corner; } else if (ok <= 0) { ok = 0; ++corner; }
   if (corner == 3) {
     oi -= isign1(oi - 1);
     oj -= isign1(oj - 1);
     ok -= isign1(ok - 1);
   }
-  const double ox = X[oi], oy = Y[oj], oz = Z[ok];
-  const double a = u - ox, b = v - oy, c = w - oz;
-  int sa = isgn(a), sb = isgn(b), sc = isgn(c);
-  if (oi + sa > imax || oi + sa < 0) { sa = -sa; boundary_stencil(sa, sb, sc, a, oi, oj, ok, X, Y, Z, imax, jmax, kmax); }
-  if (oj + sb > jmax || oj + sb < 0) { sb = -sb; boundary_stencil(sb, sa, sc, b, oj, oi, ok, Y, X, Z, jmax, imax, kmax); }
-  if (ok + sc > kmax || ok + sc < 0) { sc = -sc; boundary_stencil(sc, sb, sa, c, ok, oj, oi, Z, Y, X, kmax, jmax, imax); }
-  const double ix = X[oi + sa] - ox, iy = Y[oj + sb] - oy, iz = Z[ok + sc] - oz;
+  const R ox = X[oi], oy = Y[oj], oz = Z[ok];
+  const R a = u - ox, b = v - oy, c = w - oz;
+  int sa = isgn_r<R>(a), sb = isgn_r<R>(b), sc = isgn_r<R>(c);
+  if (oi + sa > imax || oi + sa < 0) { sa = -sa; boundary_stencil<R>(sa, sb, sc, a, oi, oj, ok, X, Y, Z, imax, jmax, kmax); }
+  if (oj + sb > jmax || oj + sb < 0) { sb = -sb; boundary_stencil<R>(sb, sa, sc, b, oj, oi, ok, Y, X, Z, jmax, imax, kmax); }
+  if (ok + sc > kmax || ok + sc < 0) { sc = -sc; boundary_stencil<R>(sc, sb, sa, c, ok, oj, oi, Z, Y, X, kmax, jmax, imax); }
+  const R ix = X[oi + sa] - ox, iy = Y[oj + sb] - oy, iz = Z[ok + sc] - oz;
   int flag = 0;
-  double ex = 0.0, ey = 0.0, ez = 0.0;
+  R ex = 0, ey = 0, ez = 0;
   if (oi - sa <= imax && oi - sa >= 0) { flag |= 1; ex = X[oi - sa] - ox; }
   if (oj - sb <= jmax && oj - sb >= 0) { flag |= 2; ey = Y[oj - sb] - oy; }
   if (ok - sc <= kmax && ok - sc >= 0) { flag |= 4; ez = Z[ok - sc] - oz; }
-  const double num = f_off * (a * a + b * b + c * c - a * ix - b * iy - c * iz);
-  const double dx = ex * (ex - ix), dy = ey * (ey - iy), dz = ez * (ez - iz);
-  double den;
+  const R num = f_off * (a * a + b * b + c * c - a * ix - b * iy - c * iz);
+  const R dx = ex * (ex - ix), dy = ey * (ey - iy), dz = ez * (ez - iz);
+  R den;
   switch (flag) {
     case 7: den = dx + dy + dz; break;
     case 1: den = dx; break;
@@ -214,26 +259,31 @@ __device__ __forceinline__ void remap(const Grid& g, double u, double v, double 
     case 3: den = dx + dy; break;
     case 5: den = dx + dz; break;
     case 6: den = dy + dz; break;
-    default: den = 1.0; break;
+    default: den = 1; break;
   }
-  const double fe = (flag != 0) ? num / den : 0.0;
-  const double fex = (flag & 1) ? fe : 0.0;
-  const double fey = (flag & 2) ? fe : 0.0;
-  const double fez = (flag & 4) ? fe : 0.0;
-  const double fix = (f_off * a - fex * ex) / ix;
-  const double fiy = (f_off * b - fey * ey) / iy;
-  const double fiz = (f_off * c - fez * ez) / iz;
+  const R fe = (flag != 0) ? quot<R>(num, den) : (R)0;
+  const R fex = (flag & 1) ? fe : (R)0;
+  const R fey = (flag & 2) ? fe : (R)0;
+  const R fez = (flag & 4) ? fe : (R)0;
+  const R fix = quot<R>(f_off * a - fex * ex, ix);
+  const R fiy = quot<R>(f_off * b - fey * ey, iy);
+  const R fiz = quot<R>(f_off * c - fez * ez, iz);
   s.oi = oi; s.oj = oj; s.ok = ok; s.a = sa; s.b = sb; s.c = sc; s.flag = flag;
   s.m[0] = f_off - fix - fiy - fiz - fex - fey - fez;
   s.m[1] = fix; s.m[2] = fiy; s.m[3] = fiz; s.m[4] = fex; s.m[5] = fey; s.m[6] = fez;
   if (flag == 0) {
 #pragma unroll
-    for (int q = 0; q < 7; ++q) s.m[q] = 0.0;
+    for (int q = 0; q < 7; ++q) s.m[q] = 0;
   }
 }
 
+__device__ __forceinline__ void remap(const Grid& g, double u, double v, double w, double f_off, Stencil& s) {
+  remap_t<double>(grid_axes(g), u, v, w, f_off, s);
+}
+
 // Node index of stencil point q (0 origin, 1..3 interior, 4..6 exterior).
-__device__ __forceinline__ void stencil_ijk(const Stencil& s, int q, int& i, int& j, int& k) {
+template <class S>
+__device__ __forceinline__ void stencil_ijk(const S& s, int q, int& i, int& j, int& k) {
   i = s.oi;
   j = s.oj;
   k = s.ok;
@@ -248,7 +298,8 @@ __device__ __forceinline__ void stencil_ijk(const Stencil& s, int q, int& i, int
   }
 }
 
-__device__ __forceinline__ int stencil_node(const Stencil& s, int q, int nx, int nxy) {
+template <class S>
+__device__ __forceinline__ int stencil_node(const S& s, int q, int nx, int nxy) {
   int i, j, k;
   stencil_ijk(s, q, i, j, k);
   return i + nx * j + nxy * k;

@@ -546,8 +546,9 @@ struct SampleRec {
   double kd1, kd2, m1[7], m2[7];
 };
 
-// FP32 collision arithmetic uses the hardware transcendental and square-root
-// instructions (v_sin_f32 / v_cos_f32 / v_sqrt_f32, ~1e-6 relative); FP64
+// FP32 collision arithmetic uses the hardware transcendental, reciprocal and
+// square-root instructions (v_sin_f32 / v_cos_f32 / v_rcp_f32 / v_sqrt_f32,
+// ~1e-6 relative or better); FP64
 // uses the correctly rounded library functions the reference's results need.
 template <class R>
 __device__ __forceinline__ void sin_cos(R x, R& s, R& c) {
@@ -559,7 +560,7 @@ __device__ __forceinline__ R rsqrt_free_sqrt(R x) {
 }
 template <class R>
 __device__ __forceinline__ R rdiv(R a, R b) {
-  if constexpr (sizeof(R) == 4) return __fdividef(a, b); else return a / b;
+  return quot<R>(a, b);
 }
 template <class R>
 __device__ __forceinline__ R rpow(R x, R y) {
@@ -697,11 +698,13 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
 // One VR sample: variance_reduction.f90:686-816 / 838-961 / 983-1116 with
 // vr_depletion_routine (:1127-1237) and the elastic branch of
 // replenish_collision (replenishing_collisions.f90:134-354).
-// R is the collision arithmetic (double, or float for the FP32-collision path);
-// the partner picks, the deviation signs and the remap weights are FP64 in both.
+// R is the collision arithmetic (double, or float for the FP32-collision path,
+// which also remaps in FP32 on the FP32 axes ax); the partner picks and the
+// deviation signs are FP64 in both, and every value scattered is FP64.
 // scatter(node, value) receives the two depletions and every stencil point.
 template <bool kRecord, class R, class Src, class Scatter>
-__device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, int cell, int ntime, int q, int t,
+__device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const CollParams& cp, int cell, int ntime,
+                                         int q, int t,
                                          int s, double factor, int lA, int lB, double signA_raw, double signB_raw,
                                          uint32_t wphi, uint32_t wth, const Src& src, Scatter&& scatter,
                                          SampleRec* rec) {
@@ -710,8 +713,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
   int iA, jA, kA, iB, jB, kB;
   node_ijk(g, lA, iA, jA, kA);
   node_ijk(g, lB, iB, jB, kB);
-  const R x1 = (R)g.ax[0][iA], y1 = (R)g.ax[1][jA], z1 = (R)g.ax[2][kA];
-  const R x2 = (R)g.ax[0][iB], y2 = (R)g.ax[1][jB], z2 = (R)g.ax[2][kB];
+  const R x1 = ax.a[0][iA], y1 = ax.a[1][jA], z1 = ax.a[2][kA];
+  const R x2 = ax.a[0][iB], y2 = ax.a[1][jB], z2 = ax.a[2][kB];
   if (kRecord) { rec->lA = lA; rec->lB = lB; rec->skipped = 0; }
   const R gx = x2 - x1, gy = y2 - y1, gz = z2 - z1;
   const R gg = rsqrt_free_sqrt<R>(gx * gx + gy * gy + gz * gz);
@@ -783,28 +786,28 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const CollParams& cp, in
       ry = gg * stt * sph;
       rz = gg * ct;
     }
-    // post-collision velocities, remapped with FP64 weights
+    // post-collision velocities, remapped in R
     // (remapping.f90:55-394 via apply_phi_replenishment :356-434)
     for (int side = 0; side < 2; ++side) {
-      Stencil st;
-      if (side == 0) remap(g, (double)(vc0 + gr1 * rx), (double)(vc1 + gr1 * ry), (double)(vc2 + gr1 * rz), 1.0, st);
-      else remap(g, (double)(vc0 + -(gr2 * rx)), (double)(vc1 + -(gr2 * ry)), (double)(vc2 + -(gr2 * rz)), 1.0, st);
+      StencilT<R> st;
+      if (side == 0) remap_t<R>(ax, vc0 + gr1 * rx, vc1 + gr1 * ry, vc2 + gr1 * rz, (R)1, st);
+      else remap_t<R>(ax, vc0 + -(gr2 * rx), vc1 + -(gr2 * ry), vc2 + -(gr2 * rz), (R)1, st);
       if (kRecord && r == 1) {
         int* o = side == 0 ? rec->st1 : rec->st2;
         double* m = side == 0 ? rec->m1 : rec->m2;
         o[0] = st.oi; o[1] = st.oj; o[2] = st.ok; o[3] = st.a; o[4] = st.b; o[5] = st.c; o[6] = st.flag;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) m[q] = st.m[q];
+        for (int q = 0; q < 7; ++q) m[q] = (double)st.m[q];
       }
       if (st.flag == 0) { bad = 1; continue; }
       if (!kRecord) {
-        const double kr = (double)(side == 0 ? kr1 : kr2);
+        const R kr = side == 0 ? kr1 : kr2;
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
           if (q < 4 || (st.flag & (1 << (q - 4)))) {
             int si, sj, sk;
             stencil_ijk(st, q, si, sj, sk);
-            scatter(si, sj, sk, si + nx * sj + nxy * sk, kr * st.m[q]);
+            scatter(si, sj, sk, si + nx * sj + nxy * sk, (double)(kr * st.m[q]));
           }
         }
       }
@@ -820,6 +823,32 @@ __device__ __forceinline__ void stage_axes(Grid& g, double* sh) {
     for (int q = threadIdx.x; q < g.n[d]; q += blockDim.x) p[q] = g.ax[d][q];
     g.ax[d] = p;
     p += g.n[d];
+  }
+}
+
+// The axes of a collision workgroup in its arithmetic R, staged into LDS at
+// sh (the caller's barrier publishes them): FP64 -- g then points at the
+// staged copy; FP32 -- a float copy, and g keeps the global FP64 axes, which
+// the FP64 conservation correction reads.
+template <class R>
+__device__ __forceinline__ Axes<R> stage_collision_axes(Grid& g, double* sh) {
+  if constexpr (sizeof(R) == 8) {
+    stage_axes(g, sh);
+    return grid_axes(g);
+  } else {
+    Axes<float> x;
+    float* p = reinterpret_cast<float*>(sh);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      for (int q = threadIdx.x; q < g.n[d]; q += blockDim.x) p[q] = (float)g.ax[d][q];
+      x.a[d] = p;
+      x.n[d] = g.n[d];
+      x.vmin[d] = (float)g.vmin[d];
+      x.vmax[d] = (float)g.vmax[d];
+      x.factor[d] = (float)g.factor[d];
+      p += g.n[d];
+    }
+    return x;
   }
 }
 
@@ -863,7 +892,8 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
 // Samples q in [q0, q1) of one cell (q indexes integral 1, then 2, then 3);
 // every depletion and stencil point goes to scatter(node, value).
 template <bool kF32, int BS, int K, class Src, class Scatter>
-__device__ __forceinline__ int collide_samples(const Grid& g, const CollParams& cp, const CellView& cv,
+__device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typename std::conditional<kF32, float, double>::type>& ax,
+                                               const CollParams& cp, const CellView& cv,
                                                int rng_cell, int ntime, int n0, int n1, int q0, int q1,
                                                const double* __restrict__ fac3, const Src& src, Scatter&& scatter) {
   using R = typename std::conditional<kF32, float, double>::type;
@@ -886,7 +916,7 @@ __device__ __forceinline__ int collide_samples(const Grid& g, const CollParams& 
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
       if (!valid[k]) continue;
-      bad |= vr_sample<false, R>(g, cp, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k], sgA[k],
+      bad |= vr_sample<false, R>(g, ax, cp, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k], sgA[k],
                                  sgB[k], wphi[k], wth[k], src, scatter, nullptr);
     }
   }
@@ -1037,7 +1067,8 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
 //         conservation correction runs here (csh its reduction space) and
 //         fd = f + corrected change is written directly.
 template <bool kF32, int BS, int K, class Src>
-__device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp, const CellView& cv, int rng_cell,
+__device__ __forceinline__ int collide_item(const Grid& g, const Axes<typename std::conditional<kF32, float, double>::type>& ax,
+                                            const CollParams& cp, const CellView& cv, int rng_cell,
                                             int ntime, int q0, int q1, const int* __restrict__ cnt,
                                             const double* __restrict__ fac, const Window& win, bool full, bool sole,
                                             double* acc, double* __restrict__ fd, double* __restrict__ da,
@@ -1051,14 +1082,14 @@ __device__ __forceinline__ int collide_item(const Grid& g, const CollParams& cp,
   double* tgt = kF32 ? da : fd;
   int bad;
   if (full) {
-    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
+    bad = collide_samples<kF32, BS, K>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int, int, int, int node, double v) {
                                          if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], v);
                                        });
   } else {
     const int o0 = win.o[0], o1 = win.o[1], o2 = win.o[2], wx = win.w[0], wxy = win.wxy;
     const unsigned w0 = win.w[0], w1 = win.w[1], w2 = win.w[2];
-    bad = collide_samples<kF32, BS, K>(g, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
+    bad = collide_samples<kF32, BS, K>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int i, int j, int k, int node, double v) {
                                          const unsigned a = i - o0, b = j - o1, c = k - o2;
                                          if (a < w0 && b < w1 && c < w2) atomicAdd(&acc[a + wx * b + wxy * c], v);
@@ -1126,7 +1157,8 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   const int N = g.N;
   Window win{};
   if (!kLds) win = make_window(g, wd.w, props + (size_t)cell * DVM_NPROP);
-  stage_axes(g, dyn);
+  using R = typename std::conditional<kF32, float, double>::type;
+  const Axes<R> ax = stage_collision_axes<R>(g, dyn);
   const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
                                 tabs + (size_t)cell * 2 * kNBucket, N);
   const int* cnt = counts + cell * 4;
@@ -1135,7 +1167,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
   Src src;
   if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
-  const int bad = collide_item<kF32, BS, K>(g, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
+  const int bad = collide_item<kF32, BS, K>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src);
   if (bad) atomicOr(err, bad);
@@ -1197,7 +1229,7 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     const int qi = q;
     pick_batch<1>(cp, cv, N, cell, ntime, &qi, &t, &s, &valid, &lA, &lB, &sgA, &sgB, &wphi, &wth, src);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
-    vr_sample<true, double>(g, cp, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth, src,
+    vr_sample<true, double>(g, grid_axes(g), cp, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth, src,
                             [](int, int, int, int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
