@@ -93,8 +93,9 @@ template <int BS, int V>
 __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ slots,
                                                   const SlotState* __restrict__ state,
                                                   const CellRef* __restrict__ cells, CdfParams base,
-                                                  BatchPools P) {
+                                                  BatchPools P, int sum_cap) {
   __shared__ CdfShared<BS, V> sh;
+  extern __shared__ float gdyn[];  // sum_cap floats: the scan's group ends
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
   const int cur = state[r.slot].cur;
@@ -107,7 +108,8 @@ __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ s
   cp.samples_per_cell = d.samples;
   const size_t n0 = cell_node0(d, r.cell);
   cdf_cell<BS, V>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
-           P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh);
+           P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh,
+           (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr);
 }
 
 // fd = f (FP64) or dacc = 0 (FP32) over the cells that take global atomics.
@@ -792,12 +794,15 @@ static int batch_lists(dvm_batch* b) {
 static int batch_enqueue_step(dvm_batch* b) {
   const bool f32 = batch_f32(b);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, b->stream, b->P.queue, kXcds);
-  if (b->n_cells >= 4 * b->cus)
-    hipLaunchKernelGGL((k_cdf_cells<kCdfBS, kCdfV>), dim3(b->n_cells), dim3(kCdfBS), 0, b->stream, b->d_slots,
-                       b->d_state, b->d_cells, b->cdfp, b->P);
-  else
-    hipLaunchKernelGGL((k_cdf_cells<kCdfWideBS, kCdfWideV>), dim3(b->n_cells), dim3(kCdfWideBS), 0, b->stream,
-                       b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P);
+  if (b->n_cells >= 4 * b->cus) {
+    const int cap = cdf_sum_cap((int)b->max_n, kCdfV);
+    hipLaunchKernelGGL((k_cdf_cells<kCdfBS, kCdfV>), dim3(b->n_cells), dim3(kCdfBS), cap * sizeof(float),
+                       b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap);
+  } else {
+    const int cap = cdf_sum_cap((int)b->max_n, kCdfWideV);
+    hipLaunchKernelGGL((k_cdf_cells<kCdfWideBS, kCdfWideV>), dim3(b->n_cells), dim3(kCdfWideBS), cap * sizeof(float),
+                       b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap);
+  }
   if (b->any_global) {
     const unsigned chunks = (unsigned)((b->max_n + 255) / 256);
     const unsigned ys = (unsigned)(b->n_cells < 65535 ? b->n_cells : 65535);

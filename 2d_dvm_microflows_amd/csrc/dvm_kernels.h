@@ -10,10 +10,19 @@ namespace dvm {
 constexpr int kMomBS = 256;      // moments / wall-flux workgroup
 constexpr int kMomCells = 4;     // cells per k_moments workgroup
 constexpr int kCdfBS = 256;      // CDF workgroup of the batch
-constexpr int kCdfCtxBS = 512;   // CDF workgroup of a context with many cells
+constexpr int kCdfCtxBS = 256;   // CDF workgroup of a context with many cells (256: 0.80 ms on the
+                                 // 64x64 x 25^3 cavity, 512: 1.05, 128: 0.96)
 constexpr int kCdfV = 8;         // consecutive nodes per lane in the CDF scan
 constexpr int kCdfWideBS = 1024; // CDF workgroup when the cells are too few to fill the chip
-constexpr int kCdfWideV = 4;     //   (its chunk tile stays within the 64 KB of static LDS)
+constexpr int kCdfWideV = 4;     //   (four consecutive nodes per lane)
+// LDS floats for the CDF scan's group ends (both distributions); cells whose
+// 2 ceil(N/V) group ends exceed it build their bucket tables from a full read
+constexpr int kCdfSumCap = 12288;
+// Dynamic LDS floats of a CDF launch whose largest grid has max_n nodes.
+inline int cdf_sum_cap(int max_n, int V) {
+  const int need = 2 * ((max_n + V - 1) / V);
+  return need < kCdfSumCap ? need : kCdfSumCap;
+}
 constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell's LDS slice
 constexpr int kConvBS = 256;     // convection lanes per workgroup
 #ifndef DVM_CONV_STRIP
@@ -81,7 +90,7 @@ template <int BS, int V>
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
                       double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
                       double* __restrict__ eqpar,
-                      int* __restrict__ counts, double* __restrict__ factors);
+                      int* __restrict__ counts, double* __restrict__ factors, int sum_cap);
 template <bool kStream>
 __global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
                         const double* __restrict__ eqv, const double* __restrict__ cdf,

@@ -11,8 +11,9 @@
 //   factors      [C][3]      depletion factor of each integral
 // Kernels:
 //   k_moments    one or four cells per workgroup; one pass, FP64 wave reductions
-//   k_cdf        one workgroup per cell; cache test, equilibrium rebuild on a
-//                miss, |deviation| prefix scan, bucket tables, pair counts
+//   k_cdf        one workgroup per cell, one pass over f: cache test,
+//                equilibrium rebuild on a miss, both prefix scans, bucket
+//                tables, pair counts
 //   k_collide    one workgroup per cell; samples spread over the lanes, the
 //                cell's increments accumulated in LDS (ds_add_f64) when the
 //                slice fits, else global atomics into fd
@@ -303,6 +304,72 @@ __device__ __forceinline__ void build_buckets(const double* __restrict__ cumul, 
   __syncthreads();
 }
 
+// build_buckets from the group ends G[gi] ~ cumul[min((gi + 1) V, N)] of the
+// ceil(N / V) groups of V consecutive nodes, held in LDS by the scan as
+// floats: a threshold in (cumul[gi V], cumul[min((gi + 1) V, N)]] is crossed
+// at a node of group gi, so only the groups whose float bounds, widened by
+// 2^-22 against the float rounding, admit a threshold are read back and given
+// the node test of build_buckets, which decides on the FP64 values; for a
+// non-decreasing CDF the brackets are those of build_buckets. A total below
+// 1e-30 (where float ends lose their relative accuracy) takes build_buckets.
+template <int BS, int V>
+__device__ __forceinline__ void build_buckets_grouped(const double* __restrict__ cumul, const float* __restrict__ G,
+                                                      int N, int2* __restrict__ tab, int* __restrict__ rights) {
+  const double total = cumul[N];
+  if (!(total >= 1e-30)) {
+    build_buckets<BS>(cumul, N, tab, rights);
+    return;
+  }
+  const int ng = (N + V - 1) / V;
+  for (int r = threadIdx.x; r < kNBucket; r += BS) rights[r] = N + 1;
+  __syncthreads();
+  const double mx = 0.001 * total;
+  const double inv = 1.0 / mx;
+  constexpr double kLo = 1.0 - 0x1.0p-22, kHi = 1.0 + 0x1.0p-22;
+  for (int gi = threadIdx.x; gi < ng; gi += BS) {
+    const double g0 = (gi > 0) ? (double)G[gi - 1] * kLo : 0.0, g1 = (double)G[gi] * kHi;
+    if (!(g1 > g0)) continue;
+    int r0 = (int)fmin(g0 * inv, 2000.0), r1 = (int)fmin(g1 * inv, 2000.0) + 1;
+    r0 = r0 < 1 ? 1 : r0;
+    r1 = r1 > kNBucket ? kNBucket : r1;
+    bool any = false;
+    for (int r = r0; r <= r1; ++r) {
+      const double th = (double)r * mx;
+      any = any || (th > g0 && th <= g1);
+    }
+    if (!any) continue;
+    const int lo = gi * V;
+    const int nq = (N - lo < V) ? N - lo : V;
+    double c[V + 1];
+#pragma unroll
+    for (int q = 0; q <= V; ++q) c[q] = (q <= nq) ? cumul[lo + q] : 0.0;
+#pragma unroll
+    for (int q = 1; q <= V; ++q) {
+      const double c0 = c[q - 1], c1 = c[q];
+      if (q > nq || !(c1 > c0)) continue;
+      int s0 = (int)fmin(c0 * inv, 2000.0), s1 = (int)fmin(c1 * inv, 2000.0) + 1;
+      s0 = s0 < 1 ? 1 : s0;
+      s1 = s1 > kNBucket ? kNBucket : s1;
+      for (int r = s0; r <= s1; ++r) {
+        const double th = (double)r * mx;
+        if (th > c0 && th <= c1) rights[r - 1] = lo + q;
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x + 1; r <= kNBucket; r += BS) {
+    const int left = (r == 1) ? 0 : rights[r - 2] - 1;
+    const int right = (r == kNBucket) ? N : rights[r - 1];
+    tab[r - 1] = make_int2(left, right);
+  }
+  __syncthreads();
+}
+
+// Position in the LDS chunk tile of node m of a chunk: lane m / V owns V
+// consecutive nodes, rows padded to V + 1 doubles against bank conflicts.
+template <int V>
+__device__ __forceinline__ int tile_pos(int m) { return (m / V) * (V + 1) + (m % V); }
+
 // Workgroup-wide scan of V consecutive values per lane: v[q] becomes
 // carry + (prefix of everything before this lane in the chunk) + the lane's
 // own running sum. Returns the chunk total.
@@ -318,25 +385,112 @@ __device__ __forceinline__ double chunk_scan(double (&v)[V], double* red, double
   return tot;
 }
 
-// Position in the LDS chunk tile of node m of a chunk: lane m / V owns V
-// consecutive nodes, rows padded to V + 1 doubles against bank conflicts.
-template <int V>
-__device__ __forceinline__ int tile_pos(int m) { return (m / V) * (V + 1) + (m % V); }
+// chunk_scan that also gives each lane the value its first node's predecessor
+// took (the previous lane's last, or prev_last for lane 0) and the chunk's
+// last value `last`, bit for bit, without another round of barriers: lane 0 of
+// a wave repeats the arithmetic by which lane 63 of the wave before formed its
+// last value, from that lane's inclusive prefix and local total (red[0], red[1]).
+template <int BS, int V>
+__device__ __forceinline__ double chunk_scan_prev(double (&v)[V], double (*red)[BS / 64], double carry,
+                                                  double prev_last, double& before, double& last) {
+#pragma unroll
+  for (int q = 1; q < V; ++q) v[q] = v[q - 1] + v[q];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double loc = v[V - 1];
+  const double inc = wave_inclusive_scan(loc, lane);
+  if (lane == 63) { red[0][wid] = inc; red[1][wid] = loc; }
+  __syncthreads();
+  double off = 0.0, tot = 0.0, offp = 0.0, offl = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) {
+    const double s = red[0][w];
+    if (w < wid) off += s;
+    if (w < wid - 1) offp += s;
+    if (w < BS / 64 - 1) offl += s;
+    tot += s;
+  }
+  // lane 63 of wave x: v[V-1] = (carry + ((red[0][x] + off_x) - red[1][x])) + red[1][x]
+  const int px = wid > 0 ? wid - 1 : 0;
+  const double lp = (carry + ((red[0][px] + offp) - red[1][px])) + red[1][px];
+  const double ll = (carry + ((red[0][BS / 64 - 1] + offl) - red[1][BS / 64 - 1])) + red[1][BS / 64 - 1];
+  __syncthreads();
+  const double excl = carry + ((inc + off) - loc);
+#pragma unroll
+  for (int q = 0; q < V; ++q) v[q] = excl + v[q];
+  const double up = __shfl_up(v[V - 1], 1, 64);
+  before = (lane > 0) ? up : (wid > 0 ? lp : prev_last);
+  last = ll;
+  return tot;
+}
+
+// chunk_scan of two sequences in one round of barriers; the chunk totals are
+// added to the carries.
+template <int BS, int V>
+__device__ __forceinline__ void chunk_scan2(double (&a)[V], double (&b)[V], double (*red)[BS / 64], double& ca,
+                                            double& cb) {
+#pragma unroll
+  for (int q = 1; q < V; ++q) {
+    a[q] = a[q - 1] + a[q];
+    b[q] = b[q - 1] + b[q];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double ia = wave_inclusive_scan(a[V - 1], lane);
+  const double ib = wave_inclusive_scan(b[V - 1], lane);
+  if (lane == 63) { red[0][wid] = ia; red[1][wid] = ib; }
+  __syncthreads();
+  double oa = 0.0, ob = 0.0, ta = 0.0, tb = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) {
+    const double x = red[0][w], y = red[1][w];
+    if (w < wid) { oa += x; ob += y; }
+    ta += x;
+    tb += y;
+  }
+  __syncthreads();
+  const double ea = ca + ((ia + oa) - a[V - 1]), eb = cb + ((ib + ob) - b[V - 1]);
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    a[q] = ea + a[q];
+    b[q] = eb + b[q];
+  }
+  ca = ca + ta;
+  cb = cb + tb;
+}
 
 // compute_cdf_vr (variance_reduction.f90:234-496) for the like-species pair,
 // plus compute_num_coll_pairs (collision_utilities.f90:106-140) and the
 // depletion factors (variance_reduction.f90:598-604,679,831,976).
-// One workgroup of BS lanes per cell. Each chunk of BS*V nodes is loaded and
-// stored coalesced through an LDS tile in which lane t owns V consecutive
-// nodes for the scan.
+// One workgroup of BS lanes per cell, one pass over the nodes in chunks of
+// BS*V: lane t owns V consecutive nodes for the scans; loads go coalesced
+// through an LDS tile, and so do stores. On an equilibrium rebuild the pass
+// computes f_eq*beta^3, its prefix eq_df, the differences eqv and both
+// cumulative distributions (eqv and |f - eqv|), so f is read once and eqv is
+// not read back; otherwise it reads f and the cached eqv. With gsum (LDS,
+// 2 ceil(N/V) floats) the lanes' group ends are kept for the bucket tables,
+// which then read back only the groups a threshold falls in.
 template <int BS, int V>
 struct CdfShared {
-  double red[BS / 64];
-  double lastp[BS];
+  double red[2][BS / 64];
   double tile[BS * (V + 1)];
   int rights[kNBucket];
   int s_new;
 };
+
+// Coalesced store of a chunk held as V consecutive values per lane, through
+// the tile (whose last reads precede a barrier the caller has passed).
+template <int BS, int V>
+__device__ __forceinline__ void tile_store(const double (&x)[V], double* tile, double* __restrict__ dst, int base,
+                                           int N) {
+  const int own = threadIdx.x * (V + 1);
+#pragma unroll
+  for (int q = 0; q < V; ++q) tile[own + q] = x[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    const int m = q * BS + threadIdx.x, l = base + m;
+    if (l < N) dst[l] = tile[tile_pos<V>(m)];
+  }
+}
 
 // One cell: fc its distribution, p its moments, ev / ec / tab2 / par / cnt4 /
 // fac3 its slices of eqv, cdf ([2][N+1]), tabs ([2][1000]), eqpar, counts and
@@ -345,40 +499,72 @@ template <int BS, int V>
 __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, const double* __restrict__ fc,
                                          const double* __restrict__ p, double* __restrict__ ev,
                                          double* __restrict__ ec, int2* __restrict__ tab2, double* __restrict__ par,
-                                         int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared<BS, V>& sh) {
+                                         int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared<BS, V>& sh,
+                                         float* gsum) {
   constexpr int CH = BS * V;
-  double* red = sh.red;
-  double* lastp = sh.lastp;
-  double* tile = sh.tile;
   int* rights = sh.rights;
-  int& s_new = sh.s_new;
+  double* tile = sh.tile;
   const int N = g.N;
-  const int own = threadIdx.x * (V + 1);
+  const int ng = (N + V - 1) / V;
   double* nc = ec + (N + 1);
   const double dens = p[DVM_P_DENS], u = p[DVM_P_U], v = p[DVM_P_V], w = p[DVM_P_W];
   double temp = p[DVM_P_TEMP];
   const double mass = cp.mass;
+  const bool fmc = cp.fmc != 0;
   if (threadIdx.x == 0) {
     const double tol = cp.cdf_tolerance;
     // the equilibrium cache test: w is not compared; a negative u or v makes
     // its test always true
-    s_new = (!cp.fmc && (fabs(dens - par[0]) > tol * dens || fabs(u - par[1]) > tol * u ||
-                         fabs(v - par[2]) > tol * v || fabs(temp - par[4]) > tol * temp)) ? 1 : 0;
+    sh.s_new = (!cp.fmc && (fabs(dens - par[0]) > tol * dens || fabs(u - par[1]) > tol * u ||
+                            fabs(v - par[2]) > tol * v || fabs(temp - par[4]) > tol * temp)) ? 1 : 0;
   }
   __syncthreads();
-  const bool new_eq = s_new != 0;
+  const bool new_eq = sh.s_new != 0;
   __syncthreads();
-  if (new_eq) {
-    double coeff = 0.0;
-    if (dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
-    // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
-    // f_eq needs no loads: each lane evaluates its own V consecutive nodes,
-    // stepping (i, j, k) along the node order
-    const int nx = g.n[0], ny = g.n[1];
-    double carry = 0.0, carry_c = 0.0, prev_last = 0.0;
-    if (threadIdx.x == 0) ec[0] = 0.0;
-    for (int base = 0; base < N; base += CH) {
-      const int l0 = base + (int)threadIdx.x * V;
+  double coeff = 0.0;
+  if (new_eq && dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
+  const int nx = g.n[0], ny = g.n[1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int own = threadIdx.x * (V + 1);
+  // carries: eq_df prefix, eqv prefix, |f - eqv| prefix; last eq_df of the previous chunk
+  double cpv = 0.0, cec = 0.0, cnc = 0.0, prev_last = 0.0;
+  if (threadIdx.x == 0) {
+    nc[0] = 0.0;
+    if (new_eq) ec[0] = 0.0;
+  }
+  // f (and, on a cache hit, eqv) of the next chunk are loaded while this one is scanned
+  double nf[V], ne[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    const int l = q * BS + threadIdx.x;
+    nf[q] = (l < N) ? fc[l] : 0.0;
+    ne[q] = (l < N && !new_eq && !fmc) ? ev[l] : 0.0;
+  }
+  for (int base = 0; base < N; base += CH) {
+    const int l0 = base + (int)threadIdx.x * V;
+    double fv[V], e[V], d[V];
+    if (new_eq) {
+#pragma unroll
+      for (int q = 0; q < V; ++q) tile[tile_pos<V>(q * BS + threadIdx.x)] = nf[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; ++q) tile[tile_pos<V>(q * BS + threadIdx.x)] = fabs(nf[q] - ne[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int l = base + CH + q * BS + threadIdx.x;
+      if (l < N) {
+        nf[q] = fc[l];
+        ne[q] = (new_eq || fmc) ? 0.0 : ev[l];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < V; ++q) fv[q] = tile[own + q];
+    if (new_eq) {
+      // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv.
+      // f_eq needs no loads: each lane evaluates its own V consecutive nodes,
+      // stepping (i, j, k) along the node order
       int i, j, k;
       node_ijk(g, l0 < N ? l0 : 0, i, j, k);
       double pv[V];
@@ -401,79 +587,44 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
         pv[q] = val;
         if (++i == nx) { i = 0; if (++j == ny) { j = 0; ++k; } }
       }
-      carry = carry + chunk_scan<BS, V>(pv, red, carry);
-      lastp[threadIdx.x] = pv[V - 1];
-      __syncthreads();
-      const double before = (threadIdx.x == 0) ? prev_last : lastp[threadIdx.x - 1];
-      prev_last = lastp[BS - 1];
-      double e[V];
+      // eq_df of the node before this lane's first: the previous lane's last
+      double before, last;
+      cpv = cpv + chunk_scan_prev<BS, V>(pv, sh.red, cpv, prev_last, before, last);
+      prev_last = last;
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         e[q] = pv[q] - (q == 0 ? before : pv[q - 1]);
-        tile[own + q] = e[q];
+        d[q] = (l0 + q < N) ? fabs(fv[q] - e[q]) : 0.0;
       }
+      tile_store<BS, V>(e, tile, ev, base, N);
+      chunk_scan2<BS, V>(e, d, sh.red, cec, cnc);
+      tile_store<BS, V>(e, tile, ec + 1, base, N);
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < V; ++q) {
-        const int m = q * BS + threadIdx.x, l = base + m;
-        if (l < N) ev[l] = tile[tile_pos<V>(m)];
+      tile_store<BS, V>(d, tile, nc + 1, base, N);
+      if (gsum && l0 < N) {
+        gsum[l0 / V] = (float)e[V - 1];
+        gsum[ng + l0 / V] = (float)d[V - 1];
       }
-      carry_c = carry_c + chunk_scan<BS, V>(e, red, carry_c);
+    } else {
+      // |f - f_eq| prefix with the cached eqv; |f| prefix for the full Monte
+      // Carlo kernel (full_mc.f90:110-180), which takes f_eq = 0
 #pragma unroll
-      for (int q = 0; q < V; ++q) tile[own + q] = e[q];
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < V; ++q) {
-        const int m = q * BS + threadIdx.x, l = base + m;
-        if (l < N) ec[l + 1] = tile[tile_pos<V>(m)];
-      }
-      // the next chunk writes the tile only after its first scan's barriers
+      for (int q = 0; q < V; ++q) d[q] = fv[q];
+      cnc = cnc + chunk_scan<BS, V>(d, sh.red[0], cnc);
+      tile_store<BS, V>(d, tile, nc + 1, base, N);
+      if (gsum && l0 < N) gsum[ng + l0 / V] = (float)d[V - 1];
     }
-    build_buckets<BS>(ec, N, tab2, rights);
+    __syncthreads();  // the tile's reads end before the next chunk writes it
+  }
+  if (new_eq) {
+    if (gsum) build_buckets_grouped<BS, V>(ec, gsum, N, tab2, rights);
+    else build_buckets<BS>(ec, N, tab2, rights);
     if (threadIdx.x == 0) {
       par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
     }
   }
-  // |f - f_eq| prefix; |f| prefix for the full Monte Carlo kernel
-  // (full_mc.f90:110-180), which takes f_eq = 0
-  const bool fmc = cp.fmc != 0;
-  double carry = 0.0;
-  if (threadIdx.x == 0) nc[0] = 0.0;
-  // f and f_eq of the next chunk are loaded while this one is scanned
-  double nf[V], ne[V];
-#pragma unroll
-  for (int q = 0; q < V; ++q) {
-    const int l = q * BS + threadIdx.x;
-    nf[q] = (l < N) ? fc[l] : 0.0;
-    ne[q] = (l < N && !fmc) ? ev[l] : 0.0;
-  }
-  for (int base = 0; base < N; base += CH) {
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-      const int m = q * BS + threadIdx.x, l = base + m;
-      tile[tile_pos<V>(m)] = (l < N) ? fabs(nf[q] - ne[q]) : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-      const int l = base + CH + q * BS + threadIdx.x;
-      if (l < N) { nf[q] = fc[l]; ne[q] = fmc ? 0.0 : ev[l]; }
-    }
-    __syncthreads();
-    double d[V];
-#pragma unroll
-    for (int q = 0; q < V; ++q) d[q] = tile[own + q];
-    carry = carry + chunk_scan<BS, V>(d, red, carry);
-#pragma unroll
-    for (int q = 0; q < V; ++q) tile[own + q] = d[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-      const int m = q * BS + threadIdx.x, l = base + m;
-      if (l < N) nc[l + 1] = tile[tile_pos<V>(m)];
-    }
-    __syncthreads();
-  }
-  build_buckets<BS>(nc, N, tab2 + kNBucket, rights);
+  if (gsum) build_buckets_grouped<BS, V>(nc, gsum + ng, N, tab2 + kNBucket, rights);
+  else build_buckets<BS>(nc, N, tab2 + kNBucket, rights);
   if (fmc) {
     // full_monte_carlo_kernel (full_mc.f90:240-317): abs_dens = dens - 2 neg_dens
     // with neg_dens = (dens - sum |f|) / 2; compute_num_coll_pairs with both
@@ -531,13 +682,16 @@ __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* 
                                                 const double* __restrict__ props, double* __restrict__ eqv,
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
                                                 double* __restrict__ eqpar,
-                                                int* __restrict__ counts, double* __restrict__ factors) {
+                                                int* __restrict__ counts, double* __restrict__ factors,
+                                                int sum_cap) {
   __shared__ CdfShared<BS, V> sh;
+  extern __shared__ float gdyn[];  // sum_cap floats: the scan's group ends
   const int cell = blockIdx.x;
   const int N = g.N;
+  float* gsum = (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr;
   cdf_cell<BS, V>(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
            cdf + (size_t)cell * 2 * (N + 1), tabs + (size_t)cell * 2 * kNBucket, eqpar + (size_t)cell * 5,
-           counts + cell * 4, factors + cell * 3, sh);
+           counts + cell * 4, factors + cell * 3, sh, gsum);
 }
 
 // --------------------------------------------------------------- collide
