@@ -479,7 +479,7 @@ def run_serving(pkg, count, rate, precision):
                         f"50 samples/node, stop at L1 <= 1e-4 or L1 plateau or 300 steps), 30% 1-D shocks "
                         f"(Mach U[1.2,3], 50-400 cells, edge 21..31 odd, 20000 samples/cell, run to steady "
                         f"state: mass flux within 2% of the inflow's and the 100-step mean density profile "
-                        f"steady within its Monte Carlo noise, in two consecutive windows, or 6000 steps); "
+                        f"changing by at most 1e-3 of its peak, in two consecutive windows, or 6000 steps); "
                         f"FP32 collision arithmetic"}
 
 
