@@ -762,13 +762,11 @@ struct StreamSrc {
 // 142-186, with referenced_search, sort_and_search.f90:40-86). The 2K
 // inverse-CDF searches advance in lockstep so that their dependent loads from
 // the cell's CDFs (L2/MALL resident, not LDS) overlap; each search follows the
-// reference's bucket bracket and bisection exactly. Also returns the sign of
-// the deviation at each deviation partner (+1 for an equilibrium partner).
+// reference's bucket bracket and bisection exactly.
 template <int K, class Src>
 __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView& cv, int N, int cell, int ntime,
                                            const int* q, const int* t, const int* s, const bool* valid, int* lA,
-                                           int* lB, double* sgA, double* sgB, uint32_t* wphi, uint32_t* wth,
-                                           const Src& src) {
+                                           int* lB, uint32_t* wphi, uint32_t* wth, const Src& src) {
   constexpr int M = 2 * K;
   // the cell's two CDFs are adjacent: eq at cv.eqc, deviation at cv.eqc + N + 1
   int off[M], left[M], right[M], idx[M];
@@ -832,20 +830,10 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
     }
     if (!any) break;
   }
-  double fa[K], ea[K], fb[K], eb[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     lA[k] = idx[2 * k];
     lB[k] = idx[2 * k + 1];
-    const int a = ((unsigned)lA[k] < (unsigned)N) ? lA[k] : 0;
-    const int b = ((unsigned)lB[k] < (unsigned)N) ? lB[k] : 0;
-    fa[k] = cv.f[a]; ea[k] = cv.eqv[a];
-    fb[k] = cv.f[b]; eb[k] = cv.eqv[b];
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    sgA[k] = (t[k] == 0) ? 1.0 : dsgn(fa[k] - ea[k]);
-    sgB[k] = (t[k] == 1) ? 1.0 : dsgn(fb[k] - eb[k]);
   }
 }
 
@@ -856,14 +844,17 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
 // which also remaps in FP32 on the FP32 axes ax); the partner picks and the
 // deviation signs are FP64 in both, and every value scattered is FP64.
 // scatter(node, value) receives the two depletions and every stencil point.
+// The sign of the deviation at each deviation partner (+1 for an equilibrium
+// partner) is read from the cell's f and f_eq at the start and first used
+// after the velocity arithmetic, which covers the loads' latency.
 template <bool kRecord, class R, class Src, class Scatter>
-__device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const CollParams& cp, int cell, int ntime,
-                                         int q, int t,
-                                         int s, double factor, int lA, int lB, double signA_raw, double signB_raw,
+__device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const CollParams& cp, const CellView& cv,
+                                         int cell, int ntime, int q, int t, int s, double factor, int lA, int lB,
                                          uint32_t wphi, uint32_t wth, const Src& src, Scatter&& scatter,
                                          SampleRec* rec) {
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
   if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) return 1;  // unreachable for finite CDFs
+  const double fa = cv.f[lA], ea = cv.eqv[lA], fb = cv.f[lB], eb = cv.eqv[lB];
   int iA, jA, kA, iB, jB, kB;
   node_ijk(g, lA, iA, jA, kA);
   node_ijk(g, lB, iB, jB, kB);
@@ -877,8 +868,16 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const
     return 0;
   }
   const R g_sigma = cp.hard_sphere ? gg : rpow<R>(gg, (R)cp.vhs_exponent);
-  const R signA = (R)signA_raw;
-  const R signB = (R)signB_raw;
+  const R mf = (R)(cp.mass / (cp.mass + cp.mass));
+  const R vc0 = mf * x1 + mf * x2, vc1 = mf * y1 + mf * y2, vc2 = mf * z1 + mf * z2;
+  const R gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
+  const R gx2 = fabs(x2 - vc0), gy2 = fabs(y2 - vc1), gz2 = fabs(z2 - vc2);
+  const R ur = gx1 + gx2, vr = gy1 + gy2, wr = gz1 + gz2;
+  const R gp1 = rsqrt_free_sqrt<R>(gx1 * gx1 + gy1 * gy1 + gz1 * gz1);
+  const R gp2 = rsqrt_free_sqrt<R>(gx2 * gx2 + gy2 * gy2 + gz2 * gz2);
+  const R gr1 = rdiv<R>(gp1, gg), gr2 = rdiv<R>(gp2, gg);
+  const R signA = (R)((t == 0) ? 1.0 : dsgn(fa - ea));
+  const R signB = (R)((t == 1) ? 1.0 : dsgn(fb - eb));
   const R sumA = signA, sumB = signB;
   const R fac = (R)factor;
   const R dep1 = sumB * (R)1 * fac * g_sigma;
@@ -892,14 +891,6 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const
   // kd / NRepl; NRepl = 1 in the reference's decks, where the quotient is kd exactly
   const R kr1 = (cp.num_repl == 1) ? kd1 : kd1 / (R)cp.num_repl;
   const R kr2 = (cp.num_repl == 1) ? kd2 : kd2 / (R)cp.num_repl;
-  const R mf = (R)(cp.mass / (cp.mass + cp.mass));
-  const R vc0 = mf * x1 + mf * x2, vc1 = mf * y1 + mf * y2, vc2 = mf * z1 + mf * z2;
-  const R gx1 = fabs(x1 - vc0), gy1 = fabs(y1 - vc1), gz1 = fabs(z1 - vc2);
-  const R gx2 = fabs(x2 - vc0), gy2 = fabs(y2 - vc1), gz2 = fabs(z2 - vc2);
-  const R ur = gx1 + gx2, vr = gy1 + gy2, wr = gz1 + gz2;
-  const R gp1 = rsqrt_free_sqrt<R>(gx1 * gx1 + gy1 * gy1 + gz1 * gz1);
-  const R gp2 = rsqrt_free_sqrt<R>(gx2 * gx2 + gy2 * gy2 + gz2 * gz2);
-  const R gr1 = rdiv<R>(gp1, gg), gr2 = rdiv<R>(gp2, gg);
   int bad = 0;
   for (int r = 1; r <= cp.num_repl; ++r) {
     double uphi, uth;
@@ -1056,7 +1047,6 @@ __device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typenam
   for (int base = q0; base < q1; base += BS * K) {
     int qq[K], tt[K], ss[K], lA[K], lB[K];
     bool valid[K];
-    double sgA[K], sgB[K];
     uint32_t wphi[K], wth[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1066,12 +1056,12 @@ __device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typenam
       tt[k] = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
       ss[k] = (tt[k] == 0) ? q : (tt[k] == 1 ? q - n0 : q - n0 - n1);
     }
-    pick_batch<K>(cp, cv, N, rng_cell, ntime, qq, tt, ss, valid, lA, lB, sgA, sgB, wphi, wth, src);
+    pick_batch<K>(cp, cv, N, rng_cell, ntime, qq, tt, ss, valid, lA, lB, wphi, wth, src);
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
       if (!valid[k]) continue;
-      bad |= vr_sample<false, R>(g, ax, cp, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k], sgA[k],
-                                 sgB[k], wphi[k], wth[k], src, scatter, nullptr);
+      bad |= vr_sample<false, R>(g, ax, cp, cv, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k],
+                                 wphi[k], wth[k], src, scatter, nullptr);
     }
   }
   return bad;
@@ -1378,12 +1368,11 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     rec.kd1 = 0.0; rec.kd2 = 0.0;
     const bool valid = true;
     int lA, lB;
-    double sgA, sgB;
     uint32_t wphi, wth;
     const int qi = q;
-    pick_batch<1>(cp, cv, N, cell, ntime, &qi, &t, &s, &valid, &lA, &lB, &sgA, &sgB, &wphi, &wth, src);
+    pick_batch<1>(cp, cv, N, cell, ntime, &qi, &t, &s, &valid, &lA, &lB, &wphi, &wth, src);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
-    vr_sample<true, double>(g, grid_axes(g), cp, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, sgA, sgB, wphi, wth, src,
+    vr_sample<true, double>(g, grid_axes(g), cp, cv, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, wphi, wth, src,
                             [](int, int, int, int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
