@@ -89,7 +89,7 @@ __global__ void k_zero_int(int* __restrict__ p, int n) {
   for (int q = threadIdx.x; q < n; q += blockDim.x) p[q] = 0;
 }
 
-template <int BS, int V>
+template <int BS, int V, bool kFused>
 __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ slots,
                                                   const SlotState* __restrict__ state,
                                                   const CellRef* __restrict__ cells, CdfParams base,
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ s
   cp.beta3 = d.beta3;
   cp.samples_per_cell = d.samples;
   const size_t n0 = cell_node0(d, r.cell);
-  cdf_cell<BS, V>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
+  cdf_cell<BS, V, kFused>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
            P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh,
            (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr);
 }
@@ -796,11 +796,11 @@ static int batch_enqueue_step(dvm_batch* b) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, b->stream, b->P.queue, kXcds);
   if (b->n_cells >= 4 * b->cus) {
     const int cap = cdf_sum_cap((int)b->max_n, kCdfV);
-    hipLaunchKernelGGL((k_cdf_cells<kCdfBS, kCdfV>), dim3(b->n_cells), dim3(kCdfBS), cap * sizeof(float),
+    hipLaunchKernelGGL((k_cdf_cells<kCdfBS, kCdfV, true>), dim3(b->n_cells), dim3(kCdfBS), cap * sizeof(float),
                        b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap);
   } else {
     const int cap = cdf_sum_cap((int)b->max_n, kCdfWideV);
-    hipLaunchKernelGGL((k_cdf_cells<kCdfWideBS, kCdfWideV>), dim3(b->n_cells), dim3(kCdfWideBS), cap * sizeof(float),
+    hipLaunchKernelGGL((k_cdf_cells<kCdfWideBS, kCdfWideV, false>), dim3(b->n_cells), dim3(kCdfWideBS), cap * sizeof(float),
                        b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap);
   }
   if (b->any_global) {

@@ -722,11 +722,11 @@ static int cdf_async(dvm_ctx* c) {
   // few cells (fewer than four workgroups per CU): 1024 lanes per cell
   if (c->ncell >= 4 * c->cus) {
     const int cap = cdf_sum_cap(c->g.N, kCdfV);
-    hipLaunchKernelGGL((k_cdf<kCdfCtxBS, kCdfV>), dim3(c->ncell), dim3(kCdfCtxBS), cap * sizeof(float), c->stream,
+    hipLaunchKernelGGL((k_cdf<kCdfCtxBS, kCdfV, true>), dim3(c->ncell), dim3(kCdfCtxBS), cap * sizeof(float), c->stream,
                        c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors, cap);
   } else {
     const int cap = cdf_sum_cap(c->g.N, kCdfWideV);
-    hipLaunchKernelGGL((k_cdf<kCdfWideBS, kCdfWideV>), dim3(c->ncell), dim3(kCdfWideBS), cap * sizeof(float),
+    hipLaunchKernelGGL((k_cdf<kCdfWideBS, kCdfWideV, false>), dim3(c->ncell), dim3(kCdfWideBS), cap * sizeof(float),
                        c->stream, c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts,
                        c->factors, cap);
   }

@@ -86,7 +86,7 @@ __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restri
                           int ncell);
 __global__ void k_moments_ext(Grid g, const double* __restrict__ f, const double* __restrict__ props, double mass,
                               double* __restrict__ ext);
-template <int BS, int V>
+template <int BS, int V, bool kFused>
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
                       double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
                       double* __restrict__ eqpar,

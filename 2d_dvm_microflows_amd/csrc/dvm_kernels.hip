@@ -492,10 +492,44 @@ __device__ __forceinline__ void tile_store(const double (&x)[V], double* tile, d
   }
 }
 
+// f_eq*beta^3 of V consecutive nodes from l0 (zero beyond N): no loads, each
+// lane steps (i, j, k) along the node order (dist_func.f90:317-344).
+template <int V>
+__device__ __forceinline__ void eq_values(const Grid& g, int l0, double coeff, double u, double v, double w,
+                                          double temp, double mass, double (&pv)[V]) {
+  const int N = g.N, nx = g.n[0], ny = g.n[1];
+  int i, j, k;
+  node_ijk(g, l0 < N ? l0 : 0, i, j, k);
+#pragma unroll
+  for (int q = 0; q < V; ++q) {
+    double val = 0.0;
+    if (l0 + q < N) {
+      const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+      const double beta3 = g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
+      double phi_eq = 0.0;
+      if (temp > 0.0) {
+        double cx = (x - u); cx = cx * cx;
+        double cy = (y - v); cy = cy * cy;
+        double cz = (z - w); cz = cz * cz;
+        const double csq = cx + cy + cz;
+        phi_eq = coeff * exp(-csq * mass / temp);
+      }
+      val = phi_eq * beta3;
+    }
+    pv[q] = val;
+    if (++i == nx) { i = 0; if (++j == ny) { j = 0; ++k; } }
+  }
+}
+
 // One cell: fc its distribution, p its moments, ev / ec / tab2 / par / cnt4 /
 // fac3 its slices of eqv, cdf ([2][N+1]), tabs ([2][1000]), eqpar, counts and
 // factors.
-template <int BS, int V>
+//   kFused: an equilibrium rebuild shares the one pass over f (many cells in
+//           flight, where the pass is bandwidth-bound); otherwise (few cells,
+//           where every barrier of a chunk is on the critical path) the
+//           equilibrium is built by a pass of its own with no loads, and the
+//           |f - f_eq| pass reads it back.
+template <int BS, int V, bool kFused>
 __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, const double* __restrict__ fc,
                                          const double* __restrict__ p, double* __restrict__ ev,
                                          double* __restrict__ ec, int2* __restrict__ tab2, double* __restrict__ par,
@@ -523,27 +557,48 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   __syncthreads();
   double coeff = 0.0;
   if (new_eq && dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
-  const int nx = g.n[0], ny = g.n[1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int own = threadIdx.x * (V + 1);
+  const bool fuse = kFused && new_eq;
   // carries: eq_df prefix, eqv prefix, |f - eqv| prefix; last eq_df of the previous chunk
   double cpv = 0.0, cec = 0.0, cnc = 0.0, prev_last = 0.0;
   if (threadIdx.x == 0) {
     nc[0] = 0.0;
     if (new_eq) ec[0] = 0.0;
   }
-  // f (and, on a cache hit, eqv) of the next chunk are loaded while this one is scanned
+  if (!kFused && new_eq) {
+    // the equilibrium's own pass: eq_df = prefix of f_eq*beta^3,
+    // eqv(l) = eq_df(l) - eq_df(l-1), eqc = prefix of eqv
+    for (int base = 0; base < N; base += CH) {
+      double pv[V], e[V];
+      eq_values<V>(g, base + (int)threadIdx.x * V, coeff, u, v, w, temp, mass, pv);
+      double before, last;
+      cpv = cpv + chunk_scan_prev<BS, V>(pv, sh.red, cpv, prev_last, before, last);
+      prev_last = last;
+#pragma unroll
+      for (int q = 0; q < V; ++q) e[q] = pv[q] - (q == 0 ? before : pv[q - 1]);
+      tile_store<BS, V>(e, tile, ev, base, N);
+      cec = cec + chunk_scan<BS, V>(e, sh.red[0], cec);
+      tile_store<BS, V>(e, tile, ec + 1, base, N);
+      // the next chunk writes the tile only after its first scan's barriers
+    }
+    build_buckets<BS>(ec, N, tab2, rights);  // its barriers also publish ev
+    if (threadIdx.x == 0) {
+      par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
+    }
+  }
+  // f (and, unless the equilibrium is built in this pass, eqv) of the next
+  // chunk are loaded while this one is scanned
   double nf[V], ne[V];
 #pragma unroll
   for (int q = 0; q < V; ++q) {
     const int l = q * BS + threadIdx.x;
     nf[q] = (l < N) ? fc[l] : 0.0;
-    ne[q] = (l < N && !new_eq && !fmc) ? ev[l] : 0.0;
+    ne[q] = (l < N && !fuse && !fmc) ? ev[l] : 0.0;
   }
   for (int base = 0; base < N; base += CH) {
     const int l0 = base + (int)threadIdx.x * V;
     double fv[V], e[V], d[V];
-    if (new_eq) {
+    if (fuse) {
 #pragma unroll
       for (int q = 0; q < V; ++q) tile[tile_pos<V>(q * BS + threadIdx.x)] = nf[q];
     } else {
@@ -555,38 +610,16 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
       const int l = base + CH + q * BS + threadIdx.x;
       if (l < N) {
         nf[q] = fc[l];
-        ne[q] = (new_eq || fmc) ? 0.0 : ev[l];
+        ne[q] = (fuse || fmc) ? 0.0 : ev[l];
       }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < V; ++q) fv[q] = tile[own + q];
-    if (new_eq) {
-      // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv.
-      // f_eq needs no loads: each lane evaluates its own V consecutive nodes,
-      // stepping (i, j, k) along the node order
-      int i, j, k;
-      node_ijk(g, l0 < N ? l0 : 0, i, j, k);
+    if (fuse) {
+      // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
       double pv[V];
-#pragma unroll
-      for (int q = 0; q < V; ++q) {
-        double val = 0.0;
-        if (l0 + q < N) {
-          const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-          const double beta3 = g.beta[0][i] * g.beta[1][j] * g.beta[2][k];
-          double phi_eq = 0.0;
-          if (temp > 0.0) {
-            double cx = (x - u); cx = cx * cx;
-            double cy = (y - v); cy = cy * cy;
-            double cz = (z - w); cz = cz * cz;
-            const double csq = cx + cy + cz;
-            phi_eq = coeff * exp(-csq * mass / temp);
-          }
-          val = phi_eq * beta3;
-        }
-        pv[q] = val;
-        if (++i == nx) { i = 0; if (++j == ny) { j = 0; ++k; } }
-      }
+      eq_values<V>(g, l0, coeff, u, v, w, temp, mass, pv);
       // eq_df of the node before this lane's first: the previous lane's last
       double before, last;
       cpv = cpv + chunk_scan_prev<BS, V>(pv, sh.red, cpv, prev_last, before, last);
@@ -616,7 +649,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     }
     __syncthreads();  // the tile's reads end before the next chunk writes it
   }
-  if (new_eq) {
+  if (fuse) {
     if (gsum) build_buckets_grouped<BS, V>(ec, gsum, N, tab2, rights);
     else build_buckets<BS>(ec, N, tab2, rights);
     if (threadIdx.x == 0) {
@@ -677,7 +710,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   }
 }
 
-template <int BS, int V>
+template <int BS, int V, bool kFused>
 __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* __restrict__ f,
                                                 const double* __restrict__ props, double* __restrict__ eqv,
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
@@ -689,7 +722,7 @@ __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* 
   const int cell = blockIdx.x;
   const int N = g.N;
   float* gsum = (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr;
-  cdf_cell<BS, V>(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
+  cdf_cell<BS, V, kFused>(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
            cdf + (size_t)cell * 2 * (N + 1), tabs + (size_t)cell * 2 * kNBucket, eqpar + (size_t)cell * 5,
            counts + cell * 4, factors + cell * 3, sh, gsum);
 }

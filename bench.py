@@ -505,6 +505,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true", help="also run a 512 MiB device copy (profiling)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-batched-cavity", action="store_true", help="skip the cavity as a batch slot")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-shock", action="store_true")
     ap.add_argument("--no-fmc", action="store_true")
@@ -577,7 +578,7 @@ def main():
             output_moment_error(finals["fp32"]["ext"], finals["fp64"]["ext"], finals["fp32"]["f"], finals["fp64"]["f"]))
         out["fp32_moment_error_after_steps"] = args.warmup + args.steps
     finals.clear()
-    if not args.no_batched:
+    if not args.no_batched and not args.no_batched_cavity:
         # the batched form of the cavity (one graph replay per step), both
         # precisions, against the plain FP64 context path above
         bc = {}
