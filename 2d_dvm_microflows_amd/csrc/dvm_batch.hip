@@ -138,7 +138,7 @@ __global__ void k_prep_cells(const SlotDesc* __restrict__ slots, const SlotState
 // turn; every workgroup reaches the exit once it has found all kXcds queues
 // drained. A cell's items sit in one queue, so the workgroups that share its
 // CDFs share one L2.
-template <bool kF32, int BS, int K>
+template <bool kF32, int BS, int K, bool kUni = false>
 __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict__ slots,
                                                       const SlotState* __restrict__ state,
                                                       const WorkItem* __restrict__ items, CollParams base,
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
                                   P.tabs + gc * 2 * kNBucket, N);
     Window win{};
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
-    const int bad = collide_item<kF32, BS, K>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
+    const int bad = collide_item<kF32, BS, K, kUni>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
                                               P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed});
     if (bad) atomicOr(P.err, bad);
@@ -314,6 +314,7 @@ struct RangeAlloc {
 
 struct HostSlot {
   bool active = false;
+  bool uniform = false;          // equally spaced axes
   int N = 0, ncell = 0, dim = 0;
   long long nodes = 0, cdf = 0;  // pool lengths held
   long long samples = 0;
@@ -361,6 +362,7 @@ struct dvm_batch {
   // launch state
   int n_cells = 0, n_cells1d = 0, n_active = 0, n_active0 = 0, n_items = 0, max_n = 0, max_n1d = 0;
   bool any_global = false;
+  bool all_uniform = false;  // every resident slot's axes equally spaced
   size_t coll_lds = 0;
   int persist = 0;
   int bs = 1024, k = 1, item_samples = 0;  // 0: sized from the resident load
@@ -394,17 +396,21 @@ static int balloc(dvm_batch* b, T** p, size_t n) {
   return DVM_OK;
 }
 
+// uni: FP32 with every resident slot's axes equally spaced (the remap then
+// rounds to the nearest node, as a context with those axes does).
 template <bool kF32, int BS>
-static const void* batch_kernel_k(int k) {
+static const void* batch_kernel_k(int k, bool uni) {
+  if (kF32 && uni)
+    return k == 2 ? (const void*)k_collide_slots<kF32, BS, 2, true> : (const void*)k_collide_slots<kF32, BS, 1, true>;
   return k == 2 ? (const void*)k_collide_slots<kF32, BS, 2> : (const void*)k_collide_slots<kF32, BS, 1>;
 }
 
-static const void* batch_collide_kernel(bool f32, int bs, int k) {
+static const void* batch_collide_kernel(bool f32, int bs, int k, bool uni) {
   switch (bs) {
-    case 256: return f32 ? batch_kernel_k<true, 256>(k) : batch_kernel_k<false, 256>(k);
-    case 512: return f32 ? batch_kernel_k<true, 512>(k) : batch_kernel_k<false, 512>(k);
-    case 768: return f32 ? batch_kernel_k<true, 768>(k) : batch_kernel_k<false, 768>(k);
-    default: return f32 ? batch_kernel_k<true, 1024>(k) : batch_kernel_k<false, 1024>(k);
+    case 256: return f32 ? batch_kernel_k<true, 256>(k, uni) : batch_kernel_k<false, 256>(k, uni);
+    case 512: return f32 ? batch_kernel_k<true, 512>(k, uni) : batch_kernel_k<false, 512>(k, uni);
+    case 768: return f32 ? batch_kernel_k<true, 768>(k, uni) : batch_kernel_k<false, 768>(k, uni);
+    default: return f32 ? batch_kernel_k<true, 1024>(k, uni) : batch_kernel_k<false, 1024>(k, uni);
   }
 }
 
@@ -475,8 +481,10 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
   b->cdfs.init(2 * (node_cap + cell_cap));
   const int lds_max = (int)(kLdsBytes - kStaticLds);
   for (int bs : {256, 512, 768, 1024})
-    for (int k : {1, 2}) BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k),
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    for (int k : {1, 2})
+      for (bool uni : {false, true})
+        BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k, uni), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    lds_max));
   return DVM_OK;
 }
 
@@ -643,6 +651,7 @@ extern "C" int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const doubl
   HostSlot& h = b->hs[slot];
   h = fresh;
   h.active = true;
+  h.uniform = hg.uniform;
   h.N = N;
   h.ncell = nc;
   h.dim = inst->dim;
@@ -704,9 +713,11 @@ static int batch_lists(dvm_batch* b) {
     if (step > (1LL << 18)) step = 1LL << 18;
   }
   b->item_used = step;
+  b->all_uniform = true;
   for (int s = 0; s < b->max_slots; ++s) {
     const HostSlot& h = b->hs[s];
     if (!h.active) continue;
+    b->all_uniform = b->all_uniform && h.uniform;
     SlotDesc& d = b->slots[s];
     act.push_back(s);
     if (h.dim == DVM_DIM_0D) act0.push_back(s);
@@ -780,7 +791,7 @@ static int batch_lists(dvm_batch* b) {
   b->n_active0 = (int)act0.size();
   b->n_items = (int)wi.size();
   b->coll_lds = (size_t)(kAxisSlots + lds_n) * sizeof(double);
-  const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k);
+  const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k, b->all_uniform);
   int per_cu = 0;
   BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, b->bs, b->coll_lds));
   if (per_cu < 1) per_cu = 1;
@@ -816,7 +827,7 @@ static int batch_enqueue_step(dvm_batch* b) {
     CollParams cp = b->collp;
     BatchPools P = b->P;
     void* args[] = {&sl, &st, &it, &cp, &P};
-    BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
+    BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k, b->all_uniform), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
                             b->stream));
   }
   if (f32)

@@ -42,6 +42,7 @@ struct dvm_ctx {
   CollParams collp{};
   ConvParams convp{};
   bool lds_collide = false;      // the FP64 slice fits LDS (window = grid)
+  bool uniform_axes = false;     // equally spaced axes (the FP32 remap rounds to the nearest node)
   WinDims wd{};                  // LDS window extents
   size_t coll_lds = 0;
   int coll_bs = 1024;            // lanes per collision workgroup (256, 512, 768 or 1024)
@@ -217,8 +218,13 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   delete c;
 }
 
+// uni: the FP32 path on equally spaced axes, whose remap takes the nearest
+// node by rounding (the FP64 path always searches, as the reference).
 template <int BS, int K>
-static const void* collide_kernel_bsk(bool lds, bool f32) {
+static const void* collide_kernel_bsk(bool lds, bool f32, bool uni) {
+  if (f32 && uni)
+    return lds ? (const void*)k_collide<true, true, BS, K, false, true>
+               : (const void*)k_collide<false, true, BS, K, false, true>;
   if (!lds)
     return f32 ? (const void*)k_collide<false, true, BS, K, false> : (const void*)k_collide<false, false, BS, K, false>;
   return f32 ? (const void*)k_collide<true, true, BS, K, false> : (const void*)k_collide<true, false, BS, K, false>;
@@ -239,21 +245,22 @@ static const void* collide_stream_kernel(const dvm_ctx* c) {
 static int stream_per_sample(const dvm_ctx* c) { return 2 + 2 * c->cfg.num_repl; }
 
 template <int BS>
-static const void* collide_kernel_bs(int k, bool lds, bool f32) {
+static const void* collide_kernel_bs(int k, bool lds, bool f32, bool uni) {
   switch (k) {
-    case 1: return collide_kernel_bsk<BS, 1>(lds, f32);
-    case 4: return collide_kernel_bsk<BS, 4>(lds, f32);
-    default: return collide_kernel_bsk<BS, 2>(lds, f32);
+    case 1: return collide_kernel_bsk<BS, 1>(lds, f32, uni);
+    case 4: return collide_kernel_bsk<BS, 4>(lds, f32, uni);
+    default: return collide_kernel_bsk<BS, 2>(lds, f32, uni);
   }
 }
 
 static const void* collide_kernel(const dvm_ctx* c) {
   const bool f32 = c->cfg.precision == DVM_PREC_FP32;
+  const bool uni = c->uniform_axes;
   switch (c->coll_bs) {
-    case 256: return collide_kernel_bs<256>(c->pick_k, c->lds_collide, f32);
-    case 768: return collide_kernel_bs<768>(c->pick_k, c->lds_collide, f32);
-    case 1024: return collide_kernel_bs<1024>(c->pick_k, c->lds_collide, f32);
-    default: return collide_kernel_bs<512>(c->pick_k, c->lds_collide, f32);
+    case 256: return collide_kernel_bs<256>(c->pick_k, c->lds_collide, f32, uni);
+    case 768: return collide_kernel_bs<768>(c->pick_k, c->lds_collide, f32, uni);
+    case 1024: return collide_kernel_bs<1024>(c->pick_k, c->lds_collide, f32, uni);
+    default: return collide_kernel_bs<512>(c->pick_k, c->lds_collide, f32, uni);
   }
 }
 
@@ -292,6 +299,7 @@ struct HostGrid {
   std::vector<double> ax[3], beta[3], packed;
   Grid g{};
   double b3min = 0.0;
+  bool uniform = false;  // every axis equally spaced (to 1e-9 of its spacing)
 };
 
 static void build_host_grid(const dvm_config* cfg, HostGrid& h) {
@@ -326,6 +334,13 @@ static void build_host_grid(const dvm_config* cfg, HostGrid& h) {
         if (b3 < b3min) b3min = b3;
       }
   h.b3min = b3min;
+  h.uniform = true;
+  for (int d = 0; d < 3; ++d) {
+    const auto& a = h.ax[d];
+    const double sp = (a[n[d] - 1] - a[0]) / (double)(n[d] - 1);
+    for (int i = 0; i < n[d]; ++i)
+      if (!(fabs(a[i] - (a[0] + i * sp)) <= 1e-9 * fabs(sp))) h.uniform = false;
+  }
 }
 
 // Point the grid at its packed image at device address p.
@@ -458,6 +473,7 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   build_host_grid(cfg, hg);
   for (int d = 0; d < 3; ++d) { c->ax[d] = hg.ax[d]; c->beta[d] = hg.beta[d]; }
   c->g = hg.g;
+  c->uniform_axes = hg.uniform;
   if (int rc = dalloc(c, &c->d_axes, hg.packed.size())) return rc;
   HIP_TRY(c, hipMemcpy(c->d_axes, hg.packed.data(), hg.packed.size() * sizeof(double), hipMemcpyHostToDevice));
   place_grid(c->g, c->d_axes);

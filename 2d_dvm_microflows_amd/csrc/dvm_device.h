@@ -217,19 +217,31 @@ struct StencilT {
 };
 using Stencil = StencilT<double>;
 
+// Nearest node of axis d. kUniform (the FP32 path on equally spaced axes): the
+// rounded position (v - vmin) (n - 1) / (vmax - vmin), clamped to the axis --
+// the nearest node up to FP32 rounding at a midpoint, without the bracket
+// search; otherwise the reference's search (nearest_node).
+template <class R, bool kUniform>
+__device__ __forceinline__ int nearest_on_axis(const Axes<R>& g, int d, R val) {
+  const int mx = g.n[d] - 1;
+  const int guess = first_guess<R>(g.factor[d], val, g.vmin[d], mx);
+  if constexpr (kUniform) return guess < 0 ? 0 : (guess > mx ? mx : guess);
+  else return nearest_node<R>(g.a[d], mx, val, guess, g.vmin[d], g.vmax[d]);
+}
+
 // compute_remapping_quantities (remapping.f90:92-394), the non-uniform branch
 // that trunk always takes. Moments 0, 1 and 2 of the 7-point stencil equal
 // f_off at (u,v,w) in exact arithmetic. flag == 0 is the reference's fatal case.
-template <class R>
+template <class R, bool kUniform = false>
 __device__ __forceinline__ void remap_t(const Axes<R>& g, R u, R v, R w, R f_off, StencilT<R>& s) {
   const R *X = g.a[0], *Y = g.a[1], *Z = g.a[2];
   const int imax = g.n[0] - 1, jmax = g.n[1] - 1, kmax = g.n[2] - 1;
   int corner = 0;
-  int oi = nearest_node<R>(X, imax, u, first_guess<R>(g.factor[0], u, g.vmin[0], imax), g.vmin[0], g.vmax[0]);
+  int oi = nearest_on_axis<R, kUniform>(g, 0, u);
   if (oi >= imax) { oi = imax; ++corner; } else if (oi <= 0) { oi = 0; ++corner; }
-  int oj = nearest_node<R>(Y, jmax, v, first_guess<R>(g.factor[1], v, g.vmin[1], jmax), g.vmin[1], g.vmax[1]);
+  int oj = nearest_on_axis<R, kUniform>(g, 1, v);
   if (oj >= jmax) { oj = jmax; ++corner; } else if (oj <= 0) { oj = 0; ++corner; }
-  int ok = nearest_node<R>(Z, kmax, w, first_guess<R>(g.factor[2], w, g.vmin[2], kmax), g.vmin[2], g.vmax[2]);
+  int ok = nearest_on_axis<R, kUniform>(g, 2, w);
   if (ok >= kmax) { ok = kmax; ++corner; } else if (ok <= 0) { ok = 0; ++corner; }
   if (corner == 3) {
     oi -= isign1(oi - 1);

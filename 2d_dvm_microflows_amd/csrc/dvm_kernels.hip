@@ -880,7 +880,7 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
 // The sign of the deviation at each deviation partner (+1 for an equilibrium
 // partner) is read from the cell's f and f_eq at the start and first used
 // after the velocity arithmetic, which covers the loads' latency.
-template <bool kRecord, class R, class Src, class Scatter>
+template <bool kRecord, class R, bool kUni, class Src, class Scatter>
 __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const CollParams& cp, const CellView& cv,
                                          int cell, int ntime, int q, int t, int s, double factor, int lA, int lB,
                                          uint32_t wphi, uint32_t wth, const Src& src, Scatter&& scatter,
@@ -968,8 +968,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const
     // (remapping.f90:55-394 via apply_phi_replenishment :356-434)
     for (int side = 0; side < 2; ++side) {
       StencilT<R> st;
-      if (side == 0) remap_t<R>(ax, vc0 + gr1 * rx, vc1 + gr1 * ry, vc2 + gr1 * rz, (R)1, st);
-      else remap_t<R>(ax, vc0 + -(gr2 * rx), vc1 + -(gr2 * ry), vc2 + -(gr2 * rz), (R)1, st);
+      if (side == 0) remap_t<R, kUni>(ax, vc0 + gr1 * rx, vc1 + gr1 * ry, vc2 + gr1 * rz, (R)1, st);
+      else remap_t<R, kUni>(ax, vc0 + -(gr2 * rx), vc1 + -(gr2 * ry), vc2 + -(gr2 * rz), (R)1, st);
       if (kRecord && r == 1) {
         int* o = side == 0 ? rec->st1 : rec->st2;
         double* m = side == 0 ? rec->m1 : rec->m2;
@@ -1069,7 +1069,7 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
 //         the repeated addresses of a cell's distribution peak.)
 // Samples q in [q0, q1) of one cell (q indexes integral 1, then 2, then 3);
 // every depletion and stencil point goes to scatter(node, value).
-template <bool kF32, int BS, int K, class Src, class Scatter>
+template <bool kF32, int BS, int K, bool kUni, class Src, class Scatter>
 __device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typename std::conditional<kF32, float, double>::type>& ax,
                                                const CollParams& cp, const CellView& cv,
                                                int rng_cell, int ntime, int n0, int n1, int q0, int q1,
@@ -1093,7 +1093,7 @@ __device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typenam
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
       if (!valid[k]) continue;
-      bad |= vr_sample<false, R>(g, ax, cp, cv, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k],
+      bad |= vr_sample<false, R, kF32 && kUni>(g, ax, cp, cv, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k],
                                  wphi[k], wth[k], src, scatter, nullptr);
     }
   }
@@ -1243,7 +1243,7 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
 //   FP32, full and sole: the whole change is in LDS, so the FP64
 //         conservation correction runs here (csh its reduction space) and
 //         fd = f + corrected change is written directly.
-template <bool kF32, int BS, int K, class Src>
+template <bool kF32, int BS, int K, bool kUni, class Src>
 __device__ __forceinline__ int collide_item(const Grid& g, const Axes<typename std::conditional<kF32, float, double>::type>& ax,
                                             const CollParams& cp, const CellView& cv, int rng_cell,
                                             int ntime, int q0, int q1, const int* __restrict__ cnt,
@@ -1259,14 +1259,14 @@ __device__ __forceinline__ int collide_item(const Grid& g, const Axes<typename s
   double* tgt = kF32 ? da : fd;
   int bad;
   if (full) {
-    bad = collide_samples<kF32, BS, K>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
+    bad = collide_samples<kF32, BS, K, kUni>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int, int, int, int node, double v) {
                                          if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], v);
                                        });
   } else {
     const int o0 = win.o[0], o1 = win.o[1], o2 = win.o[2], wx = win.w[0], wxy = win.wxy;
     const unsigned w0 = win.w[0], w1 = win.w[1], w2 = win.w[2];
-    bad = collide_samples<kF32, BS, K>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
+    bad = collide_samples<kF32, BS, K, kUni>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int i, int j, int k, int node, double v) {
                                          const unsigned a = i - o0, b = j - o1, c = k - o2;
                                          if (a < w0 && b < w1 && c < w2) atomicAdd(&acc[a + wx * b + wxy * c], v);
@@ -1309,7 +1309,7 @@ struct WinDims {
 
 // kStream: the uniforms come from the explicit list su (per-cell offsets sbase,
 // sper values per sample) instead of Philox.
-template <bool kLds, bool kF32, int BS, int K, bool kStream>
+template <bool kLds, bool kF32, int BS, int K, bool kStream, bool kUni = false>
 __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, int split, WinDims wd,
                                                 const double* __restrict__ props,
                                                 const double* __restrict__ f, double* __restrict__ fd,
@@ -1344,7 +1344,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
   Src src;
   if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
-  const int bad = collide_item<kF32, BS, K>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
+  const int bad = collide_item<kF32, BS, K, kUni>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src);
   if (bad) atomicOr(err, bad);
@@ -1405,7 +1405,7 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     const int qi = q;
     pick_batch<1>(cp, cv, N, cell, ntime, &qi, &t, &s, &valid, &lA, &lB, &wphi, &wth, src);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
-    vr_sample<true, double>(g, grid_axes(g), cp, cv, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, wphi, wth, src,
+    vr_sample<true, double, false>(g, grid_axes(g), cp, cv, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, wphi, wth, src,
                             [](int, int, int, int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;
