@@ -15,7 +15,7 @@ DVM_COLL_NONE, DVM_COLL_FMC, DVM_COLL_VR = 0, 2, 3
 DVM_BC_WALL, DVM_BC_INOUT, DVM_BC_ZEROGRAD, DVM_BC_COUETTE = 1, 2, 3, 4
 DVM_CS_CLARKE, DVM_CS_BIRD, DVM_CS_VKI = 1, 2, 3
 DVM_DIM_0D, DVM_DIM_1D, DVM_DIM_2D = 0, 1, 2
-DVM_PREC_FP64, DVM_PREC_FP32 = 0, 1
+DVM_PREC_FP64, DVM_PREC_FP32, DVM_PREC_FP32_REMAP32 = 0, 1, 2
 DVM_SAMPLES_REFERENCE, DVM_SAMPLES_EXPLICIT = 0, 1
 DVM_CONV_UPWIND1, DVM_CONV_MINMOD2 = 1, 2
 DVM_NPROP = 8

@@ -138,7 +138,7 @@ __global__ void k_prep_cells(const SlotDesc* __restrict__ slots, const SlotState
 // turn; every workgroup reaches the exit once it has found all kXcds queues
 // drained. A cell's items sit in one queue, so the workgroups that share its
 // CDFs share one L2.
-template <bool kF32, int BS, int K, bool kUni = false>
+template <bool kF32, int BS, int K, bool kUni = false, bool kRemap32 = false>
 __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict__ slots,
                                                       const SlotState* __restrict__ state,
                                                       const WorkItem* __restrict__ items, CollParams base,
@@ -167,8 +167,8 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
     const SlotState st = state[w.slot];
     const size_t gc = (size_t)d.cell0 + w.cell;
     const size_t n0 = cell_node0(d, w.cell);
-    using R = typename std::conditional<kF32, float, double>::type;
-    const Axes<R> ax = stage_collision_axes<R>(g, dyn);
+    using RM = typename std::conditional<kF32 && kRemap32, float, double>::type;
+    const Axes<RM> ax = stage_collision_axes<RM>(g, dyn);
     CollParams cp = base;
     cp.seed = d.seed;
     const int ntime = st.ntime + 1;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
                                   P.tabs + gc * 2 * kNBucket, N);
     Window win{};
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
-    const int bad = collide_item<kF32, BS, K, kUni>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
+    const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
                                               P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed});
     if (bad) atomicOr(P.err, bad);
@@ -397,24 +397,30 @@ static int balloc(dvm_batch* b, T** p, size_t n) {
 }
 
 // uni: FP32 with every resident slot's axes equally spaced (the remap then
-// rounds to the nearest node, as a context with those axes does).
+// rounds to the nearest node, as a context with those axes does); r32: with
+// uni, the remap in FP32 as well (DVM_PREC_FP32_REMAP32).
 template <bool kF32, int BS>
-static const void* batch_kernel_k(int k, bool uni) {
+static const void* batch_kernel_k(int k, bool uni, bool r32) {
+  if (kF32 && uni && r32)
+    return k == 2 ? (const void*)k_collide_slots<kF32, BS, 2, true, true>
+                  : (const void*)k_collide_slots<kF32, BS, 1, true, true>;
   if (kF32 && uni)
     return k == 2 ? (const void*)k_collide_slots<kF32, BS, 2, true> : (const void*)k_collide_slots<kF32, BS, 1, true>;
   return k == 2 ? (const void*)k_collide_slots<kF32, BS, 2> : (const void*)k_collide_slots<kF32, BS, 1>;
 }
 
-static const void* batch_collide_kernel(bool f32, int bs, int k, bool uni) {
+static const void* batch_collide_kernel(bool f32, int bs, int k, bool uni, bool r32) {
   switch (bs) {
-    case 256: return f32 ? batch_kernel_k<true, 256>(k, uni) : batch_kernel_k<false, 256>(k, uni);
-    case 512: return f32 ? batch_kernel_k<true, 512>(k, uni) : batch_kernel_k<false, 512>(k, uni);
-    case 768: return f32 ? batch_kernel_k<true, 768>(k, uni) : batch_kernel_k<false, 768>(k, uni);
-    default: return f32 ? batch_kernel_k<true, 1024>(k, uni) : batch_kernel_k<false, 1024>(k, uni);
+    case 256: return f32 ? batch_kernel_k<true, 256>(k, uni, r32) : batch_kernel_k<false, 256>(k, uni, r32);
+    case 512: return f32 ? batch_kernel_k<true, 512>(k, uni, r32) : batch_kernel_k<false, 512>(k, uni, r32);
+    case 768: return f32 ? batch_kernel_k<true, 768>(k, uni, r32) : batch_kernel_k<false, 768>(k, uni, r32);
+    default: return f32 ? batch_kernel_k<true, 1024>(k, uni, r32) : batch_kernel_k<false, 1024>(k, uni, r32);
   }
 }
 
-static bool batch_f32(const dvm_batch* b) { return b->species.precision == DVM_PREC_FP32; }
+static bool batch_r32(const dvm_batch* b) { return b->species.precision == DVM_PREC_FP32_REMAP32; }
+
+static bool batch_f32(const dvm_batch* b) { return prec_f32(b->species.precision); }
 
 extern "C" const char* dvm_batch_last_error(const dvm_batch* b) { return b ? b->msg : g_batch_msg; }
 
@@ -483,8 +489,8 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
   for (int bs : {256, 512, 768, 1024})
     for (int k : {1, 2})
       for (bool uni : {false, true})
-        BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k, uni), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    lds_max));
+        BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k, uni, batch_r32(b)),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   return DVM_OK;
 }
 
@@ -499,8 +505,8 @@ extern "C" int dvm_batch_create(const dvm_config* species, int32_t max_slots, in
   if (species->num_repl < 1) return bfail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
   if (species->coll_solver != DVM_COLL_VR && species->coll_solver != DVM_COLL_FMC)
     return bfail(nullptr, DVM_E_ARG, "batched solves use the variance-reduction or full Monte Carlo collision solver");
-  if (species->precision != DVM_PREC_FP64 && species->precision != DVM_PREC_FP32)
-    return bfail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
+  if (species->precision != DVM_PREC_FP64 && !prec_f32(species->precision))
+    return bfail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64, DVM_PREC_FP32 or DVM_PREC_FP32_REMAP32");
   if (species->sample_mode != DVM_SAMPLES_REFERENCE && species->sample_mode != DVM_SAMPLES_EXPLICIT)
     return bfail(nullptr, DVM_E_ARG, "invalid sample_mode");
   int ndev = 0;
@@ -523,7 +529,7 @@ extern "C" int dvm_batch_footprint(const dvm_config* inst, int32_t precision, in
   instance_extent(inst, nn, nc, cdf);
   if (nodes) *nodes = nn;
   if (bytes) {
-    *bytes = nn * bytes_per_node(precision == DVM_PREC_FP32) + (long long)nc * (16 * kNBucket + 8 * 20 + 16);
+    *bytes = nn * bytes_per_node(prec_f32(precision)) + (long long)nc * (16 * kNBucket + 8 * 20 + 16);
     if (inst->dim == DVM_DIM_2D && inst->enable_convection) {
       // the slot's wall buffers
       const long long N = (long long)inst->nvx * inst->nvy * inst->nvz;
@@ -791,7 +797,7 @@ static int batch_lists(dvm_batch* b) {
   b->n_active0 = (int)act0.size();
   b->n_items = (int)wi.size();
   b->coll_lds = (size_t)(kAxisSlots + lds_n) * sizeof(double);
-  const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k, b->all_uniform);
+  const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k, b->all_uniform, batch_r32(b));
   int per_cu = 0;
   BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, b->bs, b->coll_lds));
   if (per_cu < 1) per_cu = 1;
@@ -827,7 +833,7 @@ static int batch_enqueue_step(dvm_batch* b) {
     CollParams cp = b->collp;
     BatchPools P = b->P;
     void* args[] = {&sl, &st, &it, &cp, &P};
-    BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k, b->all_uniform), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
+    BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k, b->all_uniform, batch_r32(b)), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
                             b->stream));
   }
   if (f32)

@@ -182,8 +182,8 @@ static int validate(const dvm_config* cfg) {
   if (cfg->num_repl < 1) return fail(nullptr, DVM_E_ARG, "num_repl must be >= 1");
   if (cfg->coll_solver != DVM_COLL_NONE && cfg->coll_solver != DVM_COLL_FMC && cfg->coll_solver != DVM_COLL_VR)
     return fail(nullptr, DVM_E_ARG, "coll_solver must be 0 (none), 2 (full Monte Carlo) or 3 (variance reduction)");
-  if (cfg->precision != DVM_PREC_FP64 && cfg->precision != DVM_PREC_FP32)
-    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64 or DVM_PREC_FP32");
+  if (cfg->precision != DVM_PREC_FP64 && cfg->precision != DVM_PREC_FP32 && cfg->precision != DVM_PREC_FP32_REMAP32)
+    return fail(nullptr, DVM_E_ARG, "precision must be DVM_PREC_FP64, DVM_PREC_FP32 or DVM_PREC_FP32_REMAP32");
   if (cfg->sample_mode != DVM_SAMPLES_REFERENCE && cfg->sample_mode != DVM_SAMPLES_EXPLICIT)
     return fail(nullptr, DVM_E_ARG, "invalid sample_mode");
   if (cfg->sample_mode == DVM_SAMPLES_REFERENCE && !(cfg->coln_rms > 0.0))
@@ -219,9 +219,13 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
 }
 
 // uni: the FP32 path on equally spaced axes, whose remap takes the nearest
-// node by rounding (the FP64 path always searches, as the reference).
+// node by rounding (the FP64 path always searches, as the reference); r32:
+// with uni, the remap in FP32 as well (DVM_PREC_FP32_REMAP32).
 template <int BS, int K>
-static const void* collide_kernel_bsk(bool lds, bool f32, bool uni) {
+static const void* collide_kernel_bsk(bool lds, bool f32, bool uni, bool r32) {
+  if (f32 && uni && r32)
+    return lds ? (const void*)k_collide<true, true, BS, K, false, true, true>
+               : (const void*)k_collide<false, true, BS, K, false, true, true>;
   if (f32 && uni)
     return lds ? (const void*)k_collide<true, true, BS, K, false, true>
                : (const void*)k_collide<false, true, BS, K, false, true>;
@@ -233,7 +237,7 @@ static const void* collide_kernel_bsk(bool lds, bool f32, bool uni) {
 // The collision kernel of the uniform-stream parity mode (kCollBS lanes, one
 // pick per lane).
 static const void* collide_stream_kernel(const dvm_ctx* c) {
-  const bool f32 = c->cfg.precision == DVM_PREC_FP32;
+  const bool f32 = prec_f32(c->cfg.precision);
   if (!c->lds_collide)
     return f32 ? (const void*)k_collide<false, true, kCollBS, 1, true>
                : (const void*)k_collide<false, false, kCollBS, 1, true>;
@@ -245,22 +249,23 @@ static const void* collide_stream_kernel(const dvm_ctx* c) {
 static int stream_per_sample(const dvm_ctx* c) { return 2 + 2 * c->cfg.num_repl; }
 
 template <int BS>
-static const void* collide_kernel_bs(int k, bool lds, bool f32, bool uni) {
+static const void* collide_kernel_bs(int k, bool lds, bool f32, bool uni, bool r32) {
   switch (k) {
-    case 1: return collide_kernel_bsk<BS, 1>(lds, f32, uni);
-    case 4: return collide_kernel_bsk<BS, 4>(lds, f32, uni);
-    default: return collide_kernel_bsk<BS, 2>(lds, f32, uni);
+    case 1: return collide_kernel_bsk<BS, 1>(lds, f32, uni, r32);
+    case 4: return collide_kernel_bsk<BS, 4>(lds, f32, uni, r32);
+    default: return collide_kernel_bsk<BS, 2>(lds, f32, uni, r32);
   }
 }
 
 static const void* collide_kernel(const dvm_ctx* c) {
-  const bool f32 = c->cfg.precision == DVM_PREC_FP32;
+  const bool f32 = prec_f32(c->cfg.precision);
   const bool uni = c->uniform_axes;
+  const bool r32 = c->cfg.precision == DVM_PREC_FP32_REMAP32;
   switch (c->coll_bs) {
-    case 256: return collide_kernel_bs<256>(c->pick_k, c->lds_collide, f32, uni);
-    case 768: return collide_kernel_bs<768>(c->pick_k, c->lds_collide, f32, uni);
-    case 1024: return collide_kernel_bs<1024>(c->pick_k, c->lds_collide, f32, uni);
-    default: return collide_kernel_bs<512>(c->pick_k, c->lds_collide, f32, uni);
+    case 256: return collide_kernel_bs<256>(c->pick_k, c->lds_collide, f32, uni, r32);
+    case 768: return collide_kernel_bs<768>(c->pick_k, c->lds_collide, f32, uni, r32);
+    case 1024: return collide_kernel_bs<1024>(c->pick_k, c->lds_collide, f32, uni, r32);
+    default: return collide_kernel_bs<512>(c->pick_k, c->lds_collide, f32, uni, r32);
   }
 }
 
@@ -497,7 +502,7 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   // dsgn(f - 0) = dsgn(f)
   HIP_TRY(c, hipMemset(c->eqv, 0, CN * sizeof(double)));
   // collision accumulator placement: the cell's slice in LDS when it fits
-  const bool f32 = cfg->precision == DVM_PREC_FP32;
+  const bool f32 = prec_f32(cfg->precision);
   c->lds_collide = window_dims(n, c->wd.w);
   // FP64 with an LDS window (the grid exceeds LDS): 768 lanes, where the FP64
   // sampling code's registers spill less than at the 128-register cap of 1024
@@ -754,7 +759,7 @@ static int cdf_async(dvm_ctx* c) {
 // LDS window to global atomics.
 static int prep_async(dvm_ctx* c) {
   if (!collides(c) || (c->lds_collide && c->split == 1)) return DVM_OK;
-  if (c->cfg.precision == DVM_PREC_FP32)
+  if (prec_f32(c->cfg.precision))
     hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->dacc, (long long)c->ncell * c->N, 0.0);
   else
     HIP_TRY(c, hipMemcpyAsync(c->fd, c->f, (size_t)c->ncell * c->N * sizeof(double), hipMemcpyDeviceToDevice,
@@ -766,7 +771,7 @@ static int prep_async(dvm_ctx* c) {
 // The FP32 path corrects conservation inside the collide kernel when the
 // cell's slice fits LDS and one workgroup takes the cell, else in k_conserve.
 static bool separate_conserve(const dvm_ctx* c) {
-  return c->cfg.precision == DVM_PREC_FP32 && (!c->lds_collide || c->split > 1);
+  return prec_f32(c->cfg.precision) && (!c->lds_collide || c->split > 1);
 }
 
 static int collide_async(dvm_ctx* c, int ntime) {

@@ -7,6 +7,10 @@
 
 namespace dvm {
 
+// The FP32-collision precisions (the remap in FP64 or, on equally spaced
+// axes, in FP32).
+inline bool prec_f32(int p) { return p == DVM_PREC_FP32 || p == DVM_PREC_FP32_REMAP32; }
+
 constexpr int kMomBS = 256;      // moments / wall-flux workgroup
 constexpr int kMomCells = 4;     // cells per k_moments workgroup
 constexpr int kCdfBS = 256;      // CDF workgroup of the batch

@@ -873,15 +873,18 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
 // One VR sample: variance_reduction.f90:686-816 / 838-961 / 983-1116 with
 // vr_depletion_routine (:1127-1237) and the elastic branch of
 // replenish_collision (replenishing_collisions.f90:134-354).
-// R is the collision arithmetic (double, or float for the FP32-collision path,
-// which also remaps in FP32 on the FP32 axes ax); the partner picks and the
-// deviation signs are FP64 in both, and every value scattered is FP64.
+// R is the collision arithmetic (double, or float for the FP32-collision path);
+// the remap runs in RM (double, or float with kRemap32 on the FP32 axes ax),
+// with kUni its nearest nodes by rounding on equally spaced axes; the partner
+// picks and the deviation signs are FP64 always, and every value scattered is
+// FP64.
 // scatter(node, value) receives the two depletions and every stencil point.
 // The sign of the deviation at each deviation partner (+1 for an equilibrium
 // partner) is read from the cell's f and f_eq at the start and first used
 // after the velocity arithmetic, which covers the loads' latency.
-template <bool kRecord, class R, bool kUni, class Src, class Scatter>
-__device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const CollParams& cp, const CellView& cv,
+template <bool kRecord, class R, bool kUni, bool kRemap32, class Src, class Scatter,
+          class RM = typename std::conditional<kRemap32, float, double>::type>
+__device__ __forceinline__ int vr_sample(const Grid& g, const Axes<RM>& ax, const CollParams& cp, const CellView& cv,
                                          int cell, int ntime, int q, int t, int s, double factor, int lA, int lB,
                                          uint32_t wphi, uint32_t wth, const Src& src, Scatter&& scatter,
                                          SampleRec* rec) {
@@ -891,8 +894,8 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const
   int iA, jA, kA, iB, jB, kB;
   node_ijk(g, lA, iA, jA, kA);
   node_ijk(g, lB, iB, jB, kB);
-  const R x1 = ax.a[0][iA], y1 = ax.a[1][jA], z1 = ax.a[2][kA];
-  const R x2 = ax.a[0][iB], y2 = ax.a[1][jB], z2 = ax.a[2][kB];
+  const R x1 = (R)ax.a[0][iA], y1 = (R)ax.a[1][jA], z1 = (R)ax.a[2][kA];
+  const R x2 = (R)ax.a[0][iB], y2 = (R)ax.a[1][jB], z2 = (R)ax.a[2][kB];
   if (kRecord) { rec->lA = lA; rec->lB = lB; rec->skipped = 0; }
   const R gx = x2 - x1, gy = y2 - y1, gz = z2 - z1;
   const R gg = rsqrt_free_sqrt<R>(gx * gx + gy * gy + gz * gz);
@@ -964,12 +967,12 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const
       ry = gg * stt * sph;
       rz = gg * ct;
     }
-    // post-collision velocities, remapped in R
+    // post-collision velocities, remapped in RM
     // (remapping.f90:55-394 via apply_phi_replenishment :356-434)
     for (int side = 0; side < 2; ++side) {
-      StencilT<R> st;
-      if (side == 0) remap_t<R, kUni>(ax, vc0 + gr1 * rx, vc1 + gr1 * ry, vc2 + gr1 * rz, (R)1, st);
-      else remap_t<R, kUni>(ax, vc0 + -(gr2 * rx), vc1 + -(gr2 * ry), vc2 + -(gr2 * rz), (R)1, st);
+      StencilT<RM> st;
+      if (side == 0) remap_t<RM, kUni>(ax, (RM)(vc0 + gr1 * rx), (RM)(vc1 + gr1 * ry), (RM)(vc2 + gr1 * rz), (RM)1, st);
+      else remap_t<RM, kUni>(ax, (RM)(vc0 + -(gr2 * rx)), (RM)(vc1 + -(gr2 * ry)), (RM)(vc2 + -(gr2 * rz)), (RM)1, st);
       if (kRecord && r == 1) {
         int* o = side == 0 ? rec->st1 : rec->st2;
         double* m = side == 0 ? rec->m1 : rec->m2;
@@ -979,7 +982,7 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<R>& ax, const
       }
       if (st.flag == 0) { bad = 1; continue; }
       if (!kRecord) {
-        const R kr = side == 0 ? kr1 : kr2;
+        const RM kr = (RM)(side == 0 ? kr1 : kr2);
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
           if (q < 4 || (st.flag & (1 << (q - 4)))) {
@@ -1069,8 +1072,9 @@ __device__ bool solve5(double* M /* 5x5 row-major, overwritten */, double* b /* 
 //         the repeated addresses of a cell's distribution peak.)
 // Samples q in [q0, q1) of one cell (q indexes integral 1, then 2, then 3);
 // every depletion and stencil point goes to scatter(node, value).
-template <bool kF32, int BS, int K, bool kUni, class Src, class Scatter>
-__device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typename std::conditional<kF32, float, double>::type>& ax,
+template <bool kF32, int BS, int K, bool kUni, bool kRemap32, class Src, class Scatter>
+__device__ __forceinline__ int collide_samples(const Grid& g,
+                                               const Axes<typename std::conditional<kRemap32, float, double>::type>& ax,
                                                const CollParams& cp, const CellView& cv,
                                                int rng_cell, int ntime, int n0, int n1, int q0, int q1,
                                                const double* __restrict__ fac3, const Src& src, Scatter&& scatter) {
@@ -1093,7 +1097,7 @@ __device__ __forceinline__ int collide_samples(const Grid& g, const Axes<typenam
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
       if (!valid[k]) continue;
-      bad |= vr_sample<false, R, kF32 && kUni>(g, ax, cp, cv, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k],
+      bad |= vr_sample<false, R, kF32 && kUni, kRemap32>(g, ax, cp, cv, rng_cell, ntime, qq[k], tt[k], ss[k], fac3[tt[k]], lA[k], lB[k],
                                  wphi[k], wth[k], src, scatter, nullptr);
     }
   }
@@ -1243,8 +1247,9 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
 //   FP32, full and sole: the whole change is in LDS, so the FP64
 //         conservation correction runs here (csh its reduction space) and
 //         fd = f + corrected change is written directly.
-template <bool kF32, int BS, int K, bool kUni, class Src>
-__device__ __forceinline__ int collide_item(const Grid& g, const Axes<typename std::conditional<kF32, float, double>::type>& ax,
+template <bool kF32, int BS, int K, bool kUni, bool kRemap32, class Src>
+__device__ __forceinline__ int collide_item(const Grid& g,
+                                            const Axes<typename std::conditional<kRemap32, float, double>::type>& ax,
                                             const CollParams& cp, const CellView& cv, int rng_cell,
                                             int ntime, int q0, int q1, const int* __restrict__ cnt,
                                             const double* __restrict__ fac, const Window& win, bool full, bool sole,
@@ -1259,14 +1264,14 @@ __device__ __forceinline__ int collide_item(const Grid& g, const Axes<typename s
   double* tgt = kF32 ? da : fd;
   int bad;
   if (full) {
-    bad = collide_samples<kF32, BS, K, kUni>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
+    bad = collide_samples<kF32, BS, K, kUni, kRemap32>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int, int, int, int node, double v) {
                                          if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], v);
                                        });
   } else {
     const int o0 = win.o[0], o1 = win.o[1], o2 = win.o[2], wx = win.w[0], wxy = win.wxy;
     const unsigned w0 = win.w[0], w1 = win.w[1], w2 = win.w[2];
-    bad = collide_samples<kF32, BS, K, kUni>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
+    bad = collide_samples<kF32, BS, K, kUni, kRemap32>(g, ax, cp, cv, rng_cell, ntime, cnt[0], cnt[1], q0, q1, fac, src,
                                        [&](int i, int j, int k, int node, double v) {
                                          const unsigned a = i - o0, b = j - o1, c = k - o2;
                                          if (a < w0 && b < w1 && c < w2) atomicAdd(&acc[a + wx * b + wxy * c], v);
@@ -1309,7 +1314,9 @@ struct WinDims {
 
 // kStream: the uniforms come from the explicit list su (per-cell offsets sbase,
 // sper values per sample) instead of Philox.
-template <bool kLds, bool kF32, int BS, int K, bool kStream, bool kUni = false>
+// kUni / kRemap32 (FP32 path only): the remap's nearest nodes by rounding on
+// equally spaced axes / the remap in FP32 on FP32 axes.
+template <bool kLds, bool kF32, int BS, int K, bool kStream, bool kUni = false, bool kRemap32 = false>
 __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, int split, WinDims wd,
                                                 const double* __restrict__ props,
                                                 const double* __restrict__ f, double* __restrict__ fd,
@@ -1334,8 +1341,8 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   const int N = g.N;
   Window win{};
   if (!kLds) win = make_window(g, wd.w, props + (size_t)cell * DVM_NPROP);
-  using R = typename std::conditional<kF32, float, double>::type;
-  const Axes<R> ax = stage_collision_axes<R>(g, dyn);
+  using RM = typename std::conditional<kF32 && kRemap32, float, double>::type;
+  const Axes<RM> ax = stage_collision_axes<RM>(g, dyn);
   const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
                                 tabs + (size_t)cell * 2 * kNBucket, N);
   const int* cnt = counts + cell * 4;
@@ -1344,7 +1351,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
   Src src;
   if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
-  const int bad = collide_item<kF32, BS, K, kUni>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
+  const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src);
   if (bad) atomicOr(err, bad);
@@ -1405,7 +1412,7 @@ __global__ void __launch_bounds__(kCollBS) k_trace(Grid g, CollParams cp, int nt
     const int qi = q;
     pick_batch<1>(cp, cv, N, cell, ntime, &qi, &t, &s, &valid, &lA, &lB, &wphi, &wth, src);
     rec.lA = lA; rec.lB = lB; rec.skipped = 0;
-    vr_sample<true, double, false>(g, grid_axes(g), cp, cv, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, wphi, wth, src,
+    vr_sample<true, double, false, false>(g, grid_axes(g), cp, cv, cell, ntime, qi, t, s, factors[cell * 3 + t], lA, lB, wphi, wth, src,
                             [](int, int, int, int, double) {}, &rec);
     int* ir = irec + (size_t)q * 32;
     double* dr = drec + (size_t)q * 16;

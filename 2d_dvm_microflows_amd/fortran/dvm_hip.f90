@@ -14,7 +14,7 @@ module DVMHip
        DVM_BC_COUETTE = 4
   integer(c_int32_t), parameter, public :: DVM_CS_CLARKE = 1, DVM_CS_BIRD = 2, DVM_CS_VKI = 3
   integer(c_int32_t), parameter, public :: DVM_DIM_0D = 0, DVM_DIM_1D = 1, DVM_DIM_2D = 2
-  integer(c_int32_t), parameter, public :: DVM_PREC_FP64 = 0, DVM_PREC_FP32 = 1
+  integer(c_int32_t), parameter, public :: DVM_PREC_FP64 = 0, DVM_PREC_FP32 = 1, DVM_PREC_FP32_REMAP32 = 2
   integer(c_int32_t), parameter, public :: DVM_SAMPLES_REFERENCE = 0, DVM_SAMPLES_EXPLICIT = 1
   integer(c_int32_t), parameter, public :: DVM_CONV_UPWIND1 = 1, DVM_CONV_MINMOD2 = 2
   integer, parameter, public :: DVM_NPROP = 8, DVM_NPHASE = 7, DVM_NEXT = 10

@@ -57,6 +57,9 @@ extern "C" {
 #define DVM_PREC_FP64     0   /* everything FP64 (parity path)                */
 #define DVM_PREC_FP32     1   /* FP32 collision arithmetic, FP64 remap weights,
                                  moments and conservation correction          */
+#define DVM_PREC_FP32_REMAP32 2 /* as DVM_PREC_FP32, with the remap (nearest
+                                 nodes and weights) in FP32 too on equally
+                                 spaced axes (FP64 remap on other axes)       */
 /* sample-count rule */
 #define DVM_SAMPLES_REFERENCE 0 /* nint(dt*T^(2/3)*nA*nB/(2 Kn rms^2 beta3)) per integral */
 #define DVM_SAMPLES_EXPLICIT  1 /* samples_per_cell split over the 3 integrals

@@ -27,7 +27,7 @@ from tests._oracle import CellParallelOracle, load_oracle, oracle_solver
 
 pytestmark = pytest.mark.gpu
 
-FP64, FP32 = 0, 1
+FP64, FP32, FP32R = 0, 1, 2  # DVM_PREC_FP64, DVM_PREC_FP32, DVM_PREC_FP32_REMAP32
 
 
 def cellwise(a, b):
@@ -172,22 +172,23 @@ def test_shock_200_cells_full_size_matches_oracle(pkg):
     run_against_oracle(pkg, cfg, f, 2)
 
 
-def test_cavity_64x64_fp32_step_every_moment_within_1e5(pkg):
+@pytest.mark.parametrize("prec32", [FP32, FP32R], ids=["fp64_remap", "fp32_remap"])
+def test_cavity_64x64_fp32_step_every_moment_within_1e5(pkg, prec32):
     """From a developed FP64 state (200 steps: the lid has set up shear and heat
-    flux), one FP32-collision step against one FP64 step with the same seed:
-    density, temperature and entropy relative per cell; velocity in units of
-    the thermal speed; heat flux and viscous stress relative to the largest
-    magnitude of the field over the cavity (both vanish at equilibrium);
-    sum |f32 - f64| / sum |f64|. All <= 1e-5."""
+    flux), one FP32-collision step (remap in FP64, or in FP32) against one FP64
+    step with the same seed: density, temperature and entropy relative per
+    cell; velocity in units of the thermal speed; heat flux and viscous stress
+    relative to the largest magnitude of the field over the cavity (both vanish
+    at equilibrium); sum |f32 - f64| / sum |f64|. All <= 1e-5."""
     cfg, f = pkg.lid_driven_cavity()
-    print(f"[config] {cfg.name}: FP32 vs FP64, one step from a developed state", flush=True)
+    print(f"[config] {cfg.name}: precision {prec32} vs FP64, one step from a developed state", flush=True)
     with pkg.Solver(cfg.with_(precision=FP64)) as s:
         s.initialize(f)
         for n in range(1, 201):
             s.step(n)
         state = s.download_f()
     res = {}
-    for name, prec in (("64", FP64), ("32", FP32)):
+    for name, prec in (("64", FP64), ("32", prec32)):
         with pkg.Solver(cfg.with_(precision=prec)) as s:
             s.initialize(state)
             s.step(201)
@@ -217,10 +218,11 @@ def test_cavity_64x64_fp32_step_every_moment_within_1e5(pkg):
         assert v <= 1e-5, (k, v)
 
 
-def test_cavity_64x64_fp32_collision_conserves(pkg):
-    cfg, f = pkg.lid_driven_cavity(precision=FP32)
+@pytest.mark.parametrize("prec32", [FP32, FP32R], ids=["fp64_remap", "fp32_remap"])
+def test_cavity_64x64_fp32_collision_conserves(pkg, prec32):
+    cfg, f = pkg.lid_driven_cavity(precision=prec32)
     v = pkg.node_velocities(cfg)
-    print(f"[config] {cfg.name}: FP32 collision conservation, 2 collision phases", flush=True)
+    print(f"[config] {cfg.name}: precision {prec32} collision conservation, 2 collision phases", flush=True)
 
     def cons(g, vel):
         return np.stack([g.sum(1), g @ vel[:, 0], g @ vel[:, 1], g @ vel[:, 2], g @ (vel ** 2).sum(1)], axis=1)
