@@ -81,6 +81,8 @@ struct BatchPools {
   double *eqv, *dacc, *cdf, *eqpar, *props, *factors, *inflow;
   int2* tabs;
   int *counts, *err, *queue, *qstart;
+  unsigned char* sgn;  // deviation sign bytes: a cell's at the byte offset of its CDF slice
+  double* cmat;        // FP32 path: per cell the conservation factor (kCmat doubles), else null
 };
 
 __device__ __forceinline__ size_t cell_node0(const SlotDesc& d, int cell) { return (size_t)d.off + (size_t)cell * d.g.N; }
@@ -107,9 +109,10 @@ __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ s
   cp.beta3 = d.beta3;
   cp.samples_per_cell = d.samples;
   const size_t n0 = cell_node0(d, r.cell);
-  cdf_cell<BS, V, kFused>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + d.coff + (size_t)r.cell * 2 * (N + 1),
+  const size_t cs = (size_t)d.coff + (size_t)r.cell * 2 * (N + 1);
+  cdf_cell<BS, V, kFused>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + cs,
            P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh,
-           (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr);
+           (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr, P.sgn + cs);
 }
 
 // fd = f (FP64) or dacc = 0 (FP32) over the cells that take global atomics.
@@ -142,7 +145,7 @@ template <bool kF32, int BS, int K, bool kUni = false, bool kRemap32 = false>
 __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict__ slots,
                                                       const SlotState* __restrict__ state,
                                                       const WorkItem* __restrict__ items, CollParams base,
-                                                      BatchPools P) {
+                                                      BatchPools P, int fast_lds) {
   extern __shared__ double dyn[];
   __shared__ ConserveShared<BS> csh;
   __shared__ int s_item;
@@ -176,9 +179,19 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
                                   P.tabs + gc * 2 * kNBucket, N);
     Window win{};
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
+    // the fast FP32 sampling for a cell whose whole slice and sign bytes fit
+    // the launch's LDS (fast_lds bytes; 0 when the batch has no such path)
+    FastCell fcs{};
+    const FastCell* fc = nullptr;
+    if (d.full && fast_lds > 0 && (long long)(kAxisSlots + N) * 8 + ((N + 7) >> 3) <= (long long)fast_lds) {
+      const size_t cs = (size_t)d.coff + (size_t)w.cell * 2 * (N + 1);
+      fcs.sgn = P.sgn + cs;
+      fcs.cm = P.cmat ? P.cmat + gc * kCmat : nullptr;
+      fc = &fcs;
+    }
     const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
-                                              P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed});
+                                              P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed}, fc);
     if (bad) atomicOr(P.err, bad);
     __syncthreads();  // LDS axes, accumulator and s_item are reused by the next item
   }
@@ -364,6 +377,8 @@ struct dvm_batch {
   bool any_global = false;
   bool all_uniform = false;  // every resident slot's axes equally spaced
   size_t coll_lds = 0;
+  bool fast_ok = false;      // the species and precision admit the fast FP32 sampling
+  int fast_lds = 0;          // LDS bytes a cell's slice and sign bytes may take on the fast FP32 path (0: none)
   int persist = 0;
   int bs = 1024, k = 1, item_samples = 0;  // 0: sized from the resident load
   long long item_used = 0;
@@ -428,7 +443,8 @@ extern "C" void dvm_batch_destroy(dvm_batch* b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   void* ptrs[] = {b->d_slots, b->d_state, b->d_axes, b->P.f[0], b->P.f[1], b->P.eqv, b->P.dacc, b->P.cdf,
-                  b->P.eqpar, b->P.props, b->P.factors, b->P.inflow, b->P.tabs, b->P.counts, b->P.err,
+                  b->P.eqpar, b->P.props, b->P.factors, b->P.inflow, b->P.tabs, b->P.counts, b->P.err, b->P.sgn,
+                  b->P.cmat,
                   b->P.queue, b->P.qstart, b->d_cells, b->d_cells1d, b->d_adm, b->d_active, b->d_active0,
                   b->d_items, b->d_l1};
   for (void* p : ptrs)
@@ -475,6 +491,11 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
       (rc = balloc(b, &b->d_l1, max_slots)))
     return rc;
   if (f32 && (rc = balloc(b, &b->P.dacc, NC))) return rc;
+  if ((rc = balloc(b, &b->P.sgn, 2 * (NC + CC)))) return rc;
+  b->P.cmat = nullptr;
+  // the fast FP32 sampling: equally spaced axes, the FP64 remap, isotropic
+  // scattering, whole cells in LDS
+  b->fast_ok = f32 && !batch_r32(b) && !b->collp.vss;
   BHIP(b, hipMemset(b->P.err, 0, 4 * sizeof(int)));
   BHIP(b, hipMemset(b->P.counts, 0, CC * 4 * sizeof(int)));
   BHIP(b, hipMemset(b->d_state, 0, (size_t)max_slots * sizeof(SlotState)));
@@ -797,6 +818,20 @@ static int batch_lists(dvm_batch* b) {
   b->n_active0 = (int)act0.size();
   b->n_items = (int)wi.size();
   b->coll_lds = (size_t)(kAxisSlots + lds_n) * sizeof(double);
+  // the fast FP32 path also stages the sign bytes of a whole-grid cell after
+  // its slice; it applies to the slots for which the two fit
+  b->fast_lds = 0;
+  if (b->fast_ok && b->all_uniform) {
+    size_t need = b->coll_lds;
+    for (int s = 0; s < b->max_slots; ++s) {
+      const HostSlot& h = b->hs[s];
+      if (!h.active || !b->slots[s].full) continue;
+      const size_t n = (size_t)(kAxisSlots + h.N) * sizeof(double) + (size_t)((h.N + 7) >> 3);
+      if (n <= kLdsBytes - kStaticLds && n > need) need = n;
+    }
+    b->coll_lds = need;
+    b->fast_lds = (int)need;
+  }
   const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k, b->all_uniform, batch_r32(b));
   int per_cu = 0;
   BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, b->bs, b->coll_lds));
@@ -832,7 +867,8 @@ static int batch_enqueue_step(dvm_batch* b) {
     const WorkItem* it = b->d_items;
     CollParams cp = b->collp;
     BatchPools P = b->P;
-    void* args[] = {&sl, &st, &it, &cp, &P};
+    int fl = b->fast_lds;
+    void* args[] = {&sl, &st, &it, &cp, &P, &fl};
     BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k, b->all_uniform, batch_r32(b)), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
                             b->stream));
   }

@@ -53,6 +53,9 @@ struct dvm_ctx {
   int heat_ntime = -1;           // step of the values in d_heat
   double* d_heat = nullptr;      // [4][max(nx,ny)] wall heat fluxes
   double* dacc = nullptr;        // [C][N] collision change of the FP32 path
+  unsigned char* sgn = nullptr;  // [C][2 (N+1)] bytes: deviation sign bits (ceil(N/8) bytes used per cell)
+  double* cmat = nullptr;        // [C][kCmat] FP32 path: conservation factor of the CDF pass
+  int fast = 0;                  // the FP32 collision takes collide_fast (see collide_async)
   // explicit uniform list (dvm_set_uniform_stream): su_n values, read from
   // su_cursor on; sbase [C] per-cell offsets of the current collision
   double* d_su = nullptr;
@@ -205,7 +208,7 @@ extern "C" const char* dvm_last_error(const dvm_ctx* ctx) { return ctx ? ctx->ms
 extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_heat, c->d_axes, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->eqpar, c->props,
-                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc,
+                  c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc, c->sgn, c->cmat,
                   c->d_su, c->d_sbase, c->d_stotal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -386,6 +389,7 @@ static void species_params(const dvm_config* cfg, double b3min, CdfParams& cp, C
   co.ralpha = 1.0 / alpha_ab;
   co.vss = fabs(co.ralpha - 1.0) > kDoubleTol;
   co.num_repl = cfg->num_repl;
+  co.inv_repl = 1.0f / (float)cfg->num_repl;
 }
 
 // apply_fixed_in_out_flow (grid_ref_code boundary_conditions.f90:783-913):
@@ -491,7 +495,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
       (rc = dalloc(c, &c->cdf, 2 * CN1)) || (rc = dalloc(c, &c->tabs, (size_t)c->ncell * 2 * kNBucket)) ||
       (rc = dalloc(c, &c->eqpar, (size_t)c->ncell * 5)) ||
       (rc = dalloc(c, &c->props, (size_t)c->ncell * DVM_NPROP)) || (rc = dalloc(c, &c->factors, (size_t)c->ncell * 3)) ||
-      (rc = dalloc(c, &c->counts, (size_t)c->ncell * 4)) || (rc = dalloc(c, &c->err, 4)))
+      (rc = dalloc(c, &c->counts, (size_t)c->ncell * 4)) || (rc = dalloc(c, &c->err, 4)) ||
+      (rc = dalloc(c, &c->sgn, 2 * CN1)))
     return rc;
   HIP_TRY(c, hipMemset(c->err, 0, 4 * sizeof(int)));
   HIP_TRY(c, hipHostMalloc((void**)&c->err_host, 4 * sizeof(int), hipHostMallocDefault));
@@ -520,6 +525,15 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   c->coll_lds = (size_t)(kAxisSlots + c->wd.w[0] * c->wd.w[1] * c->wd.w[2]) * sizeof(double);
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
+  }
+  // The fast FP32 collision (collide_fast): equally spaced axes, the FP64
+  // remap, isotropic scattering, the whole cell's slice and its sign bytes in
+  // LDS.
+  {
+    const size_t need = c->coll_lds + (size_t)((N + 7) >> 3);
+    c->fast = (cfg->precision == DVM_PREC_FP32 && c->uniform_axes && c->lds_collide && !c->collp.vss &&
+               need <= kLdsBytes - kStaticLds) ? 1 : 0;
+    if (c->fast) c->coll_lds = need;
   }
   const void* kern = collide_kernel(c);
   HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
@@ -744,12 +758,13 @@ static int cdf_async(dvm_ctx* c) {
   if (c->ncell >= 4 * c->cus) {
     const int cap = cdf_sum_cap(c->g.N, kCdfV);
     hipLaunchKernelGGL((k_cdf<kCdfCtxBS, kCdfV, true>), dim3(c->ncell), dim3(kCdfCtxBS), cap * sizeof(float), c->stream,
-                       c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors, cap);
+                       c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors, cap,
+                       c->sgn);
   } else {
     const int cap = cdf_sum_cap(c->g.N, kCdfWideV);
     hipLaunchKernelGGL((k_cdf<kCdfWideBS, kCdfWideV, false>), dim3(c->ncell), dim3(kCdfWideBS), cap * sizeof(float),
                        c->stream, c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts,
-                       c->factors, cap);
+                       c->factors, cap, c->sgn);
   }
   HIP_TRY(c, hipGetLastError());
   return DVM_OK;
@@ -793,8 +808,10 @@ static int collide_async(dvm_ctx* c, int ntime) {
                   c->su_n - c->su_cursor, c->su_n);
     c->su_cursor += need;
   }
+  int fast = (c->fast && !stream) ? 1 : 0;
   void* args[] = {&c->g,    &c->collp, &ntime,  &c->split,  &c->wd,      &c->props, &c->f, &c->fd, &c->dacc,
-                  &c->eqv,  &c->cdf,   &c->tabs, &c->counts, &c->factors, &c->err,   &su,   &sbase, &per};
+                  &c->eqv,  &c->cdf,   &c->tabs, &c->counts, &c->factors, &c->err,   &su,   &sbase, &per,
+                  &c->sgn,  &c->cmat,  &fast};
   if (stream)
     HIP_TRY(c, hipLaunchKernel(collide_stream_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(kCollBS), args,
                                c->coll_lds, c->stream));

@@ -113,7 +113,26 @@ struct Axes {
   const R* a[3];
   int n[3];
   R vmin[3], vmax[3], factor[3];
+  // Closed-form remap on equally spaced axes (remap_uniform_interior): the
+  // spacing h of each axis, 1 / (2 (hx^2 + hy^2 + hz^2)), and the FP32 copies
+  // of vmin and factor that locate the nearest node.
+  R h[3], rden;
+  float fvmin[3], ffactor[3];
 };
+
+// Spacings and closed-form constants of Axes x from its vmin, vmax and n.
+template <class R>
+__device__ __forceinline__ void set_uniform_consts(Axes<R>& x) {
+  R s = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    x.h[d] = x.n[d] > 1 ? (x.vmax[d] - x.vmin[d]) / (R)(x.n[d] - 1) : (R)0;
+    s += x.h[d] * x.h[d];
+    x.fvmin[d] = (float)x.vmin[d];
+    x.ffactor[d] = (float)x.factor[d];
+  }
+  x.rden = s > (R)0 ? (R)1 / ((R)2 * s) : (R)0;
+}
 
 __device__ __forceinline__ Axes<double> grid_axes(const Grid& g) {
   Axes<double> x;
@@ -125,6 +144,7 @@ __device__ __forceinline__ Axes<double> grid_axes(const Grid& g) {
     x.vmax[d] = g.vmax[d];
     x.factor[d] = g.factor[d];
   }
+  set_uniform_consts(x);
   return x;
 }
 
@@ -287,6 +307,48 @@ __device__ __forceinline__ void remap_t(const Axes<R>& g, R u, R v, R w, R f_off
 #pragma unroll
     for (int q = 0; q < 7; ++q) s.m[q] = 0;
   }
+}
+
+// Conservative remap of a unit mass at (u, v, w) on equally spaced axes, for
+// the FP32-collision path, when the nearest node o lies off every axis end
+// (1 <= o_d <= n_d - 2); returns false, scattering nothing, otherwise. The
+// stencil is remap_t's (interior points o + s x, exterior o - s x, s = sign of
+// the offset, flag 7), and with ix = s hx, ex = -s hx its weight solve has the
+// closed form
+//   f_e  = (a^2 + b^2 + c^2 - |a| hx - |b| hy - |c| hz) / (2 (hx^2 + hy^2 + hz^2)),
+//   f_ix = |a| / hx + f_e  (likewise y, z),   f_o = 1 - f_ix - f_iy - f_iz - 3 f_e,
+// equal to remap_t's in exact arithmetic (moments 0, 1, 2 conserved) and
+// evaluated in R without divisions. The nearest node is located in FP32 (the
+// node closest to the FP32 velocity up to a rounding at a midpoint, where
+// either node gives a valid stencil). scatter(i, j, k, node, r * weight).
+template <class R, class Scatter>
+__device__ __forceinline__ bool remap_uniform_interior(const Axes<R>& g, float u, float v, float w, R r, int nx,
+                                                       int nxy, Scatter&& scatter) {
+  const float pu = fminf(fmaxf((u - g.fvmin[0]) * g.ffactor[0], -1.0f), (float)g.n[0]);
+  const float pv = fminf(fmaxf((v - g.fvmin[1]) * g.ffactor[1], -1.0f), (float)g.n[1]);
+  const float pw = fminf(fmaxf((w - g.fvmin[2]) * g.ffactor[2], -1.0f), (float)g.n[2]);
+  const int oi = (int)rintf(pu), oj = (int)rintf(pv), ok = (int)rintf(pw);
+  if ((unsigned)(oi - 1) >= (unsigned)(g.n[0] - 2) || (unsigned)(oj - 1) >= (unsigned)(g.n[1] - 2) ||
+      (unsigned)(ok - 1) >= (unsigned)(g.n[2] - 2))
+    return false;
+  const R a = (R)u - (g.vmin[0] + (R)oi * g.h[0]);
+  const R b = (R)v - (g.vmin[1] + (R)oj * g.h[1]);
+  const R c = (R)w - (g.vmin[2] + (R)ok * g.h[2]);
+  const R fa = fabs(a), fb = fabs(b), fc = fabs(c);
+  const R fe = (a * a + b * b + c * c - fa * g.h[0] - fb * g.h[1] - fc * g.h[2]) * g.rden;
+  const R fix = fa * g.factor[0] + fe, fiy = fb * g.factor[1] + fe, fiz = fc * g.factor[2] + fe;
+  const R fo = (R)1 - fix - fiy - fiz - (R)3 * fe;
+  const int sa = signbit(a) ? -1 : 1, sb = signbit(b) ? -1 : 1, sc = signbit(c) ? -1 : 1;
+  const int L = oi + nx * oj + nxy * ok;
+  const R re = r * fe;
+  scatter(oi, oj, ok, L, (double)(r * fo));
+  scatter(oi + sa, oj, ok, L + sa, (double)(r * fix));
+  scatter(oi, oj + sb, ok, L + sb * nx, (double)(r * fiy));
+  scatter(oi, oj, ok + sc, L + sc * nxy, (double)(r * fiz));
+  scatter(oi - sa, oj, ok, L - sa, (double)re);
+  scatter(oi, oj - sb, ok, L - sb * nx, (double)re);
+  scatter(oi, oj, ok - sc, L - sc * nxy, (double)re);
+  return true;
 }
 
 __device__ __forceinline__ void remap(const Grid& g, double u, double v, double w, double f_off, Stencil& s) {

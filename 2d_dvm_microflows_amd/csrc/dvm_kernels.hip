@@ -476,6 +476,25 @@ struct CdfShared {
   int s_new;
 };
 
+// Sign bits of the deviation f - f_eq at V consecutive nodes from l0 (bit q
+// for node l0 + q, set when the difference has its sign bit set, so -0 reads
+// as negative like dsgn): one byte per 8 nodes, byte l / 8 bit l % 8. V = 8:
+// each lane writes its byte; V = 4: lane pairs combine their halves.
+template <int V>
+__device__ __forceinline__ void store_sign_bits(const double (&dev)[V], int l0, int N,
+                                                unsigned char* __restrict__ sgn) {
+  static_assert(V == 8 || V == 4, "sign bytes need 4 or 8 nodes per lane");
+  unsigned b = 0;
+#pragma unroll
+  for (int q = 0; q < V; ++q) b |= (signbit(dev[q]) ? 1u : 0u) << q;
+  if constexpr (V == 8) {
+    if (l0 < N) sgn[l0 >> 3] = (unsigned char)b;
+  } else {
+    const unsigned hi = (unsigned)__shfl_xor((int)b, 1, 64);
+    if (!(threadIdx.x & 1) && l0 < N) sgn[l0 >> 3] = (unsigned char)(b | (hi << 4));
+  }
+}
+
 // Coalesced store of a chunk held as V consecutive values per lane, through
 // the tile (whose last reads precede a barrier the caller has passed).
 template <int BS, int V>
@@ -534,7 +553,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
                                          const double* __restrict__ p, double* __restrict__ ev,
                                          double* __restrict__ ec, int2* __restrict__ tab2, double* __restrict__ par,
                                          int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared<BS, V>& sh,
-                                         float* gsum) {
+                                         float* gsum, unsigned char* __restrict__ sgn) {
   constexpr int CH = BS * V;
   int* rights = sh.rights;
   double* tile = sh.tile;
@@ -602,8 +621,10 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
 #pragma unroll
       for (int q = 0; q < V; ++q) tile[tile_pos<V>(q * BS + threadIdx.x)] = nf[q];
     } else {
+      // the signed deviation (its magnitude is the CDF's term, its sign bit
+      // the deviation sign)
 #pragma unroll
-      for (int q = 0; q < V; ++q) tile[tile_pos<V>(q * BS + threadIdx.x)] = fabs(nf[q] - ne[q]);
+      for (int q = 0; q < V; ++q) tile[tile_pos<V>(q * BS + threadIdx.x)] = nf[q] - ne[q];
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) {
@@ -624,11 +645,14 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
       double before, last;
       cpv = cpv + chunk_scan_prev<BS, V>(pv, sh.red, cpv, prev_last, before, last);
       prev_last = last;
+      double dv[V];
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         e[q] = pv[q] - (q == 0 ? before : pv[q - 1]);
-        d[q] = (l0 + q < N) ? fabs(fv[q] - e[q]) : 0.0;
+        dv[q] = (l0 + q < N) ? fv[q] - e[q] : 0.0;
+        d[q] = fabs(dv[q]);
       }
+      if (sgn) store_sign_bits<V>(dv, l0, N, sgn);
       tile_store<BS, V>(e, tile, ev, base, N);
       chunk_scan2<BS, V>(e, d, sh.red, cec, cnc);
       tile_store<BS, V>(e, tile, ec + 1, base, N);
@@ -641,8 +665,9 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     } else {
       // |f - f_eq| prefix with the cached eqv; |f| prefix for the full Monte
       // Carlo kernel (full_mc.f90:110-180), which takes f_eq = 0
+      if (sgn) store_sign_bits<V>(fv, l0, N, sgn);
 #pragma unroll
-      for (int q = 0; q < V; ++q) d[q] = fv[q];
+      for (int q = 0; q < V; ++q) d[q] = fabs(fv[q]);
       cnc = cnc + chunk_scan<BS, V>(d, sh.red[0], cnc);
       tile_store<BS, V>(d, tile, nc + 1, base, N);
       if (gsum && l0 < N) gsum[ng + l0 / V] = (float)d[V - 1];
@@ -716,7 +741,7 @@ __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* 
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
                                                 double* __restrict__ eqpar,
                                                 int* __restrict__ counts, double* __restrict__ factors,
-                                                int sum_cap) {
+                                                int sum_cap, unsigned char* __restrict__ sgn) {
   __shared__ CdfShared<BS, V> sh;
   extern __shared__ float gdyn[];  // sum_cap floats: the scan's group ends
   const int cell = blockIdx.x;
@@ -724,7 +749,7 @@ __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* 
   float* gsum = (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr;
   cdf_cell<BS, V, kFused>(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
            cdf + (size_t)cell * 2 * (N + 1), tabs + (size_t)cell * 2 * kNBucket, eqpar + (size_t)cell * 5,
-           counts + cell * 4, factors + cell * 3, sh, gsum);
+           counts + cell * 4, factors + cell * 3, sh, gsum, sgn ? sgn + (size_t)cell * 2 * (N + 1) : nullptr);
 }
 
 // --------------------------------------------------------------- collide
@@ -844,6 +869,9 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
     if (valid[j >> 1] && u < kDoubleTol) idx[j] = 0;
     adj[j] = u * (off[j] == 0 ? cv.eq_total : cv.neq_total);
   }
+  // Bisection without branches per search: a search that has stopped keeps
+  // its index and reloads a valid address. 0 <= left <= right here, so
+  // (left + right) >> 1 is the reference's left + (right - left) / 2.
   for (int it = 0; it < kSearchCap; ++it) {
     double c0[M], c1[M];
 #pragma unroll
@@ -855,11 +883,13 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
     bool any = false;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-      if (!active[j]) continue;
-      if (adj[j] > c0[j] && adj[j] <= c1[j]) { active[j] = false; continue; }
-      if (adj[j] > c0[j]) left[j] = idx[j]; else right[j] = idx[j];
-      idx[j] = left[j] + (right[j] - left[j]) / 2;
-      any = true;
+      const bool gt = adj[j] > c0[j];
+      const bool go = active[j] && !(gt && adj[j] <= c1[j]);
+      left[j] = (go && gt) ? idx[j] : left[j];
+      right[j] = (go && !gt) ? idx[j] : right[j];
+      idx[j] = go ? (int)((unsigned)(left[j] + right[j]) >> 1) : idx[j];
+      active[j] = go;
+      any = any || go;
     }
     if (!any) break;
   }
@@ -924,9 +954,18 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<RM>& ax, cons
     scatter(iA, jA, kA, lA, -(double)kd1);
     scatter(iB, jB, kB, lB, -(double)kd2);
   }
-  // kd / NRepl; NRepl = 1 in the reference's decks, where the quotient is kd exactly
-  const R kr1 = (cp.num_repl == 1) ? kd1 : kd1 / (R)cp.num_repl;
-  const R kr2 = (cp.num_repl == 1) ? kd2 : kd2 / (R)cp.num_repl;
+  // kd / NRepl; NRepl = 1 in the reference's decks, where the quotient is kd
+  // exactly (FP32: times the FP32 reciprocal of NRepl)
+  R kr1 = kd1, kr2 = kd2;
+  if (cp.num_repl != 1) {
+    if constexpr (sizeof(R) == 4) {
+      kr1 = kd1 * (R)cp.inv_repl;
+      kr2 = kd2 * (R)cp.inv_repl;
+    } else {
+      kr1 = kd1 / (R)cp.num_repl;
+      kr2 = kd2 / (R)cp.num_repl;
+    }
+  }
   int bad = 0;
   for (int r = 1; r <= cp.num_repl; ++r) {
     double uphi, uth;
@@ -970,9 +1009,16 @@ __device__ __forceinline__ int vr_sample(const Grid& g, const Axes<RM>& ax, cons
     // post-collision velocities, remapped in RM
     // (remapping.f90:55-394 via apply_phi_replenishment :356-434)
     for (int side = 0; side < 2; ++side) {
+      const R pu = (side == 0) ? vc0 + gr1 * rx : vc0 + -(gr2 * rx);
+      const R pv = (side == 0) ? vc1 + gr1 * ry : vc1 + -(gr2 * ry);
+      const R pw = (side == 0) ? vc2 + gr1 * rz : vc2 + -(gr2 * rz);
+      if constexpr (kUni && !kRecord && sizeof(R) == 4) {
+        // equally spaced axes, FP32 path: the closed-form weight solve away
+        // from the axis ends (the general remap below otherwise)
+        if (remap_uniform_interior<RM>(ax, pu, pv, pw, (RM)(side == 0 ? kr1 : kr2), nx, nxy, scatter)) continue;
+      }
       StencilT<RM> st;
-      if (side == 0) remap_t<RM, kUni>(ax, (RM)(vc0 + gr1 * rx), (RM)(vc1 + gr1 * ry), (RM)(vc2 + gr1 * rz), (RM)1, st);
-      else remap_t<RM, kUni>(ax, (RM)(vc0 + -(gr2 * rx)), (RM)(vc1 + -(gr2 * ry)), (RM)(vc2 + -(gr2 * rz)), (RM)1, st);
+      remap_t<RM, kUni>(ax, (RM)pu, (RM)pv, (RM)pw, (RM)1, st);
       if (kRecord && r == 1) {
         int* o = side == 0 ? rec->st1 : rec->st2;
         double* m = side == 0 ? rec->m1 : rec->m2;
@@ -1029,6 +1075,7 @@ __device__ __forceinline__ Axes<R> stage_collision_axes(Grid& g, double* sh) {
       x.factor[d] = (float)g.factor[d];
       p += g.n[d];
     }
+    set_uniform_consts(x);
     return x;
   }
 }
@@ -1234,6 +1281,203 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
   return win;
 }
 
+// Inputs of the FP32 fast collision of a whole cell (collide_fast), beside
+// the cell view: its deviation sign bytes staged in LDS and the Cholesky
+// factor of sum |f| phi phi^T from the CDF pass.
+struct FastCell {
+  const unsigned char* sgn;  // bit l % 8 of byte l / 8 set when f - f_eq < 0 (or -0); staged into LDS
+  const double* cm;          // kCmat doubles: factor, then the not-positive-definite flag
+};
+
+// Conservation correction of a cell's FP32 collision change d (LDS) with the
+// Cholesky factor L of M = sum |f| phi phi^T precomputed by the CDF pass:
+// lambda solves L L^T lambda = sum d phi, and out = f + (d - |f| lambda.phi),
+// as conserve_cell but with one pass of five sums over d instead of twenty
+// over d and f.
+template <int BS>
+__device__ __forceinline__ int conserve_cell_factored(const Grid& g, const double* __restrict__ fc,
+                                                      const double* __restrict__ da, const double* __restrict__ cm,
+                                                      double* __restrict__ out, ConserveShared<BS>& sh) {
+  const int N = g.N, nx = g.n[0], nxy = g.nxy;
+  const float inv_nxy = (float)g.inv_nxy, inv_nx = (float)g.inv_n0;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double b[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int l = threadIdx.x; l < N; l += BS) {
+    const int k = (int)(((float)l + 0.5f) * inv_nxy);
+    const int r = l - k * nxy;
+    const int j = (int)(((float)r + 0.5f) * inv_nx);
+    const int i = r - j * nx;
+    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+    const double d = da[l];
+    b[0] += d;
+    b[1] += d * x;
+    b[2] += d * y;
+    b[3] += d * z;
+    b[4] += d * (x * x + y * y + z * z);
+  }
+#pragma unroll
+  for (int z = 0; z < 5; ++z) {
+    const double v = wave_sum(b[z]);
+    if (lane == 0) sh.red[wid][z] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double y[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      double v = 0.0;
+      for (int w = 0; w < BS / 64; ++w) v += sh.red[w][a];
+      y[a] = v;
+    }
+    const bool ok = cm[15] == 0.0;
+    sh.flag = ok ? 0 : kErrConserve;
+    // L y' = b, then L^T lambda = y' (L packed by rows: row i at i (i + 1) / 2)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double v = y[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) v -= cm[i * (i + 1) / 2 + k] * y[k];
+      y[i] = v / cm[i * (i + 1) / 2 + i];
+    }
+#pragma unroll
+    for (int i = 4; i >= 0; --i) {
+      double v = y[i];
+#pragma unroll
+      for (int k = i + 1; k < 5; ++k) v -= cm[k * (k + 1) / 2 + i] * y[k];
+      y[i] = v / cm[i * (i + 1) / 2 + i];
+    }
+    for (int a = 0; a < 5; ++a) sh.lam[a] = ok ? y[a] : 0.0;
+  }
+  __syncthreads();
+  const int flag = sh.flag;
+  const double l0 = sh.lam[0], l1 = sh.lam[1], l2 = sh.lam[2], l3 = sh.lam[3], l4 = sh.lam[4];
+  for (int l = threadIdx.x; l < N; l += BS) {
+    const int k = (int)(((float)l + 0.5f) * inv_nxy);
+    const int r = l - k * nxy;
+    const int j = (int)(((float)r + 0.5f) * inv_nx);
+    const int i = r - j * nx;
+    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+    const double fl = fc[l];
+    const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
+    out[l] = fl + (da[l] - corr);
+  }
+  return flag;
+}
+
+// The FP32-collision sampling of samples [q0, q1) of one cell on equally
+// spaced axes with isotropic scattering, accumulating the FP64 change in the
+// cell's LDS slice acc (the whole grid): variance_reduction.f90:686-1116 with
+// vr_depletion_routine and the elastic replenish_collision, as vr_sample, in a
+// leaner form for this case:
+//   * one Philox block per sample (plus one per further replenish location)
+//     and the two partner searches of pick_batch;
+//   * node indices decoded and node velocities formed in FP32 on the
+//     equally spaced axes;
+//   * the deviation signs read from the sign bytes in LDS;
+//   * for equal masses the post-collision velocities v_com +- r / 2 (the
+//     reference's g_1 / g = m_2 / (m_1 + m_2) = 1/2);
+//   * the closed-form remap away from the axis ends (remap_uniform_interior),
+//     remap_t at the ends.
+// Returns device error bits (kErrRemap: a pick outside the grid or a remap
+// without an exterior point).
+template <int BS>
+__device__ __forceinline__ int collide_fast(const Grid& g, const Axes<double>& ax, const CollParams& cp,
+                                            const CellView& cv, const unsigned char* sgn, int cell, int ntime,
+                                            int q0, int q1, const int* __restrict__ cnt,
+                                            const double* __restrict__ fac3, double* acc) {
+  const int N = g.N, nx = g.n[0], nxy = g.nxy;
+  const int n0 = cnt[0], n1 = cnt[1];
+  const float inv_nxy = (float)g.inv_nxy, inv_nx = (float)g.inv_n0;
+  const float vx0 = (float)ax.vmin[0], vy0 = (float)ax.vmin[1], vz0 = (float)ax.vmin[2];
+  const float hx = (float)ax.h[0], hy = (float)ax.h[1], hz = (float)ax.h[2];
+  const float fac0 = (float)fac3[0], fac1 = (float)fac3[1], fac2 = (float)fac3[2];
+  const float vexp = (float)cp.vhs_exponent;
+  const float two_pi = (float)(2.0 * kPiRef);
+  auto scat = [&](int, int, int, int node, double v) { atomicAdd(&acc[node], v); };
+  int bad = 0;
+  for (int base = q0; base < q1; base += BS) {
+    const int q = base + (int)threadIdx.x;
+    const bool valid = q < q1;
+    const int t = (q < n0) ? 0 : (q < n0 + n1 ? 1 : 2);
+    const int s = (t == 0) ? q : (t == 1 ? q - n0 : q - n0 - n1);
+    // partner picks (pick_collision_partner with referenced_search): the
+    // searches of pick_batch, whose lanes leave the bisection as their nodes
+    // are found (their loads then stop)
+    int lA, lB;
+    uint32_t wphi, wth;
+    pick_batch<1>(cp, cv, N, cell, ntime, &q, &t, &s, &valid, &lA, &lB, &wphi, &wth, PhiloxSrc{cp.seed});
+    if (!valid) continue;
+    if ((unsigned)lA >= (unsigned)N || (unsigned)lB >= (unsigned)N) {
+      bad |= kErrRemap;
+      continue;
+    }
+    const unsigned bytA = sgn[lA >> 3], bytB = sgn[lB >> 3];
+    int iA, jA, kA, iB, jB, kB;
+    {
+      kA = (int)(((float)lA + 0.5f) * inv_nxy);
+      const int rA = lA - kA * nxy;
+      jA = (int)(((float)rA + 0.5f) * inv_nx);
+      iA = rA - jA * nx;
+      kB = (int)(((float)lB + 0.5f) * inv_nxy);
+      const int rB = lB - kB * nxy;
+      jB = (int)(((float)rB + 0.5f) * inv_nx);
+      iB = rB - jB * nx;
+    }
+    const float x1 = fmaf((float)iA, hx, vx0), y1 = fmaf((float)jA, hy, vy0), z1 = fmaf((float)kA, hz, vz0);
+    const float x2 = fmaf((float)iB, hx, vx0), y2 = fmaf((float)jB, hy, vy0), z2 = fmaf((float)kB, hz, vz0);
+    const float gx = x2 - x1, gy = y2 - y1, gz = z2 - z1;
+    const float gg = __builtin_amdgcn_sqrtf(gx * gx + gy * gy + gz * gz);
+    if (gg < (float)kDoubleTol) continue;  // self-collision: counted, nothing moves
+    const float g_sigma = cp.hard_sphere ? gg : powf(gg, vexp);
+    const float vc0 = 0.5f * (x1 + x2), vc1 = 0.5f * (y1 + y2), vc2 = 0.5f * (z1 + z2);
+    // signs: +1 for an equilibrium partner, dsgn(f - f_eq) for a deviation partner
+    const float sA = (t == 0 || !((bytA >> (lA & 7)) & 1u)) ? 1.0f : -1.0f;
+    const float sB = (t == 1 || !((bytB >> (lB & 7)) & 1u)) ? 1.0f : -1.0f;
+    const float fac = (t == 0) ? fac0 : (t == 1 ? fac1 : fac2);
+    const float kd = sA * sB * fac * g_sigma;  // kin_depl of both partners
+    scat(iA, jA, kA, lA, -(double)kd);
+    scat(iB, jB, kB, lB, -(double)kd);
+    const float kr = (cp.num_repl == 1) ? kd : kd * cp.inv_repl;
+    for (int r = 1; r <= cp.num_repl; ++r) {
+      if (r > 1) {
+        uint32_t x4[4];
+        philox_block(cp.seed, (uint32_t)s, (uint32_t)((t + 1) | (r << 2)), (uint32_t)cell, (uint32_t)ntime, x4);
+        wphi = x4[0];
+        wth = x4[1];
+      }
+      // pick_random_velocity_on_sphere, isotropic (collision_utilities.f90:284-352)
+      const float uphi = (float)wphi * 0x1.0p-32f, uth = (float)wth * 0x1.0p-32f;
+      const float ph = two_pi * uphi;
+      const float sph = __sinf(ph), cph = __cosf(ph);
+      const float ct = 2.0f * uth - 1.0f;
+      const float stt = __builtin_amdgcn_sqrtf(1.0f - ct * ct);
+      const float hrx = 0.5f * (gg * stt * cph), hry = 0.5f * (gg * stt * sph), hrz = 0.5f * (gg * ct);
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const float pu = side == 0 ? vc0 + hrx : vc0 - hrx;
+        const float pv = side == 0 ? vc1 + hry : vc1 - hry;
+        const float pw = side == 0 ? vc2 + hrz : vc2 - hrz;
+        if (remap_uniform_interior<double>(ax, pu, pv, pw, (double)kr, nx, nxy, scat)) continue;
+        // an axis end: the reference's stencil
+        Stencil st;
+        remap_t<double, true>(ax, (double)pu, (double)pv, (double)pw, 1.0, st);
+        if (st.flag == 0) {
+          bad |= kErrRemap;
+          continue;
+        }
+#pragma unroll
+        for (int m = 0; m < 7; ++m) {
+          if (m < 4 || (st.flag & (1 << (m - 4)))) {
+            const int node = stencil_node(st, m, nx, nxy);
+            if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (double)kr * st.m[m]);
+          }
+        }
+      }
+    }
+  }
+  return bad;
+}
+
 // One work item of the collision phase: samples [q0, q1) of one cell
 // (q1 clamped to the cell's count) on BS lanes, accumulated in FP64 in the LDS
 // window acc[] (LDS atomics accumulate in FP64 in both precisions: ds_add_f32
@@ -1254,12 +1498,34 @@ __device__ __forceinline__ int collide_item(const Grid& g,
                                             int ntime, int q0, int q1, const int* __restrict__ cnt,
                                             const double* __restrict__ fac, const Window& win, bool full, bool sole,
                                             double* acc, double* __restrict__ fd, double* __restrict__ da,
-                                            ConserveShared<BS>* csh, const Src& src) {
+                                            ConserveShared<BS>* csh, const Src& src,
+                                            const FastCell* fast = nullptr) {
   const int N = g.N;
   const int total = cnt[0] + cnt[1] + cnt[2];
   q1 = q1 < total ? q1 : total;
   const int wn = full ? N : win.wn;
   for (int q = threadIdx.x; q < wn; q += BS) acc[q] = 0.0;
+  if constexpr (kF32 && kUni && !kRemap32 && std::is_same<Src, PhiloxSrc>::value) {
+    if (fast && full) {
+      // the cell's whole slice in LDS, its sign bytes after it
+      unsigned char* sl = reinterpret_cast<unsigned char*>(acc + N);
+      const int nb = (N + 7) >> 3;
+      for (int q = threadIdx.x; q < nb; q += BS) sl[q] = fast->sgn[q];
+      __syncthreads();
+      const int bad = collide_fast<BS>(g, ax, cp, cv, sl, rng_cell, ntime, q0, q1, cnt, fac, acc);
+      __syncthreads();
+      // the cell's only item corrects conservation here; the parts of a
+      // split cell add their slices to dacc for the separate correction
+      if (sole)
+        return bad | (fast->cm ? conserve_cell_factored<BS>(g, cv.f, acc, fast->cm, fd, *csh)
+                               : conserve_cell<BS>(g, cv.f, acc, fd, *csh));
+      for (int q = threadIdx.x; q < N; q += BS) {
+        const double v = acc[q];
+        if (v != 0.0) atomicAdd(&da[q], v);
+      }
+      return bad;
+    }
+  }
   __syncthreads();
   double* tgt = kF32 ? da : fd;
   int bad;
@@ -1325,7 +1591,8 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                                 const int2* __restrict__ tabs, const int* __restrict__ counts,
                                                 const double* __restrict__ factors, int* __restrict__ err,
                                                 const double* __restrict__ su, const long long* __restrict__ sbase,
-                                                int sper) {
+                                                int sper, const unsigned char* __restrict__ sgn,
+                                                const double* __restrict__ cmat, int fast) {
   extern __shared__ double dyn[];
   __shared__ ConserveShared<BS> csh;
   // Workgroup b is dispatched to XCD b % 8. With split > 1 the parts of one
@@ -1351,9 +1618,19 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
   Src src;
   if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
+  // FP32 on equally spaced axes with the whole grid in LDS: the fast
+  // sampling path with the sign bytes (and, for a cell's only part, the
+  // precomputed conservation factor)
+  FastCell fcs{};
+  const FastCell* fc = nullptr;
+  if (kLds && fast) {
+    fcs.sgn = sgn + (size_t)cell * 2 * (N + 1);
+    fcs.cm = cmat ? cmat + (size_t)cell * kCmat : nullptr;  // used by a cell's only part
+    fc = &fcs;
+  }
   const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
-                                            fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src);
+                                            fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src, fc);
   if (bad) atomicOr(err, bad);
 }
 

@@ -351,7 +351,8 @@ def test_serving_mix_alone_equals_in_the_mix(pkg):
     collisions conserve them exactly); shocks stop within three windows (300
     steps) of each other; and f (homogeneous) or the density and temperature
     profiles (shocks) of alone and mix differ by at most twice the difference
-    between the two seeds (and by 1e-12 when no pick moved)."""
+    between two seeds (for shocks the larger of two other seeds' differences),
+    and by 1e-12 when no pick moved."""
     reqs = pkg.synthetic_requests(count=36, rate=20.0, seed=5, precision=FP64)
     pools = dict(node_capacity=160_000_000, cell_capacity=65536, max_slots=512)
 
@@ -361,8 +362,9 @@ def test_serving_mix_alone_equals_in_the_mix(pkg):
                         sleep=clock.sleep, **pools) as srv:
             return srv.run(rs)
 
-    def reseeded(r):
-        return pkg.Request(r.rid, r.arrival, r.cfg.with_(seed=r.cfg.seed + 99991), r.f0, r.kind, r.tol, r.max_steps)
+    def reseeded(r, k=1):
+        return pkg.Request(r.rid, r.arrival, r.cfg.with_(seed=r.cfg.seed + 99991 * k), r.f0, r.kind, r.tol,
+                           r.max_steps)
 
     print("[config] serving mix, 36 requests", flush=True)
     st = serve(reqs)
@@ -389,8 +391,13 @@ def test_serving_mix_alone_equals_in_the_mix(pkg):
         dt = np.max(np.abs(a[:, 4] - b[:, 4])) / np.max(b[:, 4])
         return max(dn, dt)
 
+    # A steady shock's position wanders with the Monte Carlo noise, so the
+    # profile difference of two realisations is heavy-tailed: the mix is
+    # compared with the larger spread of two other seeds.
     for r in (shocks[0], max(shocks, key=lambda r: r.cfg.nx * r.cfg.num_points)):
-        alone, other, o = serve([r]).outcomes[0], serve([reseeded(r)]).outcomes[0], mix[r.rid]
+        alone, o = serve([r]).outcomes[0], mix[r.rid]
+        others = [serve([reseeded(r, k)]).outcomes[0] for k in (1, 2)]
+        other = max(others, key=lambda x: profile_diff(alone.props, x.props))
         d_mix, d_seed = profile_diff(alone.props, o.props), profile_diff(alone.props, other.props)
         print(f"  request {r.rid} (shock {r.cfg.nx} cells x {r.cfg.nvx}^3): {alone.steps} steps ({alone.reason}) "
               f"alone, {o.steps} ({o.reason}) in the mix, {other.steps} with another seed; profile difference "

@@ -75,7 +75,11 @@ def test_philox_words_as_stream_reproduce_default_mode(pkg, precision):
         a.step(n)
         b.step(n)
         assert b.uniform_stream_position() == u.size
-        ok, bad, worst = cellwise_close(b.download_f(), a.download_f())
+        # FP32: the default mode samples equally spaced axes through the fast
+        # FP32 path, the list mode through the general one; their picks (FP64)
+        # are the same and their FP32 roundings of the weights differ (about
+        # 1e-11 of f here)
+        ok, bad, worst = cellwise_close(b.download_f(), a.download_f(), rtol=1e-12 if precision == 0 else 1e-9)
         assert ok, (n, bad[:8], worst)
     a.close()
     b.close()
