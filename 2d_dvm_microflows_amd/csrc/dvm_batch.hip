@@ -252,10 +252,17 @@ __global__ void __launch_bounds__(kMomBS) k_moments_cells(const SlotDesc* __rest
                                                           const CellRef* __restrict__ cells, double mass,
                                                           BatchPools P) {
   __shared__ double red[1][6][kMomBS / 64];
+  __shared__ MomCmatShared<kMomBS> msh;
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
   const int cur = state[r.slot].cur;
-  moments_cell(d.g, P.f[cur] + cell_node0(d, r.cell), P.props + ((size_t)d.cell0 + r.cell) * DVM_NPROP, mass, red);
+  const size_t gc = (size_t)d.cell0 + r.cell;
+  // the FP32 fast path's batches: also the next collision's conservation factor
+  if (P.cmat)
+    moments_cmat_cell<kMomBS>(d.g, P.f[cur] + cell_node0(d, r.cell), P.props + gc * DVM_NPROP,
+                              P.cmat + gc * kCmat, mass, msh);
+  else
+    moments_cell(d.g, P.f[cur] + cell_node0(d, r.cell), P.props + gc * DVM_NPROP, mass, red);
 }
 
 // Distance of each space-homogeneous slot's f from equilibrium (l1_cell).
@@ -494,8 +501,11 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
   if ((rc = balloc(b, &b->P.sgn, 2 * (NC + CC)))) return rc;
   b->P.cmat = nullptr;
   // the fast FP32 sampling: equally spaced axes, the FP64 remap, isotropic
-  // scattering, whole cells in LDS
+  // scattering, whole cells in LDS; the moments kernel computes its
+  // conservation factors (after admission and after every step, so they are
+  // those of the f each collision starts from)
   b->fast_ok = f32 && !batch_r32(b) && !b->collp.vss;
+  if (b->fast_ok && (rc = balloc(b, &b->P.cmat, CC * kCmat))) return rc;
   BHIP(b, hipMemset(b->P.err, 0, 4 * sizeof(int)));
   BHIP(b, hipMemset(b->P.counts, 0, CC * 4 * sizeof(int)));
   BHIP(b, hipMemset(b->d_state, 0, (size_t)max_slots * sizeof(SlotState)));

@@ -56,6 +56,7 @@ struct dvm_ctx {
   unsigned char* sgn = nullptr;  // [C][2 (N+1)] bytes: deviation sign bits (ceil(N/8) bytes used per cell)
   double* cmat = nullptr;        // [C][kCmat] FP32 path: conservation factor of the CDF pass
   int fast = 0;                  // the FP32 collision takes collide_fast (see collide_async)
+  bool cmat_valid = false;       // cmat was computed from the current f (by moments_async)
   // explicit uniform list (dvm_set_uniform_stream): su_n values, read from
   // su_cursor on; sbase [C] per-cell offsets of the current collision
   double* d_su = nullptr;
@@ -533,7 +534,10 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     const size_t need = c->coll_lds + (size_t)((N + 7) >> 3);
     c->fast = (cfg->precision == DVM_PREC_FP32 && c->uniform_axes && c->lds_collide && !c->collp.vss &&
                need <= kLdsBytes - kStaticLds) ? 1 : 0;
-    if (c->fast) c->coll_lds = need;
+    if (c->fast) {
+      c->coll_lds = need;
+      if (int rc2 = dalloc(c, &c->cmat, (size_t)c->ncell * kCmat)) return rc2;
+    }
   }
   const void* kern = collide_kernel(c);
   HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
@@ -685,6 +689,7 @@ extern "C" int dvm_upload_f(dvm_ctx* c, int32_t species, const double* f) {
   if (species != 0) return fail(c, DVM_E_ARG, "only species 0 is supported");
   const size_t bytes = (size_t)c->ncell * c->N * sizeof(double);
   HIP_TRY(c, hipMemcpyAsync(c->f, f, bytes, hipMemcpyHostToDevice, c->stream));
+  c->cmat_valid = false;
   HIP_TRY(c, hipMemsetAsync(c->eqpar, 0, (size_t)c->ncell * 5 * sizeof(double), c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->f_valid = true;
@@ -735,6 +740,19 @@ static bool collides(const dvm_ctx* c) {
 }
 
 static int moments_async(dvm_ctx* c) {
+  if (c->cmat) {
+    // FP32 fast path: the moments and the next collision's conservation factor
+    if (c->ncell >= 2 * c->cus)
+      hipLaunchKernelGGL(k_moments_cmat<kMomBS>, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props,
+                         c->cmat, c->cfg.mass);
+    else
+      hipLaunchKernelGGL(k_moments_cmat<1024>, dim3(c->ncell), dim3(1024), 0, c->stream, c->g, c->f, c->props,
+                         c->cmat, c->cfg.mass);
+    HIP_TRY(c, hipGetLastError());
+    c->props_valid = true;
+    c->cmat_valid = true;
+    return DVM_OK;
+  }
   // kMomCells cells per workgroup when that still gives two workgroups per CU;
   // one cell per workgroup otherwise, on 1024 lanes when even that leaves the
   // chip short of two workgroups per CU (the 1-D shock, a homogeneous cell)
@@ -809,9 +827,12 @@ static int collide_async(dvm_ctx* c, int ntime) {
     c->su_cursor += need;
   }
   int fast = (c->fast && !stream) ? 1 : 0;
+  // the conservation factor of the moments pass when it is that of this f
+  // (else the collision kernel sums it itself)
+  const double* cm = c->cmat_valid ? c->cmat : nullptr;
   void* args[] = {&c->g,    &c->collp, &ntime,  &c->split,  &c->wd,      &c->props, &c->f, &c->fd, &c->dacc,
                   &c->eqv,  &c->cdf,   &c->tabs, &c->counts, &c->factors, &c->err,   &su,   &sbase, &per,
-                  &c->sgn,  &c->cmat,  &fast};
+                  &c->sgn,  &cm,       &fast};
   if (stream)
     HIP_TRY(c, hipLaunchKernel(collide_stream_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(kCollBS), args,
                                c->coll_lds, c->stream));
@@ -821,6 +842,7 @@ static int collide_async(dvm_ctx* c, int ntime) {
   if (!separate_conserve(c)) {
     double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
   }
+  c->cmat_valid = false;
   return DVM_OK;
 }
 
@@ -883,6 +905,7 @@ static int sweep_async(dvm_ctx* c) {
                        c->inflow, c->wallM, c->ratio, c->spec);
   HIP_TRY(c, hipGetLastError());
   double* t = c->f; c->f = c->fd; c->fd = t;  // phi := phi_conv
+  c->cmat_valid = false;
   return DVM_OK;
 }
 

@@ -91,10 +91,11 @@ __global__ void k_moments(Grid g, const double* __restrict__ f, double* __restri
                           int ncell);
 __global__ void k_moments_ext(Grid g, const double* __restrict__ f, const double* __restrict__ props, double mass,
                               double* __restrict__ ext);
-// Per-cell outputs of the CDF pass beside the CDFs: the deviation sign bytes
-// (ceil(N/8) bytes at byte offset 2 (N+1) per cell, the CDF slice's offset in
-// doubles) and, for the FP32 path, the Cholesky factor of sum |f| phi phi^T
-// (kCmat doubles per cell: 15 factor entries and a not-positive-definite flag).
+// The CDF pass writes the deviation sign bytes beside the CDFs (ceil(N/8)
+// bytes at byte offset 2 (N+1) per cell, the CDF slice's offset in doubles).
+// The FP32 path's moments kernel writes the Cholesky factor of
+// sum |f| phi phi^T per cell (kCmat doubles: 15 factor entries and a
+// not-positive-definite flag).
 constexpr int kCmat = 16;
 template <int BS, int V, bool kFused>
 __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const double* __restrict__ props,
@@ -102,6 +103,9 @@ __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const 
                       double* __restrict__ eqpar,
                       int* __restrict__ counts, double* __restrict__ factors, int sum_cap,
                       unsigned char* __restrict__ sgn);
+template <int BS>
+__global__ void k_moments_cmat(Grid g, const double* __restrict__ f, double* __restrict__ props,
+                               double* __restrict__ cmat, double mass);
 template <bool kStream>
 __global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
                         const double* __restrict__ eqv, const double* __restrict__ cdf,

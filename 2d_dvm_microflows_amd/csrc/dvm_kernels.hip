@@ -165,6 +165,119 @@ __device__ __forceinline__ void moments_cell(const Grid& g, const double* __rest
   }
 }
 
+// Moments of one cell (moments_cell) together with the Cholesky factor of
+// M = sum |f| phi phi^T, phi = (1, x, y, z, |v|^2), which the FP32 path's
+// conservation correction of the next collision solves with
+// (conserve_cell_factored): one pass over f of 6 + 15 sums. cm receives the
+// factor L (row-major packed lower triangle, 15 values) and cm[15] = 0, or
+// cm[15] = 1 and zeros when M is not numerically positive definite.
+template <int BS>
+struct MomCmatShared {
+  double red[21][BS / 64];
+  double M[25];
+};
+
+template <int BS>
+__device__ __forceinline__ void moments_cmat_cell(const Grid& g, const double* __restrict__ fc,
+                                                  double* __restrict__ p, double* __restrict__ cm, double mass,
+                                                  MomCmatShared<BS>& sh) {
+  const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
+  double a[21];
+#pragma unroll
+  for (int q = 0; q < 21; ++q) a[q] = 0.0;
+  // the loads of U nodes are issued together (one cell per workgroup: the
+  // memory parallelism of k_moments' several cells comes from here)
+  constexpr int U = 4;
+  for (int l0 = threadIdx.x; l0 < g.N; l0 += U * BS) {
+    double fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) fv[u] = (l0 + u * BS < g.N) ? fc[l0 + u * BS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int l = l0 + u * BS;
+      if (l >= g.N) break;
+      int i, j, k;
+      node_ijk(g, l, i, j, k);
+      const double x = X[i], y = Y[j], z = Z[k];
+      const double c2 = x * x + y * y + z * z;
+      const double v = fv[u];
+      // moments_sums' six, in its order
+      a[0] += v;
+      a[1] += v * x;
+      a[2] += v * y;
+      a[3] += v * z;
+      a[4] += v * c2;
+      a[5] += v * sqrt(c2);
+      // |f| phi_a phi_b, a <= b
+      const double w = fabs(v);
+      const double wx = w * x, wy = w * y, wz = w * z, we = w * c2;
+      a[6] += w; a[7] += wx; a[8] += wy; a[9] += wz; a[10] += we;
+      a[11] += wx * x; a[12] += wx * y; a[13] += wx * z; a[14] += wx * c2;
+      a[15] += wy * y; a[16] += wy * z; a[17] += wy * c2;
+      a[18] += wz * z; a[19] += wz * c2;
+      a[20] += we * c2;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 21; ++q) {
+    const double t = wave_sum(a[q]);
+    if (lane == 0) sh.red[q][wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[6];
+    for (int q = 0; q < 6; ++q) {
+      double t = 0.0;
+      for (int w = 0; w < BS / 64; ++w) t += sh.red[q][w];
+      tot[q] = t;
+    }
+    moments_finish(tot, p, mass);
+    double* M = sh.M;
+    int o = 6;
+    for (int r = 0; r < 5; ++r)
+      for (int c = r; c < 5; ++c) {
+        double t = 0.0;
+        for (int w = 0; w < BS / 64; ++w) t += sh.red[o][w];
+        M[r * 5 + c] = t;
+        M[c * 5 + r] = t;
+        ++o;
+      }
+    bool ok = true;
+    for (int j = 0; j < 5 && ok; ++j) {
+      double d = M[j * 5 + j];
+      for (int k = 0; k < j; ++k) d -= M[j * 5 + k] * M[j * 5 + k];
+      if (!(d > 0.0)) {
+        ok = false;
+        break;
+      }
+      d = sqrt(d);
+      M[j * 5 + j] = d;
+      for (int i = j + 1; i < 5; ++i) {
+        double v = M[i * 5 + j];
+        for (int k = 0; k < j; ++k) v -= M[i * 5 + k] * M[j * 5 + k];
+        M[i * 5 + j] = v / d;
+      }
+    }
+    o = 0;
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j <= i; ++j) cm[o++] = ok ? M[i * 5 + j] : 0.0;
+    cm[15] = ok ? 0.0 : 1.0;
+  }
+}
+
+// Moments and conservation factors of the cells of an FP32 context whose
+// collisions take collide_fast: one cell per workgroup.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_moments_cmat(Grid g, const double* __restrict__ f,
+                                                     double* __restrict__ props, double* __restrict__ cmat,
+                                                     double mass) {
+  __shared__ MomCmatShared<BS> sh;
+  const int cell = blockIdx.x;
+  moments_cmat_cell<BS>(g, f + (size_t)cell * g.N, props + (size_t)cell * DVM_NPROP, cmat + (size_t)cell * kCmat,
+                        mass, sh);
+}
+
 // Distance of one cell's f from equilibrium: sum |f - M| / n, where M is the
 // discrete Maxwellian (times beta^3) at the cell's own density, velocity and
 // temperature p, scaled so that sum M = n; -1 for an empty or cold cell. The
