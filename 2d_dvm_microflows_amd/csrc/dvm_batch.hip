@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
     // the launch's LDS (fast_lds bytes; 0 when the batch has no such path)
     FastCell fcs{};
     const FastCell* fc = nullptr;
-    if (d.full && fast_lds > 0 && (long long)(kAxisSlots + N) * 8 + ((N + 7) >> 3) <= (long long)fast_lds) {
+    if (d.full && fast_lds > 0 && (long long)(kAxisSlots + N) * 8 + 4 * ((N + 31) >> 5) <= (long long)fast_lds) {
       const size_t cs = (size_t)d.coff + (size_t)w.cell * 2 * (N + 1);
       fcs.sgn = P.sgn + cs;
       fcs.cm = P.cmat ? P.cmat + gc * kCmat : nullptr;
@@ -836,7 +836,7 @@ static int batch_lists(dvm_batch* b) {
     for (int s = 0; s < b->max_slots; ++s) {
       const HostSlot& h = b->hs[s];
       if (!h.active || !b->slots[s].full) continue;
-      const size_t n = (size_t)(kAxisSlots + h.N) * sizeof(double) + (size_t)((h.N + 7) >> 3);
+      const size_t n = (size_t)(kAxisSlots + h.N) * sizeof(double) + 4 * (size_t)((h.N + 31) >> 5);
       if (n <= kLdsBytes - kStaticLds && n > need) need = n;
     }
     b->coll_lds = need;

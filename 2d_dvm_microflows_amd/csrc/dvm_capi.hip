@@ -531,7 +531,7 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   // remap, isotropic scattering, the whole cell's slice and its sign bytes in
   // LDS.
   {
-    const size_t need = c->coll_lds + (size_t)((N + 7) >> 3);
+    const size_t need = c->coll_lds + 4 * (size_t)((N + 31) >> 5);  // sign words after the slice
     c->fast = (cfg->precision == DVM_PREC_FP32 && c->uniform_axes && c->lds_collide && !c->collp.vss &&
                need <= kLdsBytes - kStaticLds) ? 1 : 0;
     if (c->fast) {

@@ -972,6 +972,9 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
     lr[2 * k] = cv.eqtab[(eqA ? 0 : kNBucket) + bA - 1];
     lr[2 * k + 1] = cv.eqtab[(eqB ? 0 : kNBucket) + bB - 1];
   }
+  // st: the search's index while it runs, its bitwise complement once it has
+  // stopped (the node found, the bracket closed, or nothing to search)
+  int st[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const double u = adj[j];
@@ -980,36 +983,40 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
     idx[j] = left[j] + (right[j] - left[j]) / 2;
     active[j] = valid[j >> 1] && !(u < kDoubleTol) && left[j] != right[j];
     if (valid[j >> 1] && u < kDoubleTol) idx[j] = 0;
+    st[j] = active[j] ? idx[j] : ~idx[j];
     adj[j] = u * (off[j] == 0 ? cv.eq_total : cv.neq_total);
   }
-  // Bisection without branches per search: a search that has stopped keeps
-  // its index and reloads a valid address. 0 <= left <= right here, so
+  // Bisection without branches per search: a stopped search loads node 0 of
+  // its CDF pair (one line for all such lanes). 0 <= left <= right here, so
   // (left + right) >> 1 is the reference's left + (right - left) / 2.
   for (int it = 0; it < kSearchCap; ++it) {
     double c0[M], c1[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-      const double* a = cv.eqc + off[j] + (active[j] ? idx[j] : 0);
+      const unsigned e = (unsigned)(off[j] + max(st[j], 0));
+      const double* a = reinterpret_cast<const double*>(reinterpret_cast<const char*>(cv.eqc) + e * 8u);
       c0[j] = a[0];
       c1[j] = a[1];
     }
     bool any = false;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
+      const int id = st[j];
+      const bool act = id >= 0;
       const bool gt = adj[j] > c0[j];
-      const bool go = active[j] && !(gt && adj[j] <= c1[j]);
-      left[j] = (go && gt) ? idx[j] : left[j];
-      right[j] = (go && !gt) ? idx[j] : right[j];
-      idx[j] = go ? (int)((unsigned)(left[j] + right[j]) >> 1) : idx[j];
-      active[j] = go;
+      const bool go = act && !(gt && adj[j] <= c1[j]);
+      left[j] = (go && gt) ? id : left[j];
+      right[j] = (go && !gt) ? id : right[j];
+      const int nid = (int)((unsigned)(left[j] + right[j]) >> 1);
+      st[j] = go ? nid : (act ? ~id : id);
       any = any || go;
     }
     if (!any) break;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    lA[k] = idx[2 * k];
-    lB[k] = idx[2 * k + 1];
+    lA[k] = st[2 * k] < 0 ? ~st[2 * k] : st[2 * k];
+    lB[k] = st[2 * k + 1] < 0 ? ~st[2 * k + 1] : st[2 * k + 1];
   }
 }
 
@@ -1308,22 +1315,31 @@ __device__ __forceinline__ int conserve_cell(const Grid& g, const double* __rest
     double sums[20];
 #pragma unroll
     for (int z = 0; z < 20; ++z) sums[z] = 0.0;
-    for (int l = threadIdx.x; l < N; l += BS) {
-      int i, j, k;
-      node_ijk(g, l, i, j, k);
-      const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-      const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
-      const double d = da[l];
-      const double w = fabs(fc[l]);
+    constexpr int U = 4;  // loads of f in flight per lane
+    for (int lb = threadIdx.x; lb < N; lb += U * BS) {
+      double fv[U];
 #pragma unroll
-      for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
+      for (int u = 0; u < U; ++u) fv[u] = (lb + u * BS < N) ? fc[lb + u * BS] : 0.0;
 #pragma unroll
-      for (int b = 0; b < 5; ++b) sums[5 + b] += w * phi[0] * phi[b];
-      int o = 10;
+      for (int u = 0; u < U; ++u) {
+        const int l = lb + u * BS;
+        if (l >= N) break;
+        int i, j, k;
+        node_ijk(g, l, i, j, k);
+        const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+        const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
+        const double d = da[l];
+        const double w = fabs(fv[u]);
 #pragma unroll
-      for (int a = 1; a < 5; ++a)
+        for (int a = 0; a < 5; ++a) sums[a] += d * phi[a];
 #pragma unroll
-        for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
+        for (int b = 0; b < 5; ++b) sums[5 + b] += w * phi[0] * phi[b];
+        int o = 10;
+#pragma unroll
+        for (int a = 1; a < 5; ++a)
+#pragma unroll
+          for (int b = a; b < 5; ++b) sums[o++] += w * phi[a] * phi[b];
+      }
     }
 #pragma unroll
     for (int z = 0; z < 20; ++z) {
@@ -1356,13 +1372,22 @@ __device__ __forceinline__ int conserve_cell(const Grid& g, const double* __rest
   __syncthreads();
   const int flag = sh.flag;
   const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
-  for (int l = threadIdx.x; l < N; l += BS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-    const double fl = fc[l];
-    const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
-    out[l] = fl + (da[l] - corr);
+  constexpr int U = 8;  // loads of f in flight per lane
+  for (int lb = threadIdx.x; lb < N; lb += U * BS) {
+    double fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) fv[u] = (lb + u * BS < N) ? fc[lb + u * BS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int l = lb + u * BS;
+      if (l >= N) break;
+      int i, j, k;
+      node_ijk(g, l, i, j, k);
+      const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+      const double fl = fv[u];
+      const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
+      out[l] = fl + (da[l] - corr);
+    }
   }
   return flag;
 }
@@ -1442,37 +1467,48 @@ __device__ __forceinline__ int conserve_cell_factored(const Grid& g, const doubl
       for (int w = 0; w < BS / 64; ++w) v += sh.red[w][a];
       y[a] = v;
     }
-    const bool ok = cm[15] == 0.0;
+    const double* L = cm;
+    const bool ok = L[15] == 0.0;
     sh.flag = ok ? 0 : kErrConserve;
     // L y' = b, then L^T lambda = y' (L packed by rows: row i at i (i + 1) / 2)
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       double v = y[i];
 #pragma unroll
-      for (int k = 0; k < i; ++k) v -= cm[i * (i + 1) / 2 + k] * y[k];
-      y[i] = v / cm[i * (i + 1) / 2 + i];
+      for (int k = 0; k < i; ++k) v -= L[i * (i + 1) / 2 + k] * y[k];
+      y[i] = v / L[i * (i + 1) / 2 + i];
     }
 #pragma unroll
     for (int i = 4; i >= 0; --i) {
       double v = y[i];
 #pragma unroll
-      for (int k = i + 1; k < 5; ++k) v -= cm[k * (k + 1) / 2 + i] * y[k];
-      y[i] = v / cm[i * (i + 1) / 2 + i];
+      for (int k = i + 1; k < 5; ++k) v -= L[k * (k + 1) / 2 + i] * y[k];
+      y[i] = v / L[i * (i + 1) / 2 + i];
     }
     for (int a = 0; a < 5; ++a) sh.lam[a] = ok ? y[a] : 0.0;
   }
   __syncthreads();
   const int flag = sh.flag;
   const double l0 = sh.lam[0], l1 = sh.lam[1], l2 = sh.lam[2], l3 = sh.lam[3], l4 = sh.lam[4];
-  for (int l = threadIdx.x; l < N; l += BS) {
-    const int k = (int)(((float)l + 0.5f) * inv_nxy);
-    const int r = l - k * nxy;
-    const int j = (int)(((float)r + 0.5f) * inv_nx);
-    const int i = r - j * nx;
-    const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
-    const double fl = fc[l];
-    const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
-    out[l] = fl + (da[l] - corr);
+  // U loads of f in flight per lane (the cell's f comes from HBM)
+  constexpr int U = 8;
+  for (int lb = threadIdx.x; lb < N; lb += U * BS) {
+    double fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) fv[u] = (lb + u * BS < N) ? fc[lb + u * BS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int l = lb + u * BS;
+      if (l >= N) break;
+      const int k = (int)(((float)l + 0.5f) * inv_nxy);
+      const int r = l - k * nxy;
+      const int j = (int)(((float)r + 0.5f) * inv_nx);
+      const int i = r - j * nx;
+      const double x = g.ax[0][i], y = g.ax[1][j], z = g.ax[2][k];
+      const double fl = fv[u];
+      const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
+      out[l] = fl + (da[l] - corr);
+    }
   }
   return flag;
 }
@@ -1617,13 +1653,21 @@ __device__ __forceinline__ int collide_item(const Grid& g,
   const int total = cnt[0] + cnt[1] + cnt[2];
   q1 = q1 < total ? q1 : total;
   const int wn = full ? N : win.wn;
-  for (int q = threadIdx.x; q < wn; q += BS) acc[q] = 0.0;
   if constexpr (kF32 && kUni && !kRemap32 && std::is_same<Src, PhiloxSrc>::value) {
     if (fast && full) {
-      // the cell's whole slice in LDS, its sign bytes after it
+      // the cell's whole slice in LDS, its sign bytes after it, copied as
+      // 32-bit words (a cell's sign bytes start 4-byte aligned, 2 (N+1) bytes
+      // per cell, and the words cover at most 3 bytes past them, inside the
+      // cell's slot); the words are loaded before the slice is cleared
       unsigned char* sl = reinterpret_cast<unsigned char*>(acc + N);
-      const int nb = (N + 7) >> 3;
-      for (int q = threadIdx.x; q < nb; q += BS) sl[q] = fast->sgn[q];
+      const int nw = (N + 31) >> 5;
+      const unsigned* sw = reinterpret_cast<const unsigned*>(fast->sgn);
+      unsigned* lw = reinterpret_cast<unsigned*>(sl);
+      unsigned w0 = 0u;
+      if ((int)threadIdx.x < nw) w0 = sw[threadIdx.x];
+      for (int q = threadIdx.x; q < wn; q += BS) acc[q] = 0.0;
+      if ((int)threadIdx.x < nw) lw[threadIdx.x] = w0;
+      for (int q = threadIdx.x + BS; q < nw; q += BS) lw[q] = sw[q];
       __syncthreads();
       const int bad = collide_fast<BS>(g, ax, cp, cv, sl, rng_cell, ntime, q0, q1, cnt, fac, acc);
       __syncthreads();
@@ -1639,6 +1683,7 @@ __device__ __forceinline__ int collide_item(const Grid& g,
       return bad;
     }
   }
+  for (int q = threadIdx.x; q < wn; q += BS) acc[q] = 0.0;
   __syncthreads();
   double* tgt = kF32 ? da : fd;
   int bad;
@@ -1660,6 +1705,19 @@ __device__ __forceinline__ int collide_item(const Grid& g,
   __syncthreads();
   if (kF32 && full && sole) return bad | conserve_cell<BS>(g, cv.f, acc, fd, *csh);
   const int nx = g.n[0], nxy = g.nxy;
+  if (full && sole && !kF32) {
+    // fd = f + change over the whole grid, U loads of f in flight per lane
+    constexpr int U = 8;
+    for (int qb = threadIdx.x; qb < N; qb += U * BS) {
+      double fv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) fv[u] = (qb + u * BS < N) ? cv.f[qb + u * BS] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (qb + u * BS < N) fd[qb + u * BS] = fv[u] + acc[qb + u * BS];
+    }
+    return bad;
+  }
   for (int q = threadIdx.x; q < wn; q += BS) {
     int l = q;
     if (!full) {
