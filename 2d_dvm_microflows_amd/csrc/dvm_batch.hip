@@ -252,15 +252,15 @@ __global__ void __launch_bounds__(kMomBS) k_moments_cells(const SlotDesc* __rest
                                                           const CellRef* __restrict__ cells, double mass,
                                                           BatchPools P) {
   __shared__ double red[1][6][kMomBS / 64];
-  __shared__ MomCmatShared<kMomBS> msh;
+  __shared__ MomCmatShared<1, kMomBS> msh;
   const CellRef r = cells[blockIdx.x];
   const SlotDesc& d = slots[r.slot];
   const int cur = state[r.slot].cur;
   const size_t gc = (size_t)d.cell0 + r.cell;
   // the FP32 fast path's batches: also the next collision's conservation factor
   if (P.cmat)
-    moments_cmat_cell<kMomBS>(d.g, P.f[cur] + cell_node0(d, r.cell), P.props + gc * DVM_NPROP,
-                              P.cmat + gc * kCmat, mass, msh);
+    moments_cmat_cells<1, kMomBS>(d.g, P.f[cur] + cell_node0(d, r.cell), 0, 1, P.props + gc * DVM_NPROP,
+                                  P.cmat + gc * kCmat, mass, msh);
   else
     moments_cell(d.g, P.f[cur] + cell_node0(d, r.cell), P.props + gc * DVM_NPROP, mass, red);
 }

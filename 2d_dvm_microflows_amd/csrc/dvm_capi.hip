@@ -742,12 +742,16 @@ static bool collides(const dvm_ctx* c) {
 static int moments_async(dvm_ctx* c) {
   if (c->cmat) {
     // FP32 fast path: the moments and the next collision's conservation factor
-    if (c->ncell >= 2 * c->cus)
-      hipLaunchKernelGGL(k_moments_cmat<kMomBS>, dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f, c->props,
-                         c->cmat, c->cfg.mass);
+    // two cells per workgroup when that still gives two workgroups per CU
+    if (c->ncell >= 4 * c->cus)
+      hipLaunchKernelGGL((k_moments_cmat<2, kMomBS>), dim3((c->ncell + 1) / 2), dim3(kMomBS), 0, c->stream, c->g,
+                         c->f, c->props, c->cmat, c->cfg.mass, c->ncell);
+    else if (c->ncell >= 2 * c->cus)
+      hipLaunchKernelGGL((k_moments_cmat<1, kMomBS>), dim3(c->ncell), dim3(kMomBS), 0, c->stream, c->g, c->f,
+                         c->props, c->cmat, c->cfg.mass, c->ncell);
     else
-      hipLaunchKernelGGL(k_moments_cmat<1024>, dim3(c->ncell), dim3(1024), 0, c->stream, c->g, c->f, c->props,
-                         c->cmat, c->cfg.mass);
+      hipLaunchKernelGGL((k_moments_cmat<1, 1024>), dim3(c->ncell), dim3(1024), 0, c->stream, c->g, c->f, c->props,
+                         c->cmat, c->cfg.mass, c->ncell);
     HIP_TRY(c, hipGetLastError());
     c->props_valid = true;
     c->cmat_valid = true;

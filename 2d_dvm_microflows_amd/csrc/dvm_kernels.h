@@ -103,9 +103,9 @@ __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const 
                       double* __restrict__ eqpar,
                       int* __restrict__ counts, double* __restrict__ factors, int sum_cap,
                       unsigned char* __restrict__ sgn);
-template <int BS>
+template <int C, int BS>
 __global__ void k_moments_cmat(Grid g, const double* __restrict__ f, double* __restrict__ props,
-                               double* __restrict__ cmat, double mass);
+                               double* __restrict__ cmat, double mass, int ncell);
 template <bool kStream>
 __global__ void k_trace(Grid g, CollParams cp, int ntime, int cell, const double* __restrict__ f,
                         const double* __restrict__ eqv, const double* __restrict__ cdf,

@@ -165,33 +165,39 @@ __device__ __forceinline__ void moments_cell(const Grid& g, const double* __rest
   }
 }
 
-// Moments of one cell (moments_cell) together with the Cholesky factor of
-// M = sum |f| phi phi^T, phi = (1, x, y, z, |v|^2), which the FP32 path's
-// conservation correction of the next collision solves with
-// (conserve_cell_factored): one pass over f of 6 + 15 sums. cm receives the
-// factor L (row-major packed lower triangle, 15 values) and cm[15] = 0, or
-// cm[15] = 1 and zeros when M is not numerically positive definite.
-template <int BS>
+// Moments of C cells (moments_cell) together with the Cholesky factor of
+// M = sum |f| phi phi^T, phi = (1, x, y, z, |v|^2), per cell, which the FP32
+// path's conservation correction of the next collision solves with
+// (conserve_cell_factored): one pass over f of 6 + 15 sums per cell, the
+// node's coordinates formed once for the C cells (f of cell c at
+// fc + c * stride; nc <= C cells present). cm[c * kCmat ..] receives the
+// factor L (row-major packed lower triangle, 15 values) and a 0, or 1 and
+// zeros when M is not numerically positive definite.
+template <int C, int BS>
 struct MomCmatShared {
-  double red[21][BS / 64];
-  double M[25];
+  double red[C][21][BS / 64];
+  double M[C][25];
 };
 
-template <int BS>
-__device__ __forceinline__ void moments_cmat_cell(const Grid& g, const double* __restrict__ fc,
-                                                  double* __restrict__ p, double* __restrict__ cm, double mass,
-                                                  MomCmatShared<BS>& sh) {
+template <int C, int BS>
+__device__ __forceinline__ void moments_cmat_cells(const Grid& g, const double* __restrict__ fc, size_t stride,
+                                                   int nc, double* __restrict__ p, double* __restrict__ cm,
+                                                   double mass, MomCmatShared<C, BS>& sh) {
   const double *X = g.ax[0], *Y = g.ax[1], *Z = g.ax[2];
-  double a[21];
+  double a[C][21];
 #pragma unroll
-  for (int q = 0; q < 21; ++q) a[q] = 0.0;
-  // the loads of U nodes are issued together (one cell per workgroup: the
-  // memory parallelism of k_moments' several cells comes from here)
-  constexpr int U = 4;
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 21; ++q) a[c][q] = 0.0;
+  // with one cell the loads of U nodes are issued together
+  constexpr int U = (C == 1) ? 4 : 1;
   for (int l0 = threadIdx.x; l0 < g.N; l0 += U * BS) {
-    double fv[U];
+    double fv[U][C];
 #pragma unroll
-    for (int u = 0; u < U; ++u) fv[u] = (l0 + u * BS < g.N) ? fc[l0 + u * BS] : 0.0;
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        fv[u][c] = (l0 + u * BS < g.N && c < nc) ? fc[(size_t)c * stride + l0 + u * BS] : 0.0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int l = l0 + u * BS;
@@ -200,47 +206,54 @@ __device__ __forceinline__ void moments_cmat_cell(const Grid& g, const double* _
       node_ijk(g, l, i, j, k);
       const double x = X[i], y = Y[j], z = Z[k];
       const double c2 = x * x + y * y + z * z;
-      const double v = fv[u];
-      // moments_sums' six, in its order
-      a[0] += v;
-      a[1] += v * x;
-      a[2] += v * y;
-      a[3] += v * z;
-      a[4] += v * c2;
-      a[5] += v * sqrt(c2);
-      // |f| phi_a phi_b, a <= b
-      const double w = fabs(v);
-      const double wx = w * x, wy = w * y, wz = w * z, we = w * c2;
-      a[6] += w; a[7] += wx; a[8] += wy; a[9] += wz; a[10] += we;
-      a[11] += wx * x; a[12] += wx * y; a[13] += wx * z; a[14] += wx * c2;
-      a[15] += wy * y; a[16] += wy * z; a[17] += wy * c2;
-      a[18] += wz * z; a[19] += wz * c2;
-      a[20] += we * c2;
+      const double sp = sqrt(c2);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const double v = fv[u][c];
+        // moments_sums' six, in its order
+        a[c][0] += v;
+        a[c][1] += v * x;
+        a[c][2] += v * y;
+        a[c][3] += v * z;
+        a[c][4] += v * c2;
+        a[c][5] += v * sp;
+        // |f| phi_a phi_b, a <= b
+        const double w = fabs(v);
+        const double wx = w * x, wy = w * y, wz = w * z, we = w * c2;
+        a[c][6] += w; a[c][7] += wx; a[c][8] += wy; a[c][9] += wz; a[c][10] += we;
+        a[c][11] += wx * x; a[c][12] += wx * y; a[c][13] += wx * z; a[c][14] += wx * c2;
+        a[c][15] += wy * y; a[c][16] += wy * z; a[c][17] += wy * c2;
+        a[c][18] += wz * z; a[c][19] += wz * c2;
+        a[c][20] += we * c2;
+      }
     }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < 21; ++q) {
-    const double t = wave_sum(a[q]);
-    if (lane == 0) sh.red[q][wid] = t;
-  }
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int q = 0; q < 21; ++q) {
+      const double t = wave_sum(a[c][q]);
+      if (lane == 0) sh.red[c][q][wid] = t;
+    }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  const int c = threadIdx.x;
+  if (c < nc) {
     double tot[6];
     for (int q = 0; q < 6; ++q) {
       double t = 0.0;
-      for (int w = 0; w < BS / 64; ++w) t += sh.red[q][w];
+      for (int w = 0; w < BS / 64; ++w) t += sh.red[c][q][w];
       tot[q] = t;
     }
-    moments_finish(tot, p, mass);
-    double* M = sh.M;
+    moments_finish(tot, p + (size_t)c * DVM_NPROP, mass);
+    double* M = sh.M[c];
     int o = 6;
     for (int r = 0; r < 5; ++r)
-      for (int c = r; c < 5; ++c) {
+      for (int e = r; e < 5; ++e) {
         double t = 0.0;
-        for (int w = 0; w < BS / 64; ++w) t += sh.red[o][w];
-        M[r * 5 + c] = t;
-        M[c * 5 + r] = t;
+        for (int w = 0; w < BS / 64; ++w) t += sh.red[c][o][w];
+        M[r * 5 + e] = t;
+        M[e * 5 + r] = t;
         ++o;
       }
     bool ok = true;
@@ -259,23 +272,25 @@ __device__ __forceinline__ void moments_cmat_cell(const Grid& g, const double* _
         M[i * 5 + j] = v / d;
       }
     }
+    double* out = cm + (size_t)c * kCmat;
     o = 0;
     for (int i = 0; i < 5; ++i)
-      for (int j = 0; j <= i; ++j) cm[o++] = ok ? M[i * 5 + j] : 0.0;
-    cm[15] = ok ? 0.0 : 1.0;
+      for (int j = 0; j <= i; ++j) out[o++] = ok ? M[i * 5 + j] : 0.0;
+    out[15] = ok ? 0.0 : 1.0;
   }
 }
 
 // Moments and conservation factors of the cells of an FP32 context whose
-// collisions take collide_fast: one cell per workgroup.
-template <int BS>
+// collisions take collide_fast: C cells per workgroup.
+template <int C, int BS>
 __global__ void __launch_bounds__(BS) k_moments_cmat(Grid g, const double* __restrict__ f,
                                                      double* __restrict__ props, double* __restrict__ cmat,
-                                                     double mass) {
-  __shared__ MomCmatShared<BS> sh;
-  const int cell = blockIdx.x;
-  moments_cmat_cell<BS>(g, f + (size_t)cell * g.N, props + (size_t)cell * DVM_NPROP, cmat + (size_t)cell * kCmat,
-                        mass, sh);
+                                                     double mass, int ncell) {
+  __shared__ MomCmatShared<C, BS> sh;
+  const int c0 = blockIdx.x * C;
+  const int nc = (ncell - c0 < C) ? ncell - c0 : C;
+  moments_cmat_cells<C, BS>(g, f + (size_t)c0 * g.N, (size_t)g.N, nc, props + (size_t)c0 * DVM_NPROP,
+                            cmat + (size_t)c0 * kCmat, mass, sh);
 }
 
 // Distance of one cell's f from equilibrium: sum |f - M| / n, where M is the
