@@ -141,7 +141,7 @@ __global__ void k_prep_cells(const SlotDesc* __restrict__ slots, const SlotState
 // turn; every workgroup reaches the exit once it has found all kXcds queues
 // drained. A cell's items sit in one queue, so the workgroups that share its
 // CDFs share one L2.
-template <bool kF32, int BS, int K, bool kUni = false, bool kRemap32 = false>
+template <bool kF32, int BS, int K, bool kUni = false, bool kRemap32 = false, bool kFastOnly = false>
 __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict__ slots,
                                                       const SlotState* __restrict__ state,
                                                       const WorkItem* __restrict__ items, CollParams base,
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
       fcs.cm = P.cmat ? P.cmat + gc * kCmat : nullptr;
       fc = &fcs;
     }
-    const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
+    const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32, PhiloxSrc, kFastOnly>(g, ax, cp, cv, w.cell, ntime, w.q0, w.q1, P.counts + gc * 4,
                                               P.factors + gc * 3, win, d.full != 0, d.split == 1, dyn + kAxisSlots,
                                               P.f[st.cur ^ 1] + n0, P.dacc + n0, &csh, PhiloxSrc{cp.seed}, fc);
     if (bad) atomicOr(P.err, bad);
@@ -385,6 +385,7 @@ struct dvm_batch {
   bool all_uniform = false;  // every resident slot's axes equally spaced
   size_t coll_lds = 0;
   bool fast_ok = false;      // the species and precision admit the fast FP32 sampling
+  bool all_fast = false;     // every resident cell takes the fast FP32 sampling
   int fast_lds = 0;          // LDS bytes a cell's slice and sign bytes may take on the fast FP32 path (0: none)
   int persist = 0;
   int bs = 1024, k = 1, item_samples = 0;  // 0: sized from the resident load
@@ -431,7 +432,10 @@ static const void* batch_kernel_k(int k, bool uni, bool r32) {
   return k == 2 ? (const void*)k_collide_slots<kF32, BS, 2> : (const void*)k_collide_slots<kF32, BS, 1>;
 }
 
-static const void* batch_collide_kernel(bool f32, int bs, int k, bool uni, bool r32) {
+static const void* batch_collide_kernel(bool f32, int bs, int k, bool uni, bool r32, bool all_fast = false) {
+  // every item on the fast FP32 path: its own kernel (1024 lanes, one pick per lane)
+  if (all_fast && f32 && uni && !r32 && bs == 1024 && k == 1)
+    return (const void*)k_collide_slots<true, 1024, 1, true, false, true>;
   switch (bs) {
     case 256: return f32 ? batch_kernel_k<true, 256>(k, uni, r32) : batch_kernel_k<false, 256>(k, uni, r32);
     case 512: return f32 ? batch_kernel_k<true, 512>(k, uni, r32) : batch_kernel_k<false, 512>(k, uni, r32);
@@ -520,8 +524,9 @@ static int batch_create_impl(dvm_batch* b, const dvm_config* species, int max_sl
   for (int bs : {256, 512, 768, 1024})
     for (int k : {1, 2})
       for (bool uni : {false, true})
-        BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k, uni, batch_r32(b)),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+        for (bool af : {false, true})
+          BHIP(b, hipFuncSetAttribute(batch_collide_kernel(f32, bs, k, uni, batch_r32(b), af),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   return DVM_OK;
 }
 
@@ -831,18 +836,25 @@ static int batch_lists(dvm_batch* b) {
   // the fast FP32 path also stages the sign bytes of a whole-grid cell after
   // its slice; it applies to the slots for which the two fit
   b->fast_lds = 0;
+  b->all_fast = false;
   if (b->fast_ok && b->all_uniform) {
     size_t need = b->coll_lds;
+    bool all = true;
     for (int s = 0; s < b->max_slots; ++s) {
       const HostSlot& h = b->hs[s];
-      if (!h.active || !b->slots[s].full) continue;
+      if (!h.active) continue;
       const size_t n = (size_t)(kAxisSlots + h.N) * sizeof(double) + 4 * (size_t)((h.N + 31) >> 5);
-      if (n <= kLdsBytes - kStaticLds && n > need) need = n;
+      if (!b->slots[s].full || n > kLdsBytes - kStaticLds) {
+        all = false;
+        continue;
+      }
+      if (n > need) need = n;
     }
     b->coll_lds = need;
     b->fast_lds = (int)need;
+    b->all_fast = all;
   }
-  const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k, b->all_uniform, batch_r32(b));
+  const void* kern = batch_collide_kernel(batch_f32(b), b->bs, b->k, b->all_uniform, batch_r32(b), b->all_fast);
   int per_cu = 0;
   BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, b->bs, b->coll_lds));
   if (per_cu < 1) per_cu = 1;
@@ -879,7 +891,7 @@ static int batch_enqueue_step(dvm_batch* b) {
     BatchPools P = b->P;
     int fl = b->fast_lds;
     void* args[] = {&sl, &st, &it, &cp, &P, &fl};
-    BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k, b->all_uniform, batch_r32(b)), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
+    BHIP(b, hipLaunchKernel(batch_collide_kernel(f32, b->bs, b->k, b->all_uniform, batch_r32(b), b->all_fast), dim3(b->persist), dim3(b->bs), args, b->coll_lds,
                             b->stream));
   }
   if (f32)

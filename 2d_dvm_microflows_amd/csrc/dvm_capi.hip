@@ -262,6 +262,9 @@ static const void* collide_kernel_bs(int k, bool lds, bool f32, bool uni, bool r
 }
 
 static const void* collide_kernel(const dvm_ctx* c) {
+  // the fast FP32 path on its own kernel (1024 lanes; every cell's whole
+  // grid in LDS)
+  if (c->fast && c->coll_bs == 1024 && c->pick_k == 1) return (const void*)k_collide<true, true, 1024, 1, false, true, false, true>;
   const bool f32 = prec_f32(c->cfg.precision);
   const bool uni = c->uniform_axes;
   const bool r32 = c->cfg.precision == DVM_PREC_FP32_REMAP32;

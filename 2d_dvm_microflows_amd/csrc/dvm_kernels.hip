@@ -1655,7 +1655,7 @@ __device__ __forceinline__ int collide_fast(const Grid& g, const Axes<double>& a
 //   FP32, full and sole: the whole change is in LDS, so the FP64
 //         conservation correction runs here (csh its reduction space) and
 //         fd = f + corrected change is written directly.
-template <bool kF32, int BS, int K, bool kUni, bool kRemap32, class Src>
+template <bool kF32, int BS, int K, bool kUni, bool kRemap32, class Src, bool kFastOnly = false>
 __device__ __forceinline__ int collide_item(const Grid& g,
                                             const Axes<typename std::conditional<kRemap32, float, double>::type>& ax,
                                             const CollParams& cp, const CellView& cv, int rng_cell,
@@ -1698,6 +1698,10 @@ __device__ __forceinline__ int collide_item(const Grid& g,
       return bad;
     }
   }
+  // kFastOnly: a kernel whose items all take the fast path (the host checks
+  // that they qualify); the general path is not compiled into it, so its
+  // registers do not weigh on the fast path's
+  if constexpr (kFastOnly) return 0;
   for (int q = threadIdx.x; q < wn; q += BS) acc[q] = 0.0;
   __syncthreads();
   double* tgt = kF32 ? da : fd;
@@ -1768,7 +1772,8 @@ struct WinDims {
 // sper values per sample) instead of Philox.
 // kUni / kRemap32 (FP32 path only): the remap's nearest nodes by rounding on
 // equally spaced axes / the remap in FP32 on FP32 axes.
-template <bool kLds, bool kF32, int BS, int K, bool kStream, bool kUni = false, bool kRemap32 = false>
+template <bool kLds, bool kF32, int BS, int K, bool kStream, bool kUni = false, bool kRemap32 = false,
+          bool kFastOnly = false>
 __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime, int split, WinDims wd,
                                                 const double* __restrict__ props,
                                                 const double* __restrict__ f, double* __restrict__ fd,
@@ -1814,7 +1819,7 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
     fcs.cm = cmat ? cmat + (size_t)cell * kCmat : nullptr;  // used by a cell's only part
     fc = &fcs;
   }
-  const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
+  const int bad = collide_item<kF32, BS, K, kUni, kF32 && kRemap32, Src, kFastOnly>(g, ax, cp, cv, cell, ntime, q0, split == 1 ? 0x7fffffff : q0 + chunk, cnt,
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src, fc);
   if (bad) atomicOr(err, bad);
