@@ -39,10 +39,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell-node of one launch:
 #   sweep     read f once, write the convected f once (FP64)
-#   collide   read f, the cached equilibrium and both CDFs once, write the
-#             result (fd or the FP32 path's FP64 change) once
+#   collide   FP64: read f, the cached equilibrium (the deviation signs) and
+#             both CDFs once, write f + change once; FP32 (fast path): read f,
+#             both CDFs and the deviation sign bit once, write the corrected
+#             f + change once
 SWEEP_BYTES_PER_NODE = 2 * 8
-COLLIDE_BYTES_PER_NODE = 5 * 8
+COLLIDE_BYTES_PER_NODE = {"fp64": 5 * 8, "fp32": 4 * 8 + 1 / 8}
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
 VALU_TABLE = os.path.join(ROOT, "profiles", "valu_per_launch.txt")
 SIMDS = 1024        # MI355X: 256 CUs x 4 SIMDs
@@ -568,7 +570,7 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         # the collision kernel is the dominant one; it is bound by VALU work and
         # LDS atomics per sample rather than by HBM, so its HBM fraction is low
-        "roofline": roofline("k_collide", workload, COLLIDE_BYTES_PER_NODE * ncell * N, phase["collide"]),
+        "roofline": roofline("k_collide", workload, COLLIDE_BYTES_PER_NODE[head] * ncell * N, phase["collide"]),
         "roofline_sweep": roofline("k_convect2d", workload, SWEEP_BYTES_PER_NODE * ncell * N, phase["sweep"]),
     }
     copy_gbs = copy_bandwidth()
