@@ -179,11 +179,13 @@ __global__ void __launch_bounds__(BS) k_collide_slots(const SlotDesc* __restrict
                                   P.tabs + gc * 2 * kNBucket, N);
     Window win{};
     if (!d.full) win = make_window(g, d.wd, P.props + gc * DVM_NPROP);
-    // the fast FP32 sampling for a cell whose whole slice and sign bytes fit
-    // the launch's LDS (fast_lds bytes; 0 when the batch has no such path)
+    // the fast FP32 sampling for a cell whose slice (or window) and sign
+    // words fit the launch's LDS (fast_lds bytes; 0 when the batch has no
+    // such path)
     FastCell fcs{};
     const FastCell* fc = nullptr;
-    if (d.full && fast_lds > 0 && (long long)(kAxisSlots + N) * 8 + 4 * ((N + 31) >> 5) <= (long long)fast_lds) {
+    const int wn = d.full ? N : d.wd[0] * d.wd[1] * d.wd[2];
+    if (fast_lds > 0 && (long long)(kAxisSlots + wn) * 8 + 4 * ((N + 31) >> 5) <= (long long)fast_lds) {
       const size_t cs = (size_t)d.coff + (size_t)w.cell * 2 * (N + 1);
       fcs.sgn = P.sgn + cs;
       fcs.cm = P.cmat ? P.cmat + gc * kCmat : nullptr;
@@ -638,7 +640,10 @@ extern "C" int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const doubl
   d.seed = inst->seed;
   d.cell0 = (int)c0;
   d.ncell = nc;
-  d.full = window_dims(hg.g.n, d.wd) ? 1 : 0;
+  // a slot on the fast FP32 sampling keeps room for its sign words beside
+  // its window
+  const int N = hg.g.N;
+  d.full = window_dims(hg.g.n, d.wd, (b->fast_ok && hg.uniform) ? fast_window_cap(N) : kWindowCap) ? 1 : 0;
   d.split = 1;
   // a convecting slot's sweep brings f back into the pool the step started
   // from, so its ping-pong index stays 0 (pool 0 holds f, pool 1 the
@@ -648,7 +653,6 @@ extern "C" int dvm_batch_admit(dvm_batch* b, const dvm_config* inst, const doubl
   d.bc[0] = inst->bc_left;
   d.bc[1] = inst->bc_right;
   d.order = inst->conv_order == 2 ? 2 : 1;
-  const int N = hg.g.N;
   HostSlot fresh{};
   if (inst->dim == DVM_DIM_2D && d.convect) {
     fresh.conv2d = true;
@@ -833,8 +837,8 @@ static int batch_lists(dvm_batch* b) {
   b->n_active0 = (int)act0.size();
   b->n_items = (int)wi.size();
   b->coll_lds = (size_t)(kAxisSlots + lds_n) * sizeof(double);
-  // the fast FP32 path also stages the sign bytes of a whole-grid cell after
-  // its slice; it applies to the slots for which the two fit
+  // the fast FP32 path also stages a cell's sign words after its slice or
+  // window; it applies to the slots for which the two fit
   b->fast_lds = 0;
   b->all_fast = false;
   if (b->fast_ok && b->all_uniform) {
@@ -843,8 +847,9 @@ static int batch_lists(dvm_batch* b) {
     for (int s = 0; s < b->max_slots; ++s) {
       const HostSlot& h = b->hs[s];
       if (!h.active) continue;
-      const size_t n = (size_t)(kAxisSlots + h.N) * sizeof(double) + 4 * (size_t)((h.N + 31) >> 5);
-      if (!b->slots[s].full || n > kLdsBytes - kStaticLds) {
+      const int* w = b->slots[s].wd;
+      const size_t n = (size_t)(kAxisSlots + w[0] * w[1] * w[2]) * sizeof(double) + (size_t)sign_lds_bytes(h.N);
+      if (n > kLdsBytes - kStaticLds) {
         all = false;
         continue;
       }

@@ -262,9 +262,10 @@ static const void* collide_kernel_bs(int k, bool lds, bool f32, bool uni, bool r
 }
 
 static const void* collide_kernel(const dvm_ctx* c) {
-  // the fast FP32 path on its own kernel (1024 lanes; every cell's whole
-  // grid in LDS)
-  if (c->fast && c->coll_bs == 1024 && c->pick_k == 1) return (const void*)k_collide<true, true, 1024, 1, false, true, false, true>;
+  // the fast FP32 path on kernels of its own (1024 lanes, one pick per lane)
+  if (c->fast && c->coll_bs == 1024 && c->pick_k == 1)
+    return c->lds_collide ? (const void*)k_collide<true, true, 1024, 1, false, true, false, true>
+                          : (const void*)k_collide<false, true, 1024, 1, false, true, false, true>;
   const bool f32 = prec_f32(c->cfg.precision);
   const bool uni = c->uniform_axes;
   const bool r32 = c->cfg.precision == DVM_PREC_FP32_REMAP32;
@@ -285,24 +286,30 @@ constexpr int kWindowCap = (int)((kLdsBytes - kStaticLds) / sizeof(double)) - kA
 // Extents of the collision LDS window for a grid of n[0] x n[1] x n[2] nodes:
 // the grid itself when it fits (returns true), else the largest box, near
 // cubic, within kWindowCap nodes.
-static bool window_dims(const int n[3], int w[3]) {
+static bool window_dims(const int n[3], int w[3], int cap = kWindowCap) {
   const long long N = (long long)n[0] * n[1] * n[2];
-  if (N <= kWindowCap) {
+  if (N <= cap) {
     for (int d = 0; d < 3; ++d) w[d] = n[d];
     return true;
   }
   int e = 1;
-  while ((long long)(e + 1) * (e + 1) * (e + 1) <= kWindowCap) ++e;
+  while ((long long)(e + 1) * (e + 1) * (e + 1) <= cap) ++e;
   for (int d = 0; d < 3; ++d) w[d] = n[d] < e ? n[d] : e;
   for (bool grew = true; grew;) {
     grew = false;
     for (int d = 0; d < 3; ++d) {
       const long long rest = (long long)w[(d + 1) % 3] * w[(d + 2) % 3];
-      if (w[d] < n[d] && (w[d] + 1) * rest <= kWindowCap) { ++w[d]; grew = true; }
+      if (w[d] < n[d] && (w[d] + 1) * rest <= cap) { ++w[d]; grew = true; }
     }
   }
   return false;
 }
+
+// Bytes of a cell's deviation sign words in LDS (the fast FP32 sampling).
+static int sign_lds_bytes(long long N) { return 4 * (int)((N + 31) >> 5); }
+
+// The LDS window cap of a fast FP32 collision, whose sign words share the LDS.
+static int fast_window_cap(long long N) { return kWindowCap - (sign_lds_bytes(N) + 7) / 8; }
 
 // Host copy of one velocity grid (velocity_grid.f90:271-597): axes, the
 // trapezoid weights beta, the zero points, and the packed device image
@@ -512,7 +519,11 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   HIP_TRY(c, hipMemset(c->eqv, 0, CN * sizeof(double)));
   // collision accumulator placement: the cell's slice in LDS when it fits
   const bool f32 = prec_f32(cfg->precision);
-  c->lds_collide = window_dims(n, c->wd.w);
+  // the fast FP32 sampling (collide_fast): equally spaced axes, the FP64
+  // remap, isotropic scattering; its sign words share the LDS with the
+  // window
+  const bool fast_ok = cfg->precision == DVM_PREC_FP32 && c->uniform_axes && !c->collp.vss;
+  c->lds_collide = window_dims(n, c->wd.w, fast_ok ? fast_window_cap(N) : kWindowCap);
   // FP64 with an LDS window (the grid exceeds LDS): 768 lanes, where the FP64
   // sampling code's registers spill less than at the 128-register cap of 1024
   // lanes (1-D shock, 31^3: collide 0.933 ms against 0.966; the cavity's
@@ -530,17 +541,11 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   if (f32) {
     if (int rc2 = dalloc(c, &c->dacc, CN)) return rc2;
   }
-  // The fast FP32 collision (collide_fast): equally spaced axes, the FP64
-  // remap, isotropic scattering, the whole cell's slice and its sign bytes in
-  // LDS.
-  {
-    const size_t need = c->coll_lds + 4 * (size_t)((N + 31) >> 5);  // sign words after the slice
-    c->fast = (cfg->precision == DVM_PREC_FP32 && c->uniform_axes && c->lds_collide && !c->collp.vss &&
-               need <= kLdsBytes - kStaticLds) ? 1 : 0;
-    if (c->fast) {
-      c->coll_lds = need;
+  if (fast_ok) {
+    c->coll_lds += (size_t)sign_lds_bytes(N);  // sign words after the slice or window
+    c->fast = 1;
+    if (c->lds_collide)
       if (int rc2 = dalloc(c, &c->cmat, (size_t)c->ncell * kCmat)) return rc2;
-    }
   }
   const void* kern = collide_kernel(c);
   HIP_TRY(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));

@@ -1434,9 +1434,9 @@ __device__ __forceinline__ Window make_window(const Grid& g, const int* wd, cons
   return win;
 }
 
-// Inputs of the FP32 fast collision of a whole cell (collide_fast), beside
+// Inputs of the FP32 fast collision of a cell (collide_fast), beside
 // the cell view: its deviation sign bytes staged in LDS and the Cholesky
-// factor of sum |f| phi phi^T from the CDF pass.
+// factor of sum |f| phi phi^T from the moments pass.
 struct FastCell {
   const unsigned char* sgn;  // bit l % 8 of byte l / 8 set when f - f_eq < 0 (or -0); staged into LDS
   const double* cm;          // kCmat doubles: factor, then the not-positive-definite flag
@@ -1529,8 +1529,9 @@ __device__ __forceinline__ int conserve_cell_factored(const Grid& g, const doubl
 }
 
 // The FP32-collision sampling of samples [q0, q1) of one cell on equally
-// spaced axes with isotropic scattering, accumulating the FP64 change in the
-// cell's LDS slice acc (the whole grid): variance_reduction.f90:686-1116 with
+// spaced axes with isotropic scattering, accumulating the FP64 change through
+// scat (the cell's LDS slice, or its LDS window and global atomics beyond
+// it): variance_reduction.f90:686-1116 with
 // vr_depletion_routine and the elastic replenish_collision, as vr_sample, in a
 // leaner form for this case:
 //   * one Philox block per sample (plus one per further replenish location)
@@ -1544,11 +1545,11 @@ __device__ __forceinline__ int conserve_cell_factored(const Grid& g, const doubl
 //     remap_t at the ends.
 // Returns device error bits (kErrRemap: a pick outside the grid or a remap
 // without an exterior point).
-template <int BS>
+template <int BS, class Scatter>
 __device__ __forceinline__ int collide_fast(const Grid& g, const Axes<double>& ax, const CollParams& cp,
                                             const CellView& cv, const unsigned char* sgn, int cell, int ntime,
                                             int q0, int q1, const int* __restrict__ cnt,
-                                            const double* __restrict__ fac3, double* acc) {
+                                            const double* __restrict__ fac3, Scatter&& scat) {
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
   const int n0 = cnt[0], n1 = cnt[1];
   const float inv_nxy = (float)g.inv_nxy, inv_nx = (float)g.inv_n0;
@@ -1557,7 +1558,6 @@ __device__ __forceinline__ int collide_fast(const Grid& g, const Axes<double>& a
   const float fac0 = (float)fac3[0], fac1 = (float)fac3[1], fac2 = (float)fac3[2];
   const float vexp = (float)cp.vhs_exponent;
   const float two_pi = (float)(2.0 * kPiRef);
-  auto scat = [&](int, int, int, int node, double v) { atomicAdd(&acc[node], v); };
   int bad = 0;
   for (int base = q0; base < q1; base += BS) {
     const int q = base + (int)threadIdx.x;
@@ -1632,8 +1632,10 @@ __device__ __forceinline__ int collide_fast(const Grid& g, const Axes<double>& a
 #pragma unroll
         for (int m = 0; m < 7; ++m) {
           if (m < 4 || (st.flag & (1 << (m - 4)))) {
-            const int node = stencil_node(st, m, nx, nxy);
-            if ((unsigned)node < (unsigned)N) atomicAdd(&acc[node], (double)kr * st.m[m]);
+            int si, sj, sk;
+            stencil_ijk(st, m, si, sj, sk);
+            const int node = si + nx * sj + nxy * sk;
+            if ((unsigned)node < (unsigned)N) scat(si, sj, sk, node, (double)kr * st.m[m]);
           }
         }
       }
@@ -1669,12 +1671,12 @@ __device__ __forceinline__ int collide_item(const Grid& g,
   q1 = q1 < total ? q1 : total;
   const int wn = full ? N : win.wn;
   if constexpr (kF32 && kUni && !kRemap32 && std::is_same<Src, PhiloxSrc>::value) {
-    if (fast && full) {
-      // the cell's whole slice in LDS, its sign bytes after it, copied as
-      // 32-bit words (a cell's sign bytes start 4-byte aligned, 2 (N+1) bytes
-      // per cell, and the words cover at most 3 bytes past them, inside the
-      // cell's slot); the words are loaded before the slice is cleared
-      unsigned char* sl = reinterpret_cast<unsigned char*>(acc + N);
+    if (fast) {
+      // the cell's slice (or window) in LDS, its sign bytes after it, copied
+      // as 32-bit words (a cell's sign bytes start 4-byte aligned, 2 (N+1)
+      // bytes per cell, and the words cover at most 3 bytes past them, inside
+      // the cell's slot); the words are loaded before the slice is cleared
+      unsigned char* sl = reinterpret_cast<unsigned char*>(acc + wn);
       const int nw = (N + 31) >> 5;
       const unsigned* sw = reinterpret_cast<const unsigned*>(fast->sgn);
       unsigned* lw = reinterpret_cast<unsigned*>(sl);
@@ -1684,16 +1686,41 @@ __device__ __forceinline__ int collide_item(const Grid& g,
       if ((int)threadIdx.x < nw) lw[threadIdx.x] = w0;
       for (int q = threadIdx.x + BS; q < nw; q += BS) lw[q] = sw[q];
       __syncthreads();
-      const int bad = collide_fast<BS>(g, ax, cp, cv, sl, rng_cell, ntime, q0, q1, cnt, fac, acc);
+      int bad;
+      if (full) {
+        bad = collide_fast<BS>(g, ax, cp, cv, sl, rng_cell, ntime, q0, q1, cnt, fac,
+                               [&](int, int, int, int node, double v) { atomicAdd(&acc[node], v); });
+      } else {
+        // the window in LDS, the rest of the grid to global FP64 atomics on
+        // dacc (zeroed for the cell)
+        const int o0 = win.o[0], o1 = win.o[1], o2 = win.o[2], wx = win.w[0], wxy = win.wxy;
+        const unsigned w0u = win.w[0], w1u = win.w[1], w2u = win.w[2];
+        bad = collide_fast<BS>(g, ax, cp, cv, sl, rng_cell, ntime, q0, q1, cnt, fac,
+                               [&](int i, int j, int k, int node, double v) {
+                                 const unsigned a = i - o0, b = j - o1, c = k - o2;
+                                 if (a < w0u && b < w1u && c < w2u) atomicAdd(&acc[a + wx * b + wxy * c], v);
+                                 else atomicAdd(&da[node], v);
+                               });
+      }
       __syncthreads();
-      // the cell's only item corrects conservation here; the parts of a
-      // split cell add their slices to dacc for the separate correction
-      if (sole)
+      // the cell's only item with the whole grid in LDS corrects conservation
+      // here; otherwise the window (a cell's only item: stored, the nodes
+      // outside it having taken atomics; a part of a split cell: added) goes
+      // to dacc for the separate correction
+      if (full && sole)
         return bad | (fast->cm ? conserve_cell_factored<BS>(g, cv.f, acc, fast->cm, fd, *csh)
                                : conserve_cell<BS>(g, cv.f, acc, fd, *csh));
-      for (int q = threadIdx.x; q < N; q += BS) {
+      const int nx = g.n[0], nxy = g.nxy;
+      for (int q = threadIdx.x; q < wn; q += BS) {
+        int l = q;
+        if (!full) {
+          const int c = q / win.wxy, r = q - c * win.wxy;
+          const int b = r / win.w[0], a = r - b * win.w[0];
+          l = (win.o[0] + a) + nx * (win.o[1] + b) + nxy * (win.o[2] + c);
+        }
         const double v = acc[q];
-        if (v != 0.0) atomicAdd(&da[q], v);
+        if (sole) da[l] = v;
+        else if (v != 0.0) atomicAdd(&da[l], v);
       }
       return bad;
     }
@@ -1809,12 +1836,12 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
   using Src = typename std::conditional<kStream, StreamSrc, PhiloxSrc>::type;
   Src src;
   if constexpr (kStream) src = StreamSrc{su, sbase, sper}; else src = PhiloxSrc{cp.seed};
-  // FP32 on equally spaced axes with the whole grid in LDS: the fast
-  // sampling path with the sign bytes (and, for a cell's only part, the
+  // FP32 on equally spaced axes: the fast sampling path with the sign bytes
+  // (and, for a cell's only part with the whole grid in LDS, the
   // precomputed conservation factor)
   FastCell fcs{};
   const FastCell* fc = nullptr;
-  if (kLds && fast) {
+  if (fast) {
     fcs.sgn = sgn + (size_t)cell * 2 * (N + 1);
     fcs.cm = cmat ? cmat + (size_t)cell * kCmat : nullptr;  // used by a cell's only part
     fc = &fcs;
