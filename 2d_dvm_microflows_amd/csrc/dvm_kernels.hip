@@ -1443,7 +1443,7 @@ struct FastCell {
 };
 
 // Conservation correction of a cell's FP32 collision change d (LDS) with the
-// Cholesky factor L of M = sum |f| phi phi^T precomputed by the CDF pass:
+// Cholesky factor L of M = sum |f| phi phi^T precomputed by the moments pass:
 // lambda solves L L^T lambda = sum d phi, and out = f + (d - |f| lambda.phi),
 // as conserve_cell but with one pass of five sums over d instead of twenty
 // over d and f.
