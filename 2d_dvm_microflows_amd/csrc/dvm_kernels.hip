@@ -1016,15 +1016,17 @@ __device__ __forceinline__ void pick_batch(const CollParams& cp, const CellView&
     bool any = false;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
+      // each select is driven by compares of this step only (no boolean
+      // carried between steps), which keeps the masks in scalar registers
       const int id = st[j];
       const bool act = id >= 0;
       const bool gt = adj[j] > c0[j];
-      const bool go = act && !(gt && adj[j] <= c1[j]);
-      left[j] = (go && gt) ? id : left[j];
-      right[j] = (go && !gt) ? id : right[j];
+      const bool le = adj[j] <= c1[j];
+      left[j] = (act && gt && !le) ? id : left[j];
+      right[j] = (act && !gt) ? id : right[j];
       const int nid = (int)((unsigned)(left[j] + right[j]) >> 1);
-      st[j] = go ? nid : (act ? ~id : id);
-      any = any || go;
+      st[j] = act ? ((gt && le) ? ~id : nid) : id;
+      any = any || (act && !(gt && le));
     }
     if (!any) break;
   }
