@@ -34,6 +34,29 @@ def test_fp32_collision_conserves_to_roundoff(pkg, case):
             assert np.all(np.abs(b - a) <= 1e-13 * scale), (n, (b - a) / scale)
 
 
+@pytest.mark.parametrize("n_v", [21, 29])
+def test_fp32_collision_matches_fp64_distribution(pkg, n_v):
+    """One collision of the FP32 path against the FP64 path from the same f and
+    seed, node by node: the L1 difference of the post-collision distributions
+    stays at FP32 rounding (the picks coincide). 21^3: the whole grid in LDS;
+    29^3: an LDS window around the bulk velocity with the rest of the grid
+    taking global atomics, which the conservation test alone would not see
+    (the correction restores the conserved moments of any scatter)."""
+    cfg, f = pkg.homogeneous_relaxation(n_v=n_v, samples=40000)
+    print(f"[fp32 vs fp64 distribution] n_v {n_v}", flush=True)
+    out = {}
+    for prec in (0, FP32):
+        with pkg.Solver(cfg.with_(precision=prec)) as s:
+            s.initialize(f)
+            s.compute_collision_integral(1)
+            out[prec] = s.download_f()
+    d = np.abs(out[FP32] - out[0]).sum() / np.abs(out[0]).sum()
+    change = np.abs(out[0] - f).sum() / np.abs(out[0]).sum()
+    print(f"  L1 difference {d:.2e} of a change of {change:.2e}", flush=True)
+    assert change > 1e-3, change
+    assert d <= 1e-6, d
+
+
 def _moment_err(pa, pb):
     """Max relative error of density, velocity (in units of sqrt(T)) and T."""
     th = np.sqrt(np.abs(pa[:, 4]))
