@@ -54,8 +54,10 @@ struct dvm_ctx {
   double* d_heat = nullptr;      // [4][max(nx,ny)] wall heat fluxes
   double* dacc = nullptr;        // [C][N] collision change of the FP32 path
   unsigned char* sgn = nullptr;  // [C][2 (N+1)] bytes: deviation sign bits (ceil(N/8) bytes used per cell)
-  double* cmat = nullptr;        // [C][kCmat] FP32 path: conservation factor of the CDF pass
+  double* cmat = nullptr;        // [C][kCmat] FP32 path: conservation factor of the moments pass
   int fast = 0;                  // the FP32 collision takes collide_fast (see collide_async)
+  unsigned* tick = nullptr;      // [2] ticket and finish counters of k_collide_fast_cells (zero between launches)
+  int fast_wgs = 0;              // workgroups of k_collide_fast_cells (0: not used)
   bool cmat_valid = false;       // cmat was computed from the current f (by moments_async)
   // explicit uniform list (dvm_set_uniform_stream): su_n values, read from
   // su_cursor on; sbase [C] per-cell offsets of the current collision
@@ -210,7 +212,7 @@ extern "C" void dvm_ctx_destroy(dvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_heat, c->d_axes, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->eqpar, c->props,
                   c->factors, c->counts, c->err, c->wallM, c->ratio, c->spec, c->inflow, c->dacc, c->sgn, c->cmat,
-                  c->d_su, c->d_sbase, c->d_stotal};
+                  c->tick, c->d_su, c->d_sbase, c->d_stotal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& row : c->ev)
@@ -594,6 +596,21 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
       if (v >= 1 && v <= 1024) c->split = v;
     }
   }
+  // The fast FP32 collision of whole cells on persistent workgroups, one per
+  // CU (k_collide_fast_cells): cells each one part, the grid in LDS, 1024
+  // lanes and one pick per lane. Its ticket counters start at zero here and
+  // the kernel leaves them at zero.
+  if (c->fast && c->lds_collide && c->split == 1 && c->coll_bs == kCollBS && c->pick_k == 1) {
+    const void* pk = (const void*)k_collide_fast_cells<kCollBS>;
+    HIP_TRY(c, hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk, kCollBS, c->coll_lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    const long long wgs = (long long)per_cu * c->cus;
+    c->fast_wgs = (int)(wgs < c->ncell ? wgs : c->ncell);
+    if (int rc2 = dalloc(c, &c->tick, 2)) return rc2;
+    HIP_TRY(c, hipMemset(c->tick, 0, 2 * sizeof(unsigned)));
+  }
   // ---- convection constants
   ConvParams& p = c->convp;
   conv_params(cfg, hg, p);
@@ -845,12 +862,19 @@ static int collide_async(dvm_ctx* c, int ntime) {
   void* args[] = {&c->g,    &c->collp, &ntime,  &c->split,  &c->wd,      &c->props, &c->f, &c->fd, &c->dacc,
                   &c->eqv,  &c->cdf,   &c->tabs, &c->counts, &c->factors, &c->err,   &su,   &sbase, &per,
                   &c->sgn,  &cm,       &fast};
-  if (stream)
+  if (stream) {
     HIP_TRY(c, hipLaunchKernel(collide_stream_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(kCollBS), args,
                                c->coll_lds, c->stream));
-  else
+  } else if (fast && cm && c->fast_wgs > 0) {
+    // whole cells on persistent workgroups (the factor is that of this f)
+    hipLaunchKernelGGL(k_collide_fast_cells<kCollBS>, dim3((unsigned)c->fast_wgs), dim3(kCollBS), c->coll_lds,
+                       c->stream, c->g, c->collp, ntime, c->ncell, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->counts,
+                       c->factors, c->err, c->sgn, cm, c->tick);
+    HIP_TRY(c, hipGetLastError());
+  } else {
     HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(c->coll_bs), args,
                                c->coll_lds, c->stream));
+  }
   if (!separate_conserve(c)) {
     double* t = c->f; c->f = c->fd; c->fd = t;  // phi := delta_phi
   }

@@ -1449,10 +1449,17 @@ struct FastCell {
 // lambda solves L L^T lambda = sum d phi, and out = f + (d - |f| lambda.phi),
 // as conserve_cell but with one pass of five sums over d instead of twenty
 // over d and f.
-template <int BS>
+// kClear: the change is cleared as it is read by the write (each lane clears
+// what it reads), leaving the slice zeroed for the next cell. mid() runs on
+// every lane between the sums and the solve (the persistent kernel's hook).
+struct NoHook {
+  __device__ void operator()() const {}
+};
+template <int BS, bool kClear = false, class Hook = NoHook>
 __device__ __forceinline__ int conserve_cell_factored(const Grid& g, const double* __restrict__ fc,
-                                                      const double* __restrict__ da, const double* __restrict__ cm,
-                                                      double* __restrict__ out, ConserveShared<BS>& sh) {
+                                                      double* __restrict__ da, const double* __restrict__ cm,
+                                                      double* __restrict__ out, ConserveShared<BS>& sh,
+                                                      Hook&& mid = Hook{}) {
   const int N = g.N, nx = g.n[0], nxy = g.nxy;
   const float inv_nxy = (float)g.inv_nxy, inv_nx = (float)g.inv_n0;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1476,6 +1483,7 @@ __device__ __forceinline__ int conserve_cell_factored(const Grid& g, const doubl
     if (lane == 0) sh.red[wid][z] = v;
   }
   __syncthreads();
+  mid();
   if (threadIdx.x == 0) {
     double y[5];
 #pragma unroll
@@ -1525,6 +1533,7 @@ __device__ __forceinline__ int conserve_cell_factored(const Grid& g, const doubl
       const double fl = fv[u];
       const double corr = fabs(fl) * (l0 + l1 * x + l2 * y + l3 * z + l4 * (x * x + y * y + z * z));
       out[l] = fl + (da[l] - corr);
+      if constexpr (kClear) da[l] = 0.0;
     }
   }
   return flag;
@@ -1852,6 +1861,80 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
                                             factors + cell * 3, win, kLds, split == 1, dyn + kAxisSlots,
                                             fd + (size_t)cell * N, dacc + (size_t)cell * N, &csh, src, fc);
   if (bad) atomicOr(err, bad);
+}
+
+// The FP32 fast collision of whole cells (each its one part, the grid in
+// LDS, the conservation factor valid) on persistent workgroups, one per CU:
+// a workgroup stages the axes once and takes cells from a ticket counter
+// until they run out, and each cell's epilogue overlaps the next cell's
+// prologue: the next ticket is taken and that cell's sign words are loaded
+// while the correction runs, and the slice is cleared as the corrected
+// values are written. tick[0] is the ticket counter and tick[1] counts the
+// workgroups that have finished; the last one resets both to zero, so every
+// launch starts from zero without a memset.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_collide_fast_cells(Grid g, CollParams cp, int ntime, int ncell,
+                                                           const double* __restrict__ f, double* __restrict__ fd,
+                                                           const double* __restrict__ eqv,
+                                                           const double* __restrict__ cdf,
+                                                           const int2* __restrict__ tabs,
+                                                           const int* __restrict__ counts,
+                                                           const double* __restrict__ factors, int* __restrict__ err,
+                                                           const unsigned char* __restrict__ sgn,
+                                                           const double* __restrict__ cmat,
+                                                           unsigned* __restrict__ tick) {
+  extern __shared__ double dyn[];
+  __shared__ ConserveShared<BS> csh;
+  __shared__ int s_cell;
+  const Axes<double> ax = stage_collision_axes<double>(g, dyn);
+  const int N = g.N;
+  double* acc = dyn + kAxisSlots;
+  unsigned* lw = reinterpret_cast<unsigned*>(acc + N);
+  const int nw = (N + 31) >> 5;  // sign words of a cell (at most BS: the slice fits LDS)
+  const auto words = [&](int c) {
+    return reinterpret_cast<const unsigned*>(sgn + (size_t)c * 2 * (N + 1));
+  };
+  if (threadIdx.x == 0) s_cell = (int)atomicAdd(tick, 1u);
+  __syncthreads();
+  int cell = s_cell;
+  if (cell < ncell) {
+    const unsigned w0 = ((int)threadIdx.x < nw) ? words(cell)[threadIdx.x] : 0u;
+    for (int q = threadIdx.x; q < N; q += BS) acc[q] = 0.0;
+    if ((int)threadIdx.x < nw) lw[threadIdx.x] = w0;
+  }
+  __syncthreads();
+  int bad = 0;
+  while (cell < ncell) {
+    const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
+                                  tabs + (size_t)cell * 2 * kNBucket, N);
+    const int* cnt = counts + cell * 4;
+    const int total = cnt[0] + cnt[1] + cnt[2];
+    bad |= collide_fast<BS>(g, ax, cp, cv, reinterpret_cast<const unsigned char*>(lw), cell, ntime, 0, total, cnt,
+                            factors + cell * 3,
+                            [&](int, int, int, int node, double v) { atomicAdd(&acc[node], v); });
+    if (threadIdx.x == 0) s_cell = (int)atomicAdd(tick, 1u);
+    __syncthreads();  // the change is complete; the sign words are no longer read
+    int next = ncell;
+    unsigned wn = 0u;
+    bad |= conserve_cell_factored<BS, true>(g, cv.f, acc, cmat + (size_t)cell * kCmat, fd + (size_t)cell * N, csh,
+                                            [&] {
+                                              next = s_cell;
+                                              if (next < ncell && (int)threadIdx.x < nw)
+                                                wn = words(next)[threadIdx.x];
+                                            });
+    if ((int)threadIdx.x < nw) lw[threadIdx.x] = wn;
+    __syncthreads();  // the slice is cleared and the next cell's words staged
+    cell = next;
+  }
+  if (bad) atomicOr(err, bad);
+  // the last workgroup to finish resets the counters for the next launch
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(tick + 1, 1u) == gridDim.x - 1) {
+      atomicExch(tick, 0u);
+      atomicExch(tick + 1, 0u);
+    }
+  }
 }
 
 template <int BS>
