@@ -57,6 +57,29 @@ def test_fp32_collision_matches_fp64_distribution(pkg, n_v):
     assert d <= 1e-6, d
 
 
+def test_fp32_persistent_cells_match_fp64(pkg):
+    """A cavity with enough cells for one workgroup per cell (576 cells, so
+    the FP32 collision runs whole cells on persistent workgroups with the
+    moments pass's conservation factor): one collision against the FP64 one
+    from the same f and seed, node by node, and mass, momentum and energy of
+    every cell conserved to round-off."""
+    cfg, f = pkg.lid_driven_cavity(nx=24, ny=24, n_v=9, vmax=4.0, samples=2000)
+    print(f"[fp32 persistent cells] {cfg.name}", flush=True)
+    v = pkg.node_velocities(cfg)
+    out = {}
+    for prec in (0, FP32):
+        with pkg.Solver(cfg.with_(precision=prec)) as s:
+            s.initialize(f)
+            s.compute_collision_integral(1)
+            out[prec] = s.download_f()
+    d = np.abs(out[FP32] - out[0]).sum() / np.abs(out[0]).sum()
+    print(f"  L1 difference {d:.2e}", flush=True)
+    assert d <= 1e-6, d
+    a, b = cons(f, v), cons(out[FP32], v)
+    scale = np.abs(a) + cons(np.abs(out[FP32]), np.abs(v))
+    assert np.all(np.abs(b - a) <= 1e-13 * scale), np.max(np.abs(b - a) / scale)
+
+
 def _moment_err(pa, pb):
     """Max relative error of density, velocity (in units of sqrt(T)) and T."""
     th = np.sqrt(np.abs(pa[:, 4]))
