@@ -34,7 +34,7 @@ def test_fp32_collision_conserves_to_roundoff(pkg, case):
             assert np.all(np.abs(b - a) <= 1e-13 * scale), (n, (b - a) / scale)
 
 
-@pytest.mark.parametrize("case", ["homogeneous21", "homogeneous29", "cavity10"])
+@pytest.mark.parametrize("case", ["homogeneous21", "homogeneous29", "cavity10", "vhs21", "nrepl21"])
 def test_fp32_collision_matches_fp64_distribution(pkg, case):
     """One collision of the FP32 path against the FP64 path from the same f and
     seed, node by node: the L1 difference of the post-collision distributions
@@ -43,11 +43,16 @@ def test_fp32_collision_matches_fp64_distribution(pkg, case):
     taking global atomics, which the conservation test alone would not see
     (the correction restores the conserved moments of any scatter); a cavity
     on 10^3 nodes: an even node count, where every other cell's sign bytes
-    start 2 bytes off a 4-byte boundary."""
+    start 2 bytes off a 4-byte boundary; VHS (g^(2 - 2 omega) with omega 0.81)
+    and two replenish locations per sample, each with its own Philox block."""
     if case == "cavity10":
         cfg, f = pkg.lid_driven_cavity(nx=4, ny=4, n_v=10, vmax=4.0, samples=20000)
     else:
         cfg, f = pkg.homogeneous_relaxation(n_v=int(case[-2:]), samples=40000)
+        if case.startswith("vhs"):
+            cfg = cfg.with_(omega=0.81)
+        elif case.startswith("nrepl"):
+            cfg = cfg.with_(num_repl=2)
     print(f"[fp32 vs fp64 distribution] {cfg.name}", flush=True)
     out = {}
     for prec in (0, FP32):
