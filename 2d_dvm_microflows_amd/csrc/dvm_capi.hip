@@ -799,19 +799,24 @@ static int moments_async(dvm_ctx* c) {
   return DVM_OK;
 }
 
-static int cdf_async(dvm_ctx* c) {
+// refresh: the pass leaves f_eq stored for a reader of it (the trace and
+// uniform-stream kernels sample with the general code, which reads f_eq).
+static int cdf_async(dvm_ctx* c, bool refresh = false) {
   if (!collides(c)) return DVM_OK;
   // few cells (fewer than four workgroups per CU): 1024 lanes per cell
   if (c->ncell >= 4 * c->cus) {
     const int cap = cdf_sum_cap(c->g.N, kCdfV);
+    // the fast FP32 collision reads the sign bits, never f_eq: the pass then
+    // stores no f_eq (and computes it again on a cache hit)
+    const int eq_mode = !c->fast ? 1 : ((refresh || c->d_su) ? 2 : 0);
     hipLaunchKernelGGL((k_cdf<kCdfCtxBS, kCdfV, true>), dim3(c->ncell), dim3(kCdfCtxBS), cap * sizeof(float), c->stream,
                        c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts, c->factors, cap,
-                       c->sgn);
+                       c->sgn, eq_mode);
   } else {
     const int cap = cdf_sum_cap(c->g.N, kCdfWideV);
     hipLaunchKernelGGL((k_cdf<kCdfWideBS, kCdfWideV, false>), dim3(c->ncell), dim3(kCdfWideBS), cap * sizeof(float),
                        c->stream, c->g, c->cdfp, c->f, c->props, c->eqv, c->cdf, c->tabs, c->eqpar, c->counts,
-                       c->factors, cap, c->sgn);
+                       c->factors, cap, c->sgn, 1);
   }
   HIP_TRY(c, hipGetLastError());
   return DVM_OK;
@@ -1161,7 +1166,7 @@ extern "C" int dvm_collide_trace(dvm_ctx* c, int32_t ntime, int32_t cell, int64_
   // collide would (same f, same moments), so the trace leaves later steps
   // unchanged.
   if (!collides(c)) { *n_out = 0; return DVM_OK; }
-  if (int rc = cdf_async(c)) return rc;
+  if (int rc = cdf_async(c, true)) return rc;
   int h[4];
   HIP_TRY(c, hipMemcpyAsync(h, c->counts + (size_t)cell * 4, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));

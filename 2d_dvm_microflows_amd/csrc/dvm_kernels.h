@@ -102,7 +102,7 @@ __global__ void k_cdf(Grid g, CdfParams cp, const double* __restrict__ f, const 
                       double* __restrict__ eqv, double* __restrict__ cdf, int2* __restrict__ tabs,
                       double* __restrict__ eqpar,
                       int* __restrict__ counts, double* __restrict__ factors, int sum_cap,
-                      unsigned char* __restrict__ sgn);
+                      unsigned char* __restrict__ sgn, int eq_mode);
 template <int C, int BS>
 __global__ void k_moments_cmat(Grid g, const double* __restrict__ f, double* __restrict__ props,
                                double* __restrict__ cmat, double mass, int ncell);

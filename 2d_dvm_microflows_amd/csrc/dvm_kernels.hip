@@ -676,12 +676,17 @@ __device__ __forceinline__ void eq_values(const Grid& g, int l0, double coeff, d
 //           where every barrier of a chunk is on the critical path) the
 //           equilibrium is built by a pass of its own with no loads, and the
 //           |f - f_eq| pass reads it back.
+//   eq_mode (kFused only): 1 -- f_eq is stored on a rebuild and read back on
+//           a cache hit; 0 -- f_eq is not stored (no collision kernel reads
+//           it) and a cache hit computes it again from the cached
+//           parameters, bit for bit as the rebuild did; 2 -- computed again
+//           on a hit and stored either way (f_eq current for a reader).
 template <int BS, int V, bool kFused>
 __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, const double* __restrict__ fc,
                                          const double* __restrict__ p, double* __restrict__ ev,
                                          double* __restrict__ ec, int2* __restrict__ tab2, double* __restrict__ par,
                                          int* __restrict__ cnt4, double* __restrict__ fac3, CdfShared<BS, V>& sh,
-                                         float* gsum, unsigned char* __restrict__ sgn) {
+                                         float* gsum, unsigned char* __restrict__ sgn, int eq_mode = 1) {
   constexpr int CH = BS * V;
   int* rights = sh.rights;
   double* tile = sh.tile;
@@ -702,10 +707,17 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   __syncthreads();
   const bool new_eq = sh.s_new != 0;
   __syncthreads();
+  // the fused pass computes the equilibrium on a rebuild, and on a cache hit
+  // unless f_eq is read back (eq_mode 1), then from the cached parameters
+  const bool recompute = kFused && !new_eq && !fmc && eq_mode != 1;
+  const bool fuse = kFused && (new_eq || recompute);
+  const bool store_ev = fuse && eq_mode != 0;
+  const double edens = new_eq ? dens : par[0], eu = new_eq ? u : par[1], evv = new_eq ? v : par[2];
+  const double ew = new_eq ? w : par[3], etemp = new_eq ? temp : par[4];
   double coeff = 0.0;
-  if (new_eq && dens > 0.0) coeff = dens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * temp * temp * temp));
+  if ((new_eq || recompute) && edens > 0.0)
+    coeff = edens * sqrt(mass * mass * mass / (kPiRef * kPiRef * kPiRef * etemp * etemp * etemp));
   const int own = threadIdx.x * (V + 1);
-  const bool fuse = kFused && new_eq;
   // carries: eq_df prefix, eqv prefix, |f - eqv| prefix; last eq_df of the previous chunk
   double cpv = 0.0, cec = 0.0, cnc = 0.0, prev_last = 0.0;
   if (threadIdx.x == 0) {
@@ -730,7 +742,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     }
     build_buckets<BS>(ec, N, tab2, rights);  // its barriers also publish ev
     if (threadIdx.x == 0) {
-      par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
+      par[0] = dens; par[1] = u; par[2] = v; par[3] = w; par[4] = temp;
     }
   }
   // f (and, unless the equilibrium is built in this pass, eqv) of the next
@@ -768,7 +780,7 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     if (fuse) {
       // eq_df = prefix of f_eq*beta^3; eqv(l) = eq_df(l) - eq_df(l-1); eqc = prefix of eqv
       double pv[V];
-      eq_values<V>(g, l0, coeff, u, v, w, temp, mass, pv);
+      eq_values<V>(g, l0, coeff, eu, evv, ew, etemp, mass, pv);
       // eq_df of the node before this lane's first: the previous lane's last
       double before, last;
       cpv = cpv + chunk_scan_prev<BS, V>(pv, sh.red, cpv, prev_last, before, last);
@@ -781,13 +793,18 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
         d[q] = fabs(dv[q]);
       }
       if (sgn) store_sign_bits<V>(dv, l0, N, sgn);
-      tile_store<BS, V>(e, tile, ev, base, N);
-      chunk_scan2<BS, V>(e, d, sh.red, cec, cnc);
-      tile_store<BS, V>(e, tile, ec + 1, base, N);
-      __syncthreads();
+      if (store_ev) tile_store<BS, V>(e, tile, ev, base, N);
+      if (new_eq) {
+        chunk_scan2<BS, V>(e, d, sh.red, cec, cnc);
+        tile_store<BS, V>(e, tile, ec + 1, base, N);
+        __syncthreads();
+      } else {
+        // a cache hit keeps the stored equilibrium CDF
+        cnc = cnc + chunk_scan<BS, V>(d, sh.red[0], cnc);
+      }
       tile_store<BS, V>(d, tile, nc + 1, base, N);
       if (gsum && l0 < N) {
-        gsum[l0 / V] = (float)e[V - 1];
+        if (new_eq) gsum[l0 / V] = (float)e[V - 1];
         gsum[ng + l0 / V] = (float)d[V - 1];
       }
     } else {
@@ -802,11 +819,11 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     }
     __syncthreads();  // the tile's reads end before the next chunk writes it
   }
-  if (fuse) {
+  if (fuse && new_eq) {
     if (gsum) build_buckets_grouped<BS, V>(ec, gsum, N, tab2, rights);
     else build_buckets<BS>(ec, N, tab2, rights);
     if (threadIdx.x == 0) {
-      par[0] = dens; par[1] = u; par[2] = v; par[4] = temp;
+      par[0] = dens; par[1] = u; par[2] = v; par[3] = w; par[4] = temp;
     }
   }
   if (gsum) build_buckets_grouped<BS, V>(nc, gsum + ng, N, tab2 + kNBucket, rights);
@@ -869,7 +886,7 @@ __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* 
                                                 double* __restrict__ cdf, int2* __restrict__ tabs,
                                                 double* __restrict__ eqpar,
                                                 int* __restrict__ counts, double* __restrict__ factors,
-                                                int sum_cap, unsigned char* __restrict__ sgn) {
+                                                int sum_cap, unsigned char* __restrict__ sgn, int eq_mode) {
   __shared__ CdfShared<BS, V> sh;
   extern __shared__ float gdyn[];  // sum_cap floats: the scan's group ends
   const int cell = blockIdx.x;
@@ -877,7 +894,8 @@ __global__ void __launch_bounds__(BS) k_cdf(Grid g, CdfParams cp, const double* 
   float* gsum = (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr;
   cdf_cell<BS, V, kFused>(g, cp, f + (size_t)cell * N, props + (size_t)cell * DVM_NPROP, eqv + (size_t)cell * N,
            cdf + (size_t)cell * 2 * (N + 1), tabs + (size_t)cell * 2 * kNBucket, eqpar + (size_t)cell * 5,
-           counts + cell * 4, factors + cell * 3, sh, gsum, sgn ? sgn + (size_t)cell * 2 * (N + 1) : nullptr);
+           counts + cell * 4, factors + cell * 3, sh, gsum, sgn ? sgn + (size_t)cell * 2 * (N + 1) : nullptr,
+           eq_mode);
 }
 
 // --------------------------------------------------------------- collide
