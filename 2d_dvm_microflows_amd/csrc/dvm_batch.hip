@@ -95,7 +95,7 @@ template <int BS, int V, bool kFused>
 __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ slots,
                                                   const SlotState* __restrict__ state,
                                                   const CellRef* __restrict__ cells, CdfParams base,
-                                                  BatchPools P, int sum_cap) {
+                                                  BatchPools P, int sum_cap, int eq_mode) {
   __shared__ CdfShared<BS, V> sh;
   extern __shared__ float gdyn[];  // sum_cap floats: the scan's group ends
   const CellRef r = cells[blockIdx.x];
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(BS) k_cdf_cells(const SlotDesc* __restrict__ s
   const size_t cs = (size_t)d.coff + (size_t)r.cell * 2 * (N + 1);
   cdf_cell<BS, V, kFused>(d.g, cp, P.f[cur] + n0, P.props + gc * DVM_NPROP, P.eqv + n0, P.cdf + cs,
            P.tabs + gc * 2 * kNBucket, P.eqpar + gc * 5, P.counts + gc * 4, P.factors + gc * 3, sh,
-           (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr, P.sgn + cs);
+           (2 * ((N + V - 1) / V) <= sum_cap) ? gdyn : nullptr, P.sgn + cs, eq_mode);
 }
 
 // fd = f (FP64) or dacc = 0 (FP32) over the cells that take global atomics.
@@ -873,14 +873,20 @@ static int batch_lists(dvm_batch* b) {
 static int batch_enqueue_step(dvm_batch* b) {
   const bool f32 = batch_f32(b);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, b->stream, b->P.queue, kXcds);
+  // f_eq: stored and read back on a cache hit (1) unless the batch may take
+  // the fast FP32 sampling, whose collision never reads it: then computed
+  // again on a hit, and stored (2) only while some item takes the general
+  // sampling, which reads it; neither CDF kernel then reads a stored f_eq,
+  // whichever of them earlier steps used
+  const int eq_mode = !b->fast_ok ? 1 : (b->all_fast ? 0 : 2);
   if (b->n_cells >= 4 * b->cus) {
     const int cap = cdf_sum_cap((int)b->max_n, kCdfV);
     hipLaunchKernelGGL((k_cdf_cells<kCdfBS, kCdfV, true>), dim3(b->n_cells), dim3(kCdfBS), cap * sizeof(float),
-                       b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap);
+                       b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap, eq_mode);
   } else {
     const int cap = cdf_sum_cap((int)b->max_n, kCdfWideV);
     hipLaunchKernelGGL((k_cdf_cells<kCdfWideBS, kCdfWideV, false>), dim3(b->n_cells), dim3(kCdfWideBS), cap * sizeof(float),
-                       b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap);
+                       b->stream, b->d_slots, b->d_state, b->d_cells, b->cdfp, b->P, cap, eq_mode);
   }
   if (b->any_global) {
     const unsigned chunks = (unsigned)((b->max_n + 255) / 256);

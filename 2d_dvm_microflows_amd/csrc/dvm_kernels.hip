@@ -676,11 +676,13 @@ __device__ __forceinline__ void eq_values(const Grid& g, int l0, double coeff, d
 //           where every barrier of a chunk is on the critical path) the
 //           equilibrium is built by a pass of its own with no loads, and the
 //           |f - f_eq| pass reads it back.
-//   eq_mode (kFused only): 1 -- f_eq is stored on a rebuild and read back on
-//           a cache hit; 0 -- f_eq is not stored (no collision kernel reads
-//           it) and a cache hit computes it again from the cached
-//           parameters, bit for bit as the rebuild did; 2 -- computed again
-//           on a hit and stored either way (f_eq current for a reader).
+//   eq_mode: 1 -- f_eq is stored on a rebuild and read back on a cache hit;
+//           0 -- no collision kernel reads f_eq: the fused pass does not
+//           store it, and a cache hit computes it again from the cached
+//           parameters, bit for bit as the rebuild did (the few-cell pass,
+//           whose |f - f_eq| pass reads f_eq back, computes and stores it
+//           on a hit too); 2 -- computed again on a hit and stored either
+//           way (f_eq current for a reader).
 template <int BS, int V, bool kFused>
 __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, const double* __restrict__ fc,
                                          const double* __restrict__ p, double* __restrict__ ev,
@@ -707,9 +709,9 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
   __syncthreads();
   const bool new_eq = sh.s_new != 0;
   __syncthreads();
-  // the fused pass computes the equilibrium on a rebuild, and on a cache hit
-  // unless f_eq is read back (eq_mode 1), then from the cached parameters
-  const bool recompute = kFused && !new_eq && !fmc && eq_mode != 1;
+  // the equilibrium is computed on a rebuild, and on a cache hit unless f_eq
+  // is read back (eq_mode 1), then from the cached parameters
+  const bool recompute = !new_eq && !fmc && eq_mode != 1;
   const bool fuse = kFused && (new_eq || recompute);
   const bool store_ev = fuse && eq_mode != 0;
   const double edens = new_eq ? dens : par[0], eu = new_eq ? u : par[1], evv = new_eq ? v : par[2];
@@ -724,25 +726,32 @@ __device__ __forceinline__ void cdf_cell(const Grid& g, const CdfParams& cp, con
     nc[0] = 0.0;
     if (new_eq) ec[0] = 0.0;
   }
-  if (!kFused && new_eq) {
+  if (!kFused && (new_eq || recompute)) {
     // the equilibrium's own pass: eq_df = prefix of f_eq*beta^3,
-    // eqv(l) = eq_df(l) - eq_df(l-1), eqc = prefix of eqv
+    // eqv(l) = eq_df(l) - eq_df(l-1), eqc = prefix of eqv (the equilibrium
+    // CDF and its table only on a rebuild)
     for (int base = 0; base < N; base += CH) {
       double pv[V], e[V];
-      eq_values<V>(g, base + (int)threadIdx.x * V, coeff, u, v, w, temp, mass, pv);
+      eq_values<V>(g, base + (int)threadIdx.x * V, coeff, eu, evv, ew, etemp, mass, pv);
       double before, last;
       cpv = cpv + chunk_scan_prev<BS, V>(pv, sh.red, cpv, prev_last, before, last);
       prev_last = last;
 #pragma unroll
       for (int q = 0; q < V; ++q) e[q] = pv[q] - (q == 0 ? before : pv[q - 1]);
       tile_store<BS, V>(e, tile, ev, base, N);
-      cec = cec + chunk_scan<BS, V>(e, sh.red[0], cec);
-      tile_store<BS, V>(e, tile, ec + 1, base, N);
+      if (new_eq) {
+        cec = cec + chunk_scan<BS, V>(e, sh.red[0], cec);
+        tile_store<BS, V>(e, tile, ec + 1, base, N);
+      }
       // the next chunk writes the tile only after its first scan's barriers
     }
-    build_buckets<BS>(ec, N, tab2, rights);  // its barriers also publish ev
-    if (threadIdx.x == 0) {
-      par[0] = dens; par[1] = u; par[2] = v; par[3] = w; par[4] = temp;
+    if (new_eq) {
+      build_buckets<BS>(ec, N, tab2, rights);  // its barriers also publish ev
+      if (threadIdx.x == 0) {
+        par[0] = dens; par[1] = u; par[2] = v; par[3] = w; par[4] = temp;
+      }
+    } else {
+      __syncthreads();  // publishes ev (and ends the tile's reads)
     }
   }
   // f (and, unless the equilibrium is built in this pass, eqv) of the next
