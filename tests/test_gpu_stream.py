@@ -85,6 +85,38 @@ def test_philox_words_as_stream_reproduce_default_mode(pkg, precision):
     b.close()
 
 
+def test_stream_step_after_fast_fp32_steps(pkg):
+    """A context with enough cells for the many-cell CDF pass on the fast FP32
+    path stores no f_eq (its collision reads the sign bits); a list-mode step
+    after such steps samples through the general code, which reads f_eq, so
+    that step's CDF pass must compute f_eq again for every cell, cache hits
+    included (the cavity starts at rest, so most cells keep their cached
+    equilibrium). The list step equals the default-mode step to FP32
+    rounding."""
+    cfg, f = pkg.lid_driven_cavity(nx=32, ny=32, n_v=9, vmax=3.0, samples=400)
+    cfg = cfg.with_(precision=1)
+    ncell = cfg.nx * cfg.ny
+    print("[stream] list-mode step after fast FP32 steps, 1024 cells", flush=True)
+    a, b, c = pkg.Solver(cfg), pkg.Solver(cfg), pkg.Solver(cfg)
+    for s in (a, b, c):
+        s.initialize(f)
+    for n in (1, 2):
+        for s in (a, b, c):
+            s.step(n)
+    n = 3
+    # the sample order from a third context in the same state (its traces
+    # store f_eq themselves, which b must not get for free)
+    u = philox_stream(cfg.seed, n, sample_order(c, n, ncell), cfg.num_repl)
+    b.set_uniform_stream(u)
+    a.step(n)
+    b.step(n)
+    assert b.uniform_stream_position() == u.size
+    ok, bad, worst = cellwise_close(b.download_f(), a.download_f(), rtol=1e-9)
+    assert ok, (bad[:8], worst)
+    for s in (a, b, c):
+        s.close()
+
+
 def test_short_stream_fails_before_changing_f(pkg):
     dvm = __import__("importlib").import_module("2d_dvm_microflows_amd.dvm")
     cfg, f = pkg.homogeneous_relaxation(n_v=9, samples=2000)
