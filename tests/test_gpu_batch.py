@@ -73,6 +73,32 @@ def test_instances_match_standalone_in_the_mix(pkg):
             assert cellwise(b.moments(s), pm) <= RTOL
 
 
+def test_fp32_cdf_kernel_switch_keeps_equilibrium(pkg):
+    """A batch on the fast FP32 path stores no f_eq in its many-cell CDF pass.
+    With a 1100-cell 1-D shock resident, the first steps run that pass; once
+    the shock retires, the few-cell pass runs, which reads f_eq back and so
+    must compute it again on a cache hit (the relaxing instance, at zero bulk
+    velocity and with its moments conserved, keeps its cached equilibrium
+    every step). The instance still follows its standalone FP32 trajectory."""
+    relax = pkg.homogeneous_relaxation(n_v=9, samples=2000, seed=111)
+    relax = (relax[0].with_(precision=1), relax[1])
+    c, f = pkg.normal_shock_1d(nx=1100, n_v=7, vmax=6.0, samples=100, omega=0.5, seed=112)
+    shock = (c.with_(precision=1), f)
+    print("[batch] FP32: 1101 cells (many-cell CDF pass), then 1 cell (few-cell pass)", flush=True)
+    with pkg.Batch(relax[0], max_slots=4, node_capacity=400_000, cell_capacity=1200) as b:
+        b.set_tiles(1024, 1, 16384)
+        r = b.admit(*relax)
+        sh = b.admit(*shock)
+        b.step(2)
+        b.synchronize()
+        b.retire(sh)
+        b.step(2)
+        b.synchronize()
+        got = b.download_f(r)
+    want = standalone(pkg, relax[0], relax[1], 4)
+    assert cellwise(got, want) <= 1e-6, cellwise(got, want)
+
+
 def test_admit_and_retire_between_steps(pkg):
     insts = instances(pkg)
     print("[batch] admit / retire between steps", flush=True)
