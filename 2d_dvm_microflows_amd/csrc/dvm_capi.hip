@@ -609,7 +609,8 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
     const long long wgs = (long long)per_cu * c->cus;
     c->fast_wgs = (int)(wgs < c->ncell ? wgs : c->ncell);
     if (int rc2 = dalloc(c, &c->tick, 2)) return rc2;
-    HIP_TRY(c, hipMemset(c->tick, 0, 2 * sizeof(unsigned)));
+    // zeroed in the context's stream order (set-up; never inside a captured step)
+    HIP_TRY(c, hipMemsetAsync(c->tick, 0, 2 * sizeof(unsigned), c->stream));
   }
   // ---- convection constants
   ConvParams& p = c->convp;
