@@ -924,7 +924,7 @@ static int batch_enqueue_step(dvm_batch* b) {
     const double* fc = b->P.f[1] + d.off;
     double* fo = b->P.f[0] + d.off;
     if (p.acc > 0.0)
-      hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, b->stream, d.g, p, fc, h.ratio);
+      hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kWallBS), 0, b->stream, d.g, p, fc, h.ratio);
     if (p.acc < 1.0) {
       hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, b->stream, h.spec, (long long)4 * p.max_along * h.N, 0.0);
       hipLaunchKernelGGL(k_spec_ghost, dim3((h.N + 255) / 256, p.max_along, 4), dim3(256), 0, b->stream, d.g, p, fc,

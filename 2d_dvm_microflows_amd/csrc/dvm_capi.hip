@@ -916,7 +916,7 @@ static int walls_async(dvm_ctx* c, int ntime) {
     return DVM_OK;
   }
   if (p.acc > 0.0)
-    hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kMomBS), 0, c->stream, c->g, p, c->f, c->ratio);
+    hipLaunchKernelGGL(k_wall_ratio, dim3(p.max_along, 4), dim3(kWallBS), 0, c->stream, c->g, p, c->f, c->ratio);
   if (p.acc < 1.0) {
     const long long ns = (long long)4 * p.max_along * N;
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, c->spec, ns, 0.0);

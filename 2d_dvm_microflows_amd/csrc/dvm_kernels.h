@@ -29,6 +29,7 @@ inline int cdf_sum_cap(int max_n, int V) {
 }
 constexpr int kCollBS = 1024;    // collision workgroup: 16 waves share one cell's LDS slice
 constexpr int kConvBS = 256;     // convection lanes per workgroup
+constexpr int kWallBS = 1024;    // wall flux-balance lanes per workgroup (one workgroup per wall cell)
 #ifndef DVM_CONV_STRIP
 #define DVM_CONV_STRIP 4
 #endif

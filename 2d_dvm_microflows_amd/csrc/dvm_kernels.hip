@@ -2053,28 +2053,60 @@ __global__ void k_remap_batch(Grid g, long long n, const double* __restrict__ uv
 // (boundary_conditions.f90:753-829 and the right/bottom/top analogues):
 // ratio = flux_in / flux_out with flux_in = sum over incoming nodes of
 // |v_n| * acc * f. blockIdx.y = side (0 left, 1 right, 2 bottom, 3 top).
-__global__ void __launch_bounds__(kMomBS) k_wall_ratio(Grid g, ConvParams p, const double* __restrict__ f,
-                                                       double* __restrict__ ratio) {
-  __shared__ double red[kMomBS / 64];
+// The lanes run over the incoming nodes only: runs of consecutive i in each
+// (j, k) row for the left and right walls, blocks of whole rows in each k
+// plane for the bottom and top walls; four loads in flight per lane.
+__global__ void __launch_bounds__(kWallBS) k_wall_ratio(Grid g, ConvParams p, const double* __restrict__ f,
+                                                        double* __restrict__ ratio) {
+  __shared__ double red[kWallBS / 64];
   const int side = blockIdx.y, pos = blockIdx.x;
   const int along = (side < 2) ? p.ny : p.nx;
   if (pos >= along || !p.wall_side[side]) return;
   const int ix = (side == 0) ? 0 : (side == 1 ? p.nx - 1 : pos);
   const int iy = (side == 2) ? 0 : (side == 3 ? p.ny - 1 : pos);
   const double* fc = f + (size_t)(ix + p.nx * iy) * g.N;
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  // the incoming range of the wall-normal index: [lo, lo + cnt)
+  const bool xwall = side < 2;
+  const int nn = xwall ? n0 : n1;
+  const int lo = (side == 0 || side == 2) ? 0 : (xwall ? g.pos[0] : g.pos[1]);
+  const int hi = (side == 0 || side == 2) ? (xwall ? g.neg[0] : g.neg[1]) + 1 : nn;
+  const int cnt = hi > lo ? hi - lo : 0;
+  // x walls: t = ii + cnt * r over rows r of (j, k); y walls: t = rem + (n0 cnt) * k
+  const int run = xwall ? cnt : n0 * cnt;
+  const int total = run * (xwall ? n1 * n2 : n2);
+  const double inv_run = run > 0 ? 1.0 / (double)run : 0.0;
+  const double inv_n0 = g.inv_n0;
+  const double* __restrict__ axn = xwall ? g.ax[0] : g.ax[1];
+  constexpr int U = 4;
   double s = 0.0;
-  for (int l = threadIdx.x; l < g.N; l += kMomBS) {
-    int i, j, k;
-    node_ijk(g, l, i, j, k);
-    bool in;
-    double vn;
-    if (side == 0) { in = i <= g.neg[0]; vn = g.ax[0][i]; }
-    else if (side == 1) { in = i >= g.pos[0]; vn = g.ax[0][i]; }
-    else if (side == 2) { in = j <= g.neg[1]; vn = g.ax[1][j]; }
-    else { in = j >= g.pos[1]; vn = g.ax[1][j]; }
-    if (in) s += fabs(vn) * p.acc * fc[l];
+  for (int t0 = threadIdx.x; t0 < total; t0 += U * kWallBS) {
+    double fv[U], vn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u * kWallBS;
+      fv[u] = 0.0;
+      vn[u] = 0.0;
+      if (t < total) {
+        const int r = (int)(((double)t + 0.5) * inv_run);
+        const int q = t - r * run;
+        int l, w;
+        if (xwall) {
+          w = lo + q;
+          l = w + n0 * r;
+        } else {
+          const int jj = (int)(((double)q + 0.5) * inv_n0);
+          w = lo + jj;
+          l = r * g.nxy + lo * n0 + q;
+        }
+        fv[u] = fc[l];
+        vn[u] = axn[w];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += fabs(vn[u]) * p.acc * fv[u];
   }
-  s = block_sum<kMomBS>(s, red);
+  s = block_sum<kWallBS>(s, red);
   if (threadIdx.x == 0) ratio[side * p.max_along + pos] = s / p.flux_out[side];
 }
 
