@@ -69,13 +69,15 @@ def test_fp32_collision_matches_fp64_distribution(pkg, case):
     assert d <= 1e-5 * change, (d, change)
 
 
-def test_fp32_persistent_cells_match_fp64(pkg):
+@pytest.mark.parametrize("n_v", [9, 10])
+def test_fp32_persistent_cells_match_fp64(pkg, n_v):
     """A cavity with enough cells for one workgroup per cell (576 cells, so
     the FP32 collision runs whole cells on persistent workgroups with the
     moments pass's conservation factor): one collision against the FP64 one
     from the same f and seed, node by node, and mass, momentum and energy of
-    every cell conserved to round-off."""
-    cfg, f = pkg.lid_driven_cavity(nx=24, ny=24, n_v=9, vmax=4.0, samples=2000)
+    every cell conserved to round-off; 10^3 nodes put every other cell's sign
+    words 2 bytes off a 4-byte boundary."""
+    cfg, f = pkg.lid_driven_cavity(nx=24, ny=24, n_v=n_v, vmax=4.0, samples=2000)
     print(f"[fp32 persistent cells] {cfg.name}", flush=True)
     v = pkg.node_velocities(cfg)
     out = {}
@@ -88,7 +90,11 @@ def test_fp32_persistent_cells_match_fp64(pkg):
     change = np.abs(out[0] - f).sum() / np.abs(out[0]).sum()
     print(f"  L1 difference {d:.2e} of a change of {change:.2e}", flush=True)
     assert change > 1e-6, change
-    assert d <= 1e-5 * change, (d, change)
+    # near equilibrium with 2000 samples per cell the net change is a small
+    # difference of depletions and replenishments that are each rounded in
+    # FP32 (measured 4e-6 and 1e-5 of it); a misplaced scatter or sign would
+    # be of the order of the change itself
+    assert d <= 1e-4 * change, (d, change)
     a, b = cons(f, v), cons(out[FP32], v)
     scale = np.abs(a) + cons(np.abs(out[FP32]), np.abs(v))
     assert np.all(np.abs(b - a) <= 1e-13 * scale), np.max(np.abs(b - a) / scale)
