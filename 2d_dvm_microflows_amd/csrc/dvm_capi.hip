@@ -58,6 +58,8 @@ struct dvm_ctx {
   int fast = 0;                  // the FP32 collision takes collide_fast (see collide_async)
   unsigned* tick = nullptr;      // [2] ticket and finish counters of k_collide_fast_cells (zero between launches)
   int fast_wgs = 0;              // workgroups of k_collide_fast_cells (0: not used)
+  size_t fast_lds = 0;           // its dynamic LDS bytes
+  bool fast_tab_lds = false;     // the bucket tables staged in LDS beside the sign words
   bool cmat_valid = false;       // cmat was computed from the current f (by moments_async)
   // explicit uniform list (dvm_set_uniform_stream): su_n values, read from
   // su_cursor on; sbase [C] per-cell offsets of the current collision
@@ -601,10 +603,15 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   // lanes and one pick per lane. Its ticket counters start at zero here and
   // the kernel leaves them at zero.
   if (c->fast && c->lds_collide && c->split == 1 && c->coll_bs == kCollBS && c->pick_k == 1) {
-    const void* pk = (const void*)k_collide_fast_cells<kCollBS>;
-    HIP_TRY(c, hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->coll_lds));
+    // the cell's bucket tables join the sign words in LDS when they fit
+    const size_t tab_lds = ((c->coll_lds + 7) & ~(size_t)7) + 2 * kNBucket * sizeof(int2);
+    c->fast_tab_lds = tab_lds <= kLdsBytes - kStaticLds;
+    c->fast_lds = c->fast_tab_lds ? tab_lds : c->coll_lds;
+    const void* pk = c->fast_tab_lds ? (const void*)k_collide_fast_cells<kCollBS, true>
+                                     : (const void*)k_collide_fast_cells<kCollBS, false>;
+    HIP_TRY(c, hipFuncSetAttribute(pk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->fast_lds));
     int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk, kCollBS, c->coll_lds) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk, kCollBS, c->fast_lds) != hipSuccess || per_cu < 1)
       per_cu = 1;
     const long long wgs = (long long)per_cu * c->cus;
     c->fast_wgs = (int)(wgs < c->ncell ? wgs : c->ncell);
@@ -873,9 +880,14 @@ static int collide_async(dvm_ctx* c, int ntime) {
                                c->coll_lds, c->stream));
   } else if (fast && cm && c->fast_wgs > 0) {
     // whole cells on persistent workgroups (the factor is that of this f)
-    hipLaunchKernelGGL(k_collide_fast_cells<kCollBS>, dim3((unsigned)c->fast_wgs), dim3(kCollBS), c->coll_lds,
-                       c->stream, c->g, c->collp, ntime, c->ncell, c->f, c->fd, c->eqv, c->cdf, c->tabs, c->counts,
-                       c->factors, c->err, c->sgn, cm, c->tick);
+    if (c->fast_tab_lds)
+      hipLaunchKernelGGL((k_collide_fast_cells<kCollBS, true>), dim3((unsigned)c->fast_wgs), dim3(kCollBS),
+                         c->fast_lds, c->stream, c->g, c->collp, ntime, c->ncell, c->f, c->fd, c->eqv, c->cdf,
+                         c->tabs, c->counts, c->factors, c->err, c->sgn, cm, c->tick);
+    else
+      hipLaunchKernelGGL((k_collide_fast_cells<kCollBS, false>), dim3((unsigned)c->fast_wgs), dim3(kCollBS),
+                         c->fast_lds, c->stream, c->g, c->collp, ntime, c->ncell, c->f, c->fd, c->eqv, c->cdf,
+                         c->tabs, c->counts, c->factors, c->err, c->sgn, cm, c->tick);
     HIP_TRY(c, hipGetLastError());
   } else {
     HIP_TRY(c, hipLaunchKernel(collide_kernel(c), dim3((unsigned)c->ncell * c->split), dim3(c->coll_bs), args,

@@ -1898,8 +1898,10 @@ __global__ void __launch_bounds__(BS) k_collide(Grid g, CollParams cp, int ntime
 // while the correction runs, and the slice is cleared as the corrected
 // values are written. tick[0] is the ticket counter and tick[1] counts the
 // workgroups that have finished; the last one resets both to zero, so every
-// launch starts from zero without a memset.
-template <int BS>
+// launch starts from zero without a memset. With kTabLds the cell's two
+// bucket tables (2 x 1000 brackets, 16 KB) are staged in LDS after the sign
+// words, with them, so that each partner search starts from an LDS read.
+template <int BS, bool kTabLds>
 __global__ void __launch_bounds__(BS) k_collide_fast_cells(Grid g, CollParams cp, int ntime, int ncell,
                                                            const double* __restrict__ f, double* __restrict__ fd,
                                                            const double* __restrict__ eqv,
@@ -1921,6 +1923,9 @@ __global__ void __launch_bounds__(BS) k_collide_fast_cells(Grid g, CollParams cp
   const auto words = [&](int c) {
     return reinterpret_cast<const unsigned*>(sgn + (size_t)c * 2 * (N + 1));
   };
+  // the bucket tables after the sign words, 8-byte aligned
+  int2* ltab = reinterpret_cast<int2*>(lw + ((nw + 1) & ~1));
+  constexpr int kTabPer = (2 * kNBucket + BS - 1) / BS;  // brackets staged per lane
   if (threadIdx.x == 0) s_cell = (int)atomicAdd(tick, 1u);
   __syncthreads();
   int cell = s_cell;
@@ -1928,12 +1933,18 @@ __global__ void __launch_bounds__(BS) k_collide_fast_cells(Grid g, CollParams cp
     const unsigned w0 = ((int)threadIdx.x < nw) ? words(cell)[threadIdx.x] : 0u;
     for (int q = threadIdx.x; q < N; q += BS) acc[q] = 0.0;
     if ((int)threadIdx.x < nw) lw[threadIdx.x] = w0;
+    if constexpr (kTabLds)
+      for (int r = threadIdx.x; r < 2 * kNBucket; r += BS) ltab[r] = tabs[(size_t)cell * 2 * kNBucket + r];
   }
   __syncthreads();
   int bad = 0;
   while (cell < ncell) {
-    const CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
-                                  tabs + (size_t)cell * 2 * kNBucket, N);
+    CellView cv = cell_view(f + (size_t)cell * N, eqv + (size_t)cell * N, cdf + (size_t)cell * 2 * (N + 1),
+                            tabs + (size_t)cell * 2 * kNBucket, N);
+    if constexpr (kTabLds) {
+      cv.eqtab = ltab;
+      cv.neqtab = ltab + kNBucket;
+    }
     const int* cnt = counts + cell * 4;
     const int total = cnt[0] + cnt[1] + cnt[2];
     bad |= collide_fast<BS>(g, ax, cp, cv, reinterpret_cast<const unsigned char*>(lw), cell, ntime, 0, total, cnt,
@@ -1943,14 +1954,32 @@ __global__ void __launch_bounds__(BS) k_collide_fast_cells(Grid g, CollParams cp
     __syncthreads();  // the change is complete; the sign words are no longer read
     int next = ncell;
     unsigned wn = 0u;
+    int2 tn[kTabPer];
     bad |= conserve_cell_factored<BS, true>(g, cv.f, acc, cmat + (size_t)cell * kCmat, fd + (size_t)cell * N, csh,
                                             [&] {
                                               next = s_cell;
                                               if (next < ncell && (int)threadIdx.x < nw)
                                                 wn = words(next)[threadIdx.x];
+                                              if constexpr (kTabLds) {
+#pragma unroll
+                                                for (int k = 0; k < kTabPer; ++k) {
+                                                  const int r = (int)threadIdx.x + k * BS;
+                                                  tn[k] = (next < ncell && r < 2 * kNBucket)
+                                                              ? tabs[(size_t)next * 2 * kNBucket + r]
+                                                              : make_int2(0, 0);
+                                                }
+                                              }
                                             });
     if ((int)threadIdx.x < nw) lw[threadIdx.x] = wn;
-    __syncthreads();  // the slice is cleared and the next cell's words staged
+    if constexpr (kTabLds) {
+      // the current cell's searches ended at the barrier before the correction
+#pragma unroll
+      for (int k = 0; k < kTabPer; ++k) {
+        const int r = (int)threadIdx.x + k * BS;
+        if (r < 2 * kNBucket) ltab[r] = tn[k];
+      }
+    }
+    __syncthreads();  // the slice is cleared and the next cell's words and brackets staged
     cell = next;
   }
   if (bad) atomicOr(err, bad);
