@@ -605,7 +605,10 @@ static int create_impl(const dvm_config* cfg, dvm_ctx* c) {
   if (c->fast && c->lds_collide && c->split == 1 && c->coll_bs == kCollBS && c->pick_k == 1) {
     // the cell's bucket tables join the sign words in LDS when they fit
     const size_t tab_lds = ((c->coll_lds + 7) & ~(size_t)7) + 2 * kNBucket * sizeof(int2);
-    c->fast_tab_lds = tab_lds <= kLdsBytes - kStaticLds;
+    // DVM_FAST_LDS_TABLES=0 keeps them in global memory (a test compares
+    // the two)
+    const char* lt = getenv("DVM_FAST_LDS_TABLES");
+    c->fast_tab_lds = tab_lds <= kLdsBytes - kStaticLds && !(lt && atoi(lt) == 0);
     c->fast_lds = c->fast_tab_lds ? tab_lds : c->coll_lds;
     const void* pk = c->fast_tab_lds ? (const void*)k_collide_fast_cells<kCollBS, true>
                                      : (const void*)k_collide_fast_cells<kCollBS, false>;

@@ -154,3 +154,28 @@ def test_fp32_trajectory_tracks_fp64(pkg):
         err = _moment_err(a.calculate_physical_properties(), b.calculate_physical_properties())
         print(f"  step {n}: max moment rel err {err:.2e}", flush=True)
         assert err <= 1e-5, (n, err)
+
+
+def test_fp32_lds_brackets_pick_the_same_nodes(pkg, monkeypatch):
+    """The persistent FP32 collision stages each cell's bucket brackets in
+    LDS beside the sign words. Four full steps of a 32x32 cavity on the 25^3
+    grid with the tables in LDS against the same steps with the global tables
+    (DVM_FAST_LDS_TABLES=0): equal up to the summation order of the LDS
+    atomics, while a single different pick (a bracket staged from the wrong
+    cell or entry) would move its nodes by the sample's weight, many orders
+    above that."""
+    cfg, f = pkg.lid_driven_cavity(nx=32, ny=32, n_v=25, vmax=5.0, samples=2000)
+    print(f"[fp32 lds brackets] {cfg.name}", flush=True)
+    out = {}
+    for lds in ("1", "0"):
+        monkeypatch.setenv("DVM_FAST_LDS_TABLES", lds)
+        with pkg.Solver(cfg.with_(precision=FP32)) as s:
+            s.initialize(f)
+            s.solve(4)
+            out[lds] = s.download_f()
+    scale = np.abs(out["0"]).max()
+    d = np.abs(out["1"] - out["0"]).max() / scale
+    change = np.abs(out["0"] - f).max() / scale
+    print(f"  max difference {d:.2e} of max f, change {change:.2e}", flush=True)
+    assert change > 1e-5, change
+    assert d <= 1e-12, d
